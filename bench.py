@@ -1,0 +1,207 @@
+"""Benchmark: VI iterations/s (and test points/s) of LocalKNearestKDE on one
+MI355X, with the E-step kernel's roofline and the CPU oracle beside it.
+
+    python bench.py [--steps K] [--warmup W] [--workload mix32_d6] [--block-size 512]
+
+A step is one VI iteration: one sweep of partial steps over all kernel blocks
+of the workload (lakde/kdes.py:64-67), with the data already resident in HBM.
+Prints one JSON line.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mix32_d6")
+    ap.add_argument("--block-size", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+class KernelTimer:
+    """HIP-event timing of one kernel launch site on the stream it runs on."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.pairs = []
+
+    def __call__(self, fn):
+        ev0 = self.torch.cuda.Event(enable_timing=True)
+        ev1 = self.torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        fn()
+        ev1.record()
+        self.pairs.append((ev0, ev1))
+
+    def mean_ms(self):
+        self.torch.cuda.synchronize()
+        ts = [a.elapsed_time(b) for a, b in self.pairs]
+        return sum(ts) / len(ts), len(ts)
+
+
+def cpu_baseline(X, graph, block_size, threshold, seconds):
+    """Time the oracle's partial step on the full workload (bounded: as many
+    partial steps as fit in ``seconds``) and extrapolate to a sweep."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from oracle import lakde_np as O
+
+    with threadpool_limits(limits=1):
+        m = O.LocalKNearestKDE(k_or_knn_graph=graph, block_size=block_size)
+        m.init_parameters_(X, threshold)
+        # state as after initialisation, without the O(N^2) init passes:
+        # normalisers from one LSE pass of block 0 only (values do not change
+        # the work of a partial step)
+        N = X.shape[0]
+        m.rnm_log_consts = np.zeros(N, np.float32)
+        m.nu[:] = m.nu_0 + 1
+        t0 = time.perf_counter()
+        steps = 0
+        nb = (N - 1) // m.block_size + 1
+        while True:
+            m.partial_step_(X, steps % nb, sparsity_threshold=threshold)
+            steps += 1
+            if time.perf_counter() - t0 > seconds:
+                break
+        dt = (time.perf_counter() - t0) / steps
+    return {"value": 1.0 / (dt * nb), "unit": "VI iters/sec", "cores": 1, "kind": "port",
+            "sample": f"{steps} partial steps of the full workload (block size {block_size}), "
+                      f"{dt:.3f} s each, extrapolated to {nb} steps per VI iteration"}
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(no multi-GPU path is built)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from lakde_amd import LocalKNearestKDE, synthetic
+    from lakde_amd import _fused as F
+    from lakde_amd import _lib
+    from lakde_amd.utils import compute_knn_graph
+
+    w = synthetic.WORKLOADS[args.workload]
+    Xh, Yh = synthetic.make(args.workload)
+    N, D = Xh.shape
+    M = Yh.shape[0]
+    dev = torch.device("cuda:0")
+    X = torch.from_numpy(Xh).to(dev)
+    Y = torch.from_numpy(Yh).to(dev)
+    t = w["threshold"]
+    print(f"# workload {args.workload}: N={N} M={M} D={D} k={w['k']} threshold={t:.6g} "
+          f"bs={args.block_size}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    graph = compute_knn_graph(X, w["k"], 1024, include_self=False)
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+
+    model = LocalKNearestKDE(k_or_knn_graph=graph, block_size=args.block_size)
+    t0 = time.perf_counter()
+    model.init_parameters_(X, t)
+    model.init_summaries_(X, t)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    print(f"# knn graph {t_graph:.2f}s, init {t_init:.2f}s, nnz {model.arena.total_nnz()}",
+          file=sys.stderr, flush=True)
+    nblocks = (N - 1) // model.block_size + 1
+
+    def sweep():
+        for b in range(nblocks):
+            model.partial_step_(X, b, sparsity_threshold=t)
+
+    for i in range(args.warmup):
+        sweep()
+        print(f"# warmup sweep {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+
+    timer = KernelTimer()
+    model._estep_timer = timer
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sweep()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    model._estep_timer = None
+    ms_step = elapsed / args.steps * 1e3
+    k_ms, k_n = timer.mean_ms()
+    print(f"# timed {args.steps} sweeps: {ms_step:.1f} ms/sweep; E-step main kernel "
+          f"{k_ms:.4f} ms avg over {k_n} launches", file=sys.stderr, flush=True)
+
+    # predictive density
+    lp = model.log_pred_density(X, Y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        lp = model.log_pred_density(X, Y)
+    torch.cuda.synchronize()
+    t_pred = (time.perf_counter() - t0) / reps
+    mean_lp = float(lp.mean())
+
+    # roofline of the E-step main kernel (UPDATE mode): algorithmic work is
+    # D(D+1) FLOP per (kernel, point) pair, nb * N pairs per launch
+    bs = model.block_size
+    pairs_per_launch = (N * N) / nblocks
+    flop = pairs_per_launch * D * (D + 1)
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": None, "kernel": f"estep_main_kernel<{D},UPDATE>",
+                "kernel_ms": round(k_ms, 5), "pairs_per_launch": int(pairs_per_launch)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        g = graph.cpu().numpy()
+        cpu = cpu_baseline(Xh, g, bs, t, args.cpu_seconds)
+
+    out = {
+        "metric": "VI iters/sec & test pts/sec, BF16+refine vs FP32 at <=1e-3 nats/pt vs FP64",
+        "value": round(1e3 / ms_step, 4),
+        "unit": "VI iters/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: LocalKNearestKDE fit, N={N}, D={D}, "
+                               f"k={w['k']}, threshold=1/{round(1 / t) if t else 0}, "
+                               f"block_size={bs}, FP32",
+                   "N": N, "M": M, "D": D, "k": w["k"], "block_size": bs},
+        "test_pts_per_sec": round(M / t_pred, 1),
+        "predict_ms": round(t_pred * 1e3, 3),
+        "mean_log_density": round(mean_lp, 6),
+        "init_s": round(t_init, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/*
+ * lakde_amd -- C ABI of the MI355X (gfx950) kernels behind lakde's hot path.
+ *
+ * Plain pointers, sizes and strides; no framework types.  Every entry point
+ * enqueues work on `stream` (a hipStream_t; NULL = the null stream), never
+ * synchronises the host, never allocates device memory, and returns a status
+ * code (LAKDE_OK on success).  lakde_last_error() gives a message for the
+ * most recent failure on the calling thread.
+ *
+ * Device pointers are expected for every array argument.  Arguments are
+ * validated on the host (sizes, dtype codes, null pointers); values inside
+ * device arrays (indices in range, sorted rows) are the caller's contract,
+ * exactly as for the reference CUDA ops.
+ *
+ * Reference interfaces replaced (reference repository paths):
+ *   lakde_ind2ptr        <- torch.ops.lakde.ind2ptr_cuda
+ *                           csrc/cuda/convert_cuda.h:27, lakde/kernels.py:93-100
+ *   lakde_w_inv          <- torch.ops.lakde.W_inv_cuda
+ *                           csrc/cuda/W_inv_cuda.h:5-7, lakde/kernels.py:102-113
+ *   lakde_chol_quad_form <- torch.ops.lakde.chol_quad_form_cuda
+ *                           csrc/cuda/chol_quad_form.h:5-6, lakde/kernels.py:115-120
+ * The fused entry points below have no single reference op; each names the
+ * reference Python code whose device work it performs.
+ */
+#ifndef LAKDE_AMD_H
+#define LAKDE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *lakde_stream_t; /* hipStream_t */
+
+/* status codes */
+#define LAKDE_OK 0
+#define LAKDE_EINVAL 1      /* bad argument (size, stride, null pointer) */
+#define LAKDE_EDTYPE 2      /* unsupported dtype combination */
+#define LAKDE_EHIP 3        /* a HIP runtime call failed */
+
+/* dtype codes */
+#define LAKDE_I32 0
+#define LAKDE_I64 1
+#define LAKDE_F32 2
+#define LAKDE_F64 3
+
+/* E-step modes (lakde_estep) */
+#define LAKDE_ESTEP_ACCUM 0  /* c[m] = logaddexp(c[m], LSE_n log rho_nm)         */
+#define LAKDE_ESTEP_FIXED 1  /* responsibilities with c fixed (no renormalise)   */
+#define LAKDE_ESTEP_UPDATE 2 /* incremental renormalise, then responsibilities   */
+
+/* prior modes (lakde_posterior) */
+#define LAKDE_PRIOR_KNN 0    /* per-point kNN covariance, factored with jitter   */
+#define LAKDE_PRIOR_SHARED 1 /* one shared D x D prior covariance                */
+
+int lakde_version(void);
+const char *lakde_last_error(void);
+
+/* ------------------------------------------------------------------ *
+ * Reference ops                                                       *
+ * ------------------------------------------------------------------ */
+
+/* rowptr[i] = #{e < nnz : ind[e] < i} for i = 0..M, ind sorted ascending.
+ * ind: int32/int64 (ind_dtype); out: M+1 entries of out_dtype (int32/int64).
+ * Replaces ind2ptr_cuda (csrc/cuda/convert_cuda.h:27; lakde/kernels.py:93-100). */
+int lakde_ind2ptr(const void *ind, int ind_dtype, int64_t nnz, int64_t M,
+                  void *out, int out_dtype, lakde_stream_t stream);
+
+/* out[n] = sum_{e in [rowptr[n], rowptr[n+1])} value[e] d_e d_e^T,
+ * d_e = Xm[col[e]] - Xn[n]; out is N x D x D contiguous, both triangles.
+ * rowptr (N+1) and col are int32/int64; value, Xn (N x D), Xm (M x D) and out
+ * are fp_dtype (F32/F64), all contiguous.
+ * Replaces W_inv_cuda (csrc/cuda/W_inv_cuda.h:5-7; lakde/kernels.py:102-113). */
+int lakde_w_inv(const void *rowptr, int rowptr_dtype, const void *col, int col_dtype,
+                const void *value, const void *Xn, const void *Xm, int64_t N, int64_t M,
+                int64_t D, int fp_dtype, void *out, lakde_stream_t stream);
+
+/* out[n, m] = || W[n]^T (Xm[m] - Xn[n]) ||^2 with W[n] upper triangular
+ * (entries below the diagonal are ignored).  Strides are in elements and may
+ * be zero or negative-free arbitrary (stride-0 broadcasts are allowed).
+ * fp_dtype F32/F64.  Replaces chol_quad_form_cuda (csrc/cuda/chol_quad_form.h:5-6;
+ * lakde/kernels.py:115-120). */
+int lakde_chol_quad_form(const void *W, int64_t sW0, int64_t sW1, int64_t sW2,
+                         const void *Xn, int64_t sXn0, int64_t sXn1,
+                         const void *Xm, int64_t sXm0, int64_t sXm1,
+                         void *out, int64_t so0, int64_t so1,
+                         int64_t N, int64_t M, int64_t D, int fp_dtype,
+                         lakde_stream_t stream);
+
+/* ------------------------------------------------------------------ *
+ * Fused hot path (float32)                                            *
+ * ------------------------------------------------------------------ */
+
+/* Floats per packed kernel record: L^{-1} (lower, packed row-major),
+ * the centre x_n, and four per-kernel constants; padded to a multiple of 4. */
+int64_t lakde_record_floats(int64_t D);
+
+/* kNN prior covariance of rows [row0, row0+nb):
+ * out[n] = (sum_k (X[g[n,k]] - X[row0+n]) (...)^T) / K.
+ * graph: nb x K (int32/int64), rows already offset to the block.
+ * Performs batched_knn_cov (lakde/utils.py:40-65, self_center=True). */
+int lakde_knn_prior_cov(const float *X, int64_t N, int64_t D, int64_t row0, int64_t nb,
+                        const void *graph, int graph_dtype, int64_t K, float *out,
+                        lakde_stream_t stream);
+
+/* S[n] as lakde_w_inv on one block's CSR plus nu[n] = nu0 + sum_e value[e]
+ * (lakde/local_adaptive_kde.py:66-81). rowptr/col int32/int64. */
+int lakde_posterior_contribs(const void *rowptr, int rowptr_dtype, const void *col,
+                             int col_dtype, const float *value, const float *Xn,
+                             const float *Xm, int64_t nb, int64_t M, int64_t D, float nu0,
+                             float *S, float *nu, lakde_stream_t stream);
+
+/* Batched posterior factorisation (lakde/local_adaptive_kde.py:83-96,
+ * lakde/linalg_util.py:30-37,49-52,109-170, lakde/local_knn_kde.py:195-207,220-222,
+ * lakde/local_hierarchical_kde.py:269-271,289-291): one wavefront per kernel.
+ *   kNN prior:   P_n = chol(prior[n] + j0_n I) with persistent jitter j0;
+ *                prior_j = prior[n] + j0_n I; T = P_n.
+ *   shared:      prior_j = prior (one D x D matrix); T = prior.
+ *   W_inv = prior_j * (nu0 * a_n / b_n) + S[n];  L = chol(W_inv + j_n I), with
+ *   the reference's jitter schedule decided in FP32; jittered factors are
+ *   refactored and inverted in FP64.  Optionally the tau update
+ *   b_out = b0 + nu0 * trace / 2 (lakde/local_knn_kde.py:236,
+ *   lakde/local_hierarchical_kde.py:290) after E[tau] has been read.  Outputs: Linv = L^{-1} (lower; the
+ *   reference's W_triu is Linv^T), logdet_W (FP64-accumulated), trace =
+ *   nu_n * ||T^T Linv^T||_F^2, records for the E-step / predictive kernels.
+ * *status (device int) is set to 1 when a stabilisation does not converge
+ * within 32 doublings (the reference raises RuntimeError). */
+typedef struct {
+  int64_t nb, D;
+  int prior_mode;
+  const float *prior;     /* kNN: nb x D x D; shared: D x D */
+  float *prior_jitter;    /* kNN: nb, in/out; shared: NULL */
+  const float *S;         /* nb x D x D */
+  const float *nu;        /* nb */
+  const float *a, *b;     /* nb (E[tau] = a / b) */
+  float nu0;
+  float *W_jitter;        /* nb, in/out */
+  const float *Xn;        /* nb x D */
+  float *Linv;            /* nb x D x D out */
+  float *prior_tril;      /* kNN: nb x D x D out; shared: NULL */
+  float *logdet_W;        /* nb out */
+  float *logdet_prior;    /* kNN: nb out; shared: NULL */
+  float *trace;           /* nb out */
+  float *rec;             /* nb x lakde_record_floats(D) out */
+  int *status;            /* device int */
+  float b0;               /* with b_out: b_out[n] = b0 + nu0 * trace[n] / 2 */
+  float *b_out;           /* nb out (may alias b), or NULL */
+} lakde_posterior_args;
+
+int lakde_posterior(const lakde_posterior_args *args, lakde_stream_t stream);
+
+/* Column sums of one block's responsibilities: sums[m] = sum_{e: col[e]=m} val[e]
+ * (sums is cleared first).  (lakde/local_adaptive_kde.py:120) */
+int lakde_colsum(const int32_t *col, const float *val, const int32_t *nnz_dev, int64_t N,
+                 float *sums, lakde_stream_t stream);
+
+/* Fused E-step of one kernel block over all N points
+ * (lakde/local_adaptive_kde.py:36-44,98-153, lakde/local_knn_kde.py:157-166):
+ * quadratic forms from the records, log rho epilogue, diagonal exclusion,
+ * streaming log-sum-exp over the block's kernels, incremental renormalisation
+ * (UPDATE), exp / threshold and compaction to canonical CSR (row ascending,
+ * column ascending).  Responsibilities r > threshold are kept (threshold 0:
+ * r != 0).  Output arrays hold `cap` entries; *nnz_dev receives the true
+ * count and nothing is written past cap (the caller grows and re-runs
+ * lakde_estep_compact).  Workspace: lakde_estep_workspace() bytes. */
+typedef struct {
+  int64_t N, D, nb, row0;
+  const float *X;          /* N x D */
+  const float *rec;        /* nb records */
+  int mode;
+  float threshold;
+  const float *c_old;      /* N (UPDATE: previous log normalisers) */
+  const float *old_colsum; /* N (UPDATE: column sums of the block's old values) */
+  float *c;                /* N: ACCUM in/out, FIXED in, UPDATE out */
+  int32_t *rowptr;         /* nb + 1 out */
+  int32_t *out_row, *out_col;
+  float *out_val;
+  int64_t cap;
+  int32_t *nnz_dev;        /* device int out */
+  void *workspace;
+  size_t workspace_bytes;
+} lakde_estep_args;
+
+size_t lakde_estep_workspace(int64_t N, int64_t nb);
+int lakde_estep(const lakde_estep_args *args, lakde_stream_t stream);
+/* lakde_estep in two halves: the quad-form / LSE / ballot kernel, then the
+ * row scans and the compaction (lets a caller time the main kernel alone). */
+int lakde_estep_main(const lakde_estep_args *args, lakde_stream_t stream);
+int lakde_estep_finish(const lakde_estep_args *args, lakde_stream_t stream);
+/* Re-run only the compaction of the last lakde_estep (same args, larger cap). */
+int lakde_estep_compact(const lakde_estep_args *args, lakde_stream_t stream);
+
+/* rebalance_responsibilities_ (lakde/local_adaptive_kde.py:156-181) over a
+ * block arena: block b's entries live at [b*cap, b*cap + nnz[b]).  Blocks other
+ * than `exclude_block` are scaled by clamp(exp(c_prev - c), tiny, max)[col];
+ * column sums s are clamped to eps; c += log s; every value is scaled by
+ * clamp(1/s, tiny, max)[col].  c_prev may alias c.  ws: 2*N floats. */
+int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_block,
+                    const int32_t *nnz, const int32_t *cols, float *vals,
+                    const float *c_prev, float *c, float *ws, lakde_stream_t stream);
+
+/* Streaming predictive density (lakde/local_adaptive_kde.py:200-223):
+ * out[m] = -log N + LSE_n [coeff_n - (nu_n+1)/2 log1p(Q_nm)] over all N
+ * kernel records.  Workspace: lakde_pred_workspace() bytes. */
+size_t lakde_pred_workspace(int64_t N, int64_t M);
+int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, int64_t D,
+                       float *out, void *workspace, size_t workspace_bytes,
+                       lakde_stream_t stream);
+
+/* fill helper (device memset replacement that is safe under graph capture) */
+int lakde_fill_f32(float *p, int64_t n, float v, lakde_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAKDE_AMD_H */

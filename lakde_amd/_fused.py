@@ -1,0 +1,150 @@
+"""Thin wrappers of the fused entry points and the block arena that holds the
+responsibilities on the device."""
+
+import math
+
+import torch
+
+from . import _lib
+from ._lib import EstepArgs, PosteriorArgs, check, idx_code, ptr, stream_of
+
+
+def record_floats(D):
+    return int(_lib.load().lakde_record_floats(D))
+
+
+class Arena:
+    """Responsibilities r_nm of all kernel blocks, in one device allocation.
+
+    Block b keeps its entries at [b*cap, b*cap + nnz[b]) in canonical CSR
+    order: rows local to the block (ascending), global columns ascending
+    within a row.  This is the data of the reference's per-block
+    ``rnm_blocks[b] = (indices (2, nnz) int32, values)``
+    (lakde/local_adaptive_kde.py:145-153) with the row pointer kept alongside.
+    ``nnz_dev`` carries one extra slot used as the posterior status flag.
+    """
+
+    def __init__(self, num_blocks, block_size, cap, device):
+        self.num_blocks = num_blocks
+        self.block_size = block_size
+        self.cap = int(cap)
+        self.device = device
+        i32 = torch.int32
+        self.row = torch.zeros(num_blocks, self.cap, dtype=i32, device=device)
+        self.col = torch.zeros(num_blocks, self.cap, dtype=i32, device=device)
+        self.val = torch.zeros(num_blocks, self.cap, dtype=torch.float32, device=device)
+        self.rowptr = torch.zeros(num_blocks, block_size + 1, dtype=i32, device=device)
+        self.nnz_dev = torch.zeros(num_blocks + 1, dtype=i32, device=device)
+        self.nnz = [0] * num_blocks
+
+    @property
+    def status_ptr(self):
+        return self.nnz_dev[self.num_blocks:]
+
+    def grow(self, need):
+        new_cap = max(int(need * 1.25), int(self.cap * 1.5)) + 255
+        new_cap = new_cap // 256 * 256
+        for name in ("row", "col", "val"):
+            old = getattr(self, name)
+            new = torch.zeros(self.num_blocks, new_cap, dtype=old.dtype, device=old.device)
+            new[:, :self.cap] = old
+            setattr(self, name, new)
+        self.cap = new_cap
+
+    def block(self, b):
+        """(indices (2, nnz) int32, values) views of block b."""
+        n = self.nnz[b]
+        return torch.stack([self.row[b, :n], self.col[b, :n]]), self.val[b, :n]
+
+    def total_nnz(self):
+        return sum(self.nnz)
+
+
+def knn_prior_cov(X, row0, graph_blk, out):
+    N, D = X.shape
+    nb, K = graph_blk.shape
+    check(_lib.load().lakde_knn_prior_cov(ptr(X), N, D, row0, nb, ptr(graph_blk),
+                                          idx_code(graph_blk.dtype), K, ptr(out), stream_of(X)),
+          "knn_prior_cov")
+    return out
+
+
+def posterior_contribs(rowptr, col, val, Xn, X, nu0, S, nu):
+    nb, D = Xn.shape
+    check(_lib.load().lakde_posterior_contribs(ptr(rowptr), idx_code(rowptr.dtype), ptr(col),
+                                               idx_code(col.dtype), ptr(val), ptr(Xn), ptr(X),
+                                               nb, X.size(0), D, float(nu0), ptr(S), ptr(nu),
+                                               stream_of(X)),
+          "posterior_contribs")
+
+
+def posterior(prior_mode, prior, prior_jitter, S, nu, a, b, nu0, W_jitter, Xn, Linv,
+              prior_tril, logdet_W, logdet_prior, trace, rec, status, b0=0.0, b_out=None):
+    nb, D = Xn.shape
+    args = PosteriorArgs(nb, D, prior_mode, ptr(prior), ptr(prior_jitter), ptr(S), ptr(nu),
+                         ptr(a), ptr(b), float(nu0), ptr(W_jitter), ptr(Xn), ptr(Linv),
+                         ptr(prior_tril), ptr(logdet_W), ptr(logdet_prior), ptr(trace), ptr(rec),
+                         ptr(status), float(b0), ptr(b_out))
+    check(_lib.load().lakde_posterior(args, stream_of(Xn)), "posterior")
+
+
+def colsum(col, val, nnz_dev, N, out):
+    check(_lib.load().lakde_colsum(ptr(col), ptr(val), ptr(nnz_dev), N, ptr(out), stream_of(val)),
+          "colsum")
+
+
+class EstepWorkspace:
+    def __init__(self, N, nb, device):
+        nbytes = int(_lib.load().lakde_estep_workspace(N, nb))
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
+          arena=None, b=None, compact_only=False, timer=None):
+    N, D = X.shape
+    nb = rec_blk.size(0)
+    if arena is not None:
+        out = (ptr(arena.rowptr[b]), ptr(arena.row[b]), ptr(arena.col[b]), ptr(arena.val[b]),
+               arena.cap, ptr(arena.nnz_dev[b:b + 1]))
+    else:
+        out = (ptr(None), ptr(None), ptr(None), ptr(None), 0, ptr(None))
+    args = EstepArgs(N, D, nb, row0, ptr(X), ptr(rec_blk), mode, float(threshold or 0.0),
+                     ptr(c_old), ptr(old_colsum), ptr(c), out[0], out[1], out[2], out[3], out[4],
+                     out[5], ptr(ws.buf), ws.buf.numel())
+    lib = _lib.load()
+    st = stream_of(X)
+    if compact_only:
+        check(lib.lakde_estep_compact(args, st), "estep_compact")
+    elif timer is not None:
+        timer(lambda: check(lib.lakde_estep_main(args, st), "estep_main"))
+        check(lib.lakde_estep_finish(args, st), "estep_finish")
+    else:
+        check(lib.lakde_estep(args, st), "estep")
+
+
+def rebalance(arena, exclude_block, c_prev, c, ws):
+    N = c.numel()
+    check(_lib.load().lakde_rebalance(N, arena.num_blocks, arena.cap, exclude_block,
+                                      ptr(arena.nnz_dev), ptr(arena.col), ptr(arena.val),
+                                      ptr(c_prev), ptr(c), ptr(ws), stream_of(c)),
+          "rebalance")
+
+
+def pred_density(rec, Y, out):
+    N = rec.size(0)
+    M, D = Y.shape
+    lib = _lib.load()
+    nbytes = int(lib.lakde_pred_workspace(N, M))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=Y.device)
+    check(lib.lakde_pred_density(ptr(rec), N, ptr(Y), M, D, ptr(out), ptr(ws), nbytes,
+                                 stream_of(Y)),
+          "pred_density")
+    return out
+
+
+def fill_(t, v):
+    check(_lib.load().lakde_fill_f32(ptr(t), t.numel(), float(v), stream_of(t)), "fill")
+    return t
+
+
+NEG_INF = -math.inf

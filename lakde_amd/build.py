@@ -1,0 +1,99 @@
+"""Build liblakde_amd.so for gfx950 with hipcc (no cmake/ninja needed).
+
+    python -m lakde_amd.build [-j JOBS]
+
+Sources live in lakde_amd/csrc.  The dimension-templated fused kernels are
+explicitly instantiated in generated translation units (one per group of
+dimensions, under lakde_amd/_build/) so hipcc compiles them in parallel.
+"""
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "liblakde_amd.so")
+ARCH = os.environ.get("LAKDE_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+CFLAGS = [
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+    # exact, context-independent rounding (the E-step ballot and compaction
+    # recompute the same values) and no packed-FP32 instruction selection
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+    "-Wno-unused-command-line-argument",
+]
+
+SOURCES = ["ops_ref.hip", "posterior.hip", "fused.hip"]
+D_GROUPS = [(1, 6), (7, 12), (13, 18), (19, 24), (25, 32), (33, 40), (41, 48), (49, 56),
+            (57, 63)]
+
+
+def _gen_inst(lo, hi):
+    path = os.path.join(BUILD, f"fused_d{lo}_{hi}.hip")
+    lines = ['#include "%s"' % os.path.join(CSRC, "fused_impl.hpp"), "namespace lakde {"]
+    for d in range(lo, hi + 1):
+        lines += [
+            f"template int launch_estep_main<{d}>(const EParams &, int, hipStream_t);",
+            f"template int launch_estep_compact<{d}>(const EParams &, hipStream_t);",
+            f"template int launch_pred_main<{d}>(const PParams &, hipStream_t);",
+        ]
+    lines.append("}")
+    text = "\n".join(lines) + "\n"
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return path
+
+
+def _compile(src):
+    obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
+    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    t = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")
+    return obj, time.time() - t
+
+
+def build(jobs=None, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    srcs += [_gen_inst(lo, hi) for lo, hi in D_GROUPS]
+    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    objs = []
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for obj, dt in ex.map(_compile, srcs):
+            objs.append(obj)
+            if verbose:
+                print(f"  compiled {os.path.basename(obj)} in {dt:.1f}s", flush=True)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB} in {time.time() - t0:.1f}s", flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=None)
+    args = ap.parse_args()
+    try:
+        build(args.j)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

@@ -1,0 +1,88 @@
+// Shared host/device helpers for the lakde_amd HIP library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/lakde_amd.h"
+
+namespace lakde {
+
+// ------------------------------------------------------------ host errors
+
+void set_error(const std::string &msg);
+
+#define LAKDE_CHECK_ARG(cond, msg)                                  \
+  do {                                                              \
+    if (!(cond)) {                                                  \
+      ::lakde::set_error(std::string(__func__) + ": " + (msg));     \
+      return LAKDE_EINVAL;                                          \
+    }                                                               \
+  } while (0)
+
+#define LAKDE_HIP(call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::lakde::set_error(std::string(__func__) + ": " #call ": " +             \
+                         hipGetErrorString(e_));                               \
+      return LAKDE_EHIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+// after a kernel launch
+#define LAKDE_LAUNCHED() LAKDE_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(lakde_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------- device helpers
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float neg_inf() { return -__builtin_huge_valf(); }
+
+// torch.logaddexp semantics
+__device__ __forceinline__ float logaddexp_f(float a, float b) {
+  if (a == b && __builtin_isinf(a)) return a;
+  float m = fmaxf(a, b);
+  float d = -fabsf(a - b);
+  return m + log1pf(expf(d));
+}
+
+// a + log1p(clamp_min(-exp(b - a), -1)), NaN propagating like torch.clamp_min
+// (lakde/linalg_util.py:41-42)
+__device__ __forceinline__ float logsubexp_f(float a, float b) {
+  float t = -expf(b - a);
+  t = (t < -1.0f) ? -1.0f : t;
+  return a + log1pf(t);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// Packed record layout (floats): [ Linv packed lower row-major: T = D(D+1)/2 ]
+// [ x_n: D ] [ estep_const, estep_expo, pred_coeff, pred_expo ], padded to 4.
+__host__ __device__ constexpr int64_t rec_tri(int64_t D) { return D * (D + 1) / 2; }
+__host__ __device__ constexpr int64_t rec_floats(int64_t D) {
+  return ((rec_tri(D) + D + 4) + 3) / 4 * 4;
+}
+__host__ __device__ constexpr int64_t rec_x(int64_t D) { return rec_tri(D); }
+__host__ __device__ constexpr int64_t rec_c(int64_t D) { return rec_tri(D) + D; }
+
+}  // namespace lakde

@@ -1,0 +1,326 @@
+// Entry points of the fused E-step, rebalance and predictive density, plus
+// their dimension-independent kernels (row scans, rebalance passes, the
+// split combine).  The D-templated kernels live in fused_impl.hpp.
+#include <cfloat>
+
+#include "fused_impl.hpp"
+
+namespace lakde {
+
+LAKDE_FOR_EACH_D(LAKDE_DECL_D)
+
+static int dispatch_estep_main(int64_t D, const EParams &p, int mode, hipStream_t s) {
+  switch (D) {
+#define CASE(d) case d: return launch_estep_main<d>(p, mode, s);
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  return LAKDE_EINVAL;
+}
+
+static int dispatch_estep_compact(int64_t D, const EParams &p, hipStream_t s) {
+  switch (D) {
+#define CASE(d) case d: return launch_estep_compact<d>(p, s);
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  return LAKDE_EINVAL;
+}
+
+static int dispatch_pred_main(int64_t D, const PParams &p, hipStream_t s) {
+  switch (D) {
+#define CASE(d) case d: return launch_pred_main<d>(p, s);
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  return LAKDE_EINVAL;
+}
+
+// ----------------------------------------------------------- row scans
+// Exclusive scan of per-(kernel, chunk) popcounts along each kernel row.
+__global__ void __launch_bounds__(256) estep_scan_rows_kernel(EParams p) {
+  __shared__ int32_t sh[256];
+  const int64_t n = blockIdx.x;
+  const int tid = threadIdx.x;
+  const uint64_t *mrow = p.masks + n * p.nchunks;
+  const int64_t per = ceil_div(p.nchunks, 256);
+  const int64_t c0 = tid * per;
+  const int64_t c1 = (c0 + per) < p.nchunks ? (c0 + per) : p.nchunks;
+  int32_t s = 0;
+  for (int64_t c = c0; c < c1; ++c) s += __popcll(mrow[c]);
+  sh[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int32_t v = (tid >= off) ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  int32_t run = sh[tid] - s;  // exclusive
+  for (int64_t c = c0; c < c1; ++c) {
+    p.offs[n * p.nchunks + c] = run;
+    run += __popcll(mrow[c]);
+  }
+  if (tid == 255) p.totals[n] = sh[255];
+}
+
+// rowptr = exclusive scan of the row totals; nnz = rowptr[nb].
+__global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
+  __shared__ int32_t sh[1024];
+  const int tid = threadIdx.x;
+  const int64_t per = ceil_div(p.nb, 1024);
+  const int64_t r0 = tid * per;
+  const int64_t r1 = (r0 + per) < p.nb ? (r0 + per) : p.nb;
+  int32_t s = 0;
+  for (int64_t r = r0; r < r1; ++r) s += p.totals[r];
+  sh[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int32_t v = (tid >= off) ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  int32_t run = sh[tid] - s;
+  for (int64_t r = r0; r < r1; ++r) {
+    p.rowptr[r] = run;
+    run += p.totals[r];
+  }
+  if (tid == 1023) {
+    p.rowptr[p.nb] = sh[1023];
+    *p.nnz_dev = sh[1023];
+  }
+}
+
+// ------------------------------------------------------------ rebalance
+__device__ __forceinline__ float clamp_nan(float x, float lo, float hi) {
+  return (x != x) ? x : fminf(fmaxf(x, lo), hi);
+}
+
+// coeff[m] = clamp(exp(c_prev[m] - c[m]), tiny, max); sums[m] = 0
+__global__ void __launch_bounds__(256) rebal_coeff_kernel(int64_t N, const float *c_prev,
+                                                         const float *c, float *coeff,
+                                                         float *sums) {
+  int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  coeff[m] = clamp_nan(expf(c_prev[m] - c[m]), FLT_MIN, FLT_MAX);
+  sums[m] = 0.f;
+}
+
+// other blocks: v *= coeff[col]; every entry: sums[col] += v
+__global__ void __launch_bounds__(256) rebal_scale_sum_kernel(int64_t num_blocks, int64_t cap,
+                                                             int64_t exclude, const int32_t *nnz,
+                                                             const int32_t *cols, float *vals,
+                                                             const float *coeff, float *sums) {
+  const int64_t total = num_blocks * cap;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / cap;
+    if (e - b * cap >= nnz[b]) continue;
+    const int32_t col = cols[e];
+    float v = vals[e];
+    if (b != exclude) {
+      v = v * coeff[col];
+      vals[e] = v;
+    }
+    atomicAdd(&sums[col], v);
+  }
+}
+
+// s = clamp_min(sums, eps); c += log s; coeff = clamp(1/s, tiny, max)
+__global__ void __launch_bounds__(256) rebal_norm_kernel(int64_t N, float *c, const float *sums,
+                                                        float *coeff) {
+  int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  float s = sums[m];
+  s = (s != s) ? s : fmaxf(s, FLT_EPSILON);
+  c[m] = c[m] + logf(s);
+  coeff[m] = clamp_nan(1.0f / s, FLT_MIN, FLT_MAX);
+}
+
+__global__ void __launch_bounds__(256) rebal_rescale_kernel(int64_t num_blocks, int64_t cap,
+                                                           const int32_t *nnz,
+                                                           const int32_t *cols, float *vals,
+                                                           const float *coeff) {
+  const int64_t total = num_blocks * cap;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / cap;
+    if (e - b * cap >= nnz[b]) continue;
+    vals[e] = vals[e] * coeff[cols[e]];
+  }
+}
+
+// ----------------------------------------------------------- predictive
+__global__ void __launch_bounds__(256) pred_combine_kernel(PParams p, float log_n) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= p.M) return;
+  Lse t;
+  t.init();
+  for (int64_t s = 0; s < p.nsplit; ++s) t.merge(p.part_m[s * p.M + m], p.part_s[s * p.M + m]);
+  p.out[m] = -log_n + t.value();
+}
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+static void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit) {
+  const int64_t chunks = ceil_div(M, kWave);
+  int64_t ns = ceil_div(16384, chunks * 4);
+  const int64_t max_ns = ceil_div(N, 256);
+  if (ns > max_ns) ns = max_ns;
+  if (ns < 1) ns = 1;
+  *per_split = ceil_div(N, ns);
+  *nsplit = ceil_div(N, *per_split);
+}
+
+}  // namespace lakde
+
+using namespace lakde;
+
+extern "C" {
+
+size_t lakde_estep_workspace(int64_t N, int64_t nb) {
+  const int64_t nchunks = ceil_div(N, kWave);
+  return align256((size_t)nb * nchunks * 8) + align256((size_t)nb * nchunks * 4) +
+         align256((size_t)nb * 4);
+}
+
+static int fill_eparams(const lakde_estep_args &a, EParams *p) {
+  LAKDE_CHECK_ARG(a.D >= 1 && a.D <= 63, "D must be in [1, 63]");
+  LAKDE_CHECK_ARG(a.N >= 1 && a.nb >= 1 && a.row0 >= 0 && a.row0 + a.nb <= a.N, "bad block");
+  LAKDE_CHECK_ARG(a.N < (1LL << 31), "N must fit int32 column indices");
+  LAKDE_CHECK_ARG(a.X && a.rec && a.c, "null pointer");
+  LAKDE_CHECK_ARG(a.mode == LAKDE_ESTEP_ACCUM || a.mode == LAKDE_ESTEP_FIXED ||
+                      a.mode == LAKDE_ESTEP_UPDATE, "bad mode");
+  if (a.mode == LAKDE_ESTEP_UPDATE) LAKDE_CHECK_ARG(a.c_old && a.old_colsum, "UPDATE needs c_old/old_colsum");
+  if (a.mode != LAKDE_ESTEP_ACCUM) {
+    LAKDE_CHECK_ARG(a.rowptr && a.nnz_dev && a.out_row && a.out_col && a.out_val, "null output");
+    LAKDE_CHECK_ARG(a.workspace && a.workspace_bytes >= lakde_estep_workspace(a.N, a.nb),
+                    "workspace too small");
+    LAKDE_CHECK_ARG(a.threshold >= 0.f, "negative threshold");
+  }
+  p->N = a.N;
+  p->nb = a.nb;
+  p->row0 = a.row0;
+  p->nchunks = ceil_div(a.N, kWave);
+  p->X = a.X;
+  p->rec = a.rec;
+  p->threshold = a.threshold;
+  // log-domain pre-test with a 1% margin: exp(d) > t is only evaluated when
+  // d > log(t) - 0.01, which cannot drop an entry (expf error << 1%)
+  p->log_t_lo = a.threshold > 0.f ? logf(a.threshold) - 0.01f : -110.f;
+  p->c_old = a.c_old;
+  p->old_colsum = a.old_colsum;
+  p->c = a.c;
+  char *ws = (char *)a.workspace;
+  const int64_t cells = a.nb * p->nchunks;
+  p->masks = (uint64_t *)ws;
+  p->offs = (int32_t *)(ws + align256((size_t)cells * 8));
+  p->totals = (int32_t *)(ws + align256((size_t)cells * 8) + align256((size_t)cells * 4));
+  p->rowptr = a.rowptr;
+  p->out_row = a.out_row;
+  p->out_col = a.out_col;
+  p->out_val = a.out_val;
+  p->cap = a.cap;
+  p->nnz_dev = a.nnz_dev;
+  return LAKDE_OK;
+}
+
+int lakde_estep_main(const lakde_estep_args *args, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr, "null args");
+  EParams p;
+  int rc = fill_eparams(*args, &p);
+  if (rc) return rc;
+  rc = dispatch_estep_main(args->D, p, args->mode, as_stream(stream));
+  if (rc) set_error("lakde_estep_main: launch failed");
+  return rc;
+}
+
+int lakde_estep_finish(const lakde_estep_args *args, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr, "null args");
+  if (args->mode == LAKDE_ESTEP_ACCUM) return LAKDE_OK;
+  EParams p;
+  int rc = fill_eparams(*args, &p);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(estep_scan_rows_kernel, dim3((unsigned)p.nb), dim3(256), 0, s, p);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(estep_rowptr_kernel, dim3(1), dim3(1024), 0, s, p);
+  LAKDE_LAUNCHED();
+  rc = dispatch_estep_compact(args->D, p, s);
+  if (rc) set_error("lakde_estep_finish: compaction launch failed");
+  return rc;
+}
+
+int lakde_estep(const lakde_estep_args *args, lakde_stream_t stream) {
+  int rc = lakde_estep_main(args, stream);
+  if (rc) return rc;
+  return lakde_estep_finish(args, stream);
+}
+
+int lakde_estep_compact(const lakde_estep_args *args, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr && args->mode != LAKDE_ESTEP_ACCUM, "bad args");
+  EParams p;
+  int rc = fill_eparams(*args, &p);
+  if (rc) return rc;
+  rc = dispatch_estep_compact(args->D, p, as_stream(stream));
+  if (rc) set_error("lakde_estep_compact: launch failed");
+  return rc;
+}
+
+int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_block,
+                    const int32_t *nnz, const int32_t *cols, float *vals, const float *c_prev,
+                    float *c, float *ws, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(N >= 1 && num_blocks >= 1 && cap >= 1, "bad size");
+  LAKDE_CHECK_ARG(nnz && cols && vals && c_prev && c && ws, "null pointer");
+  hipStream_t s = as_stream(stream);
+  float *coeff = ws, *sums = ws + N;
+  const unsigned gN = (unsigned)ceil_div(N, 256);
+  int64_t ge = ceil_div(num_blocks * cap, 256);
+  if (ge > 8192) ge = 8192;
+  hipLaunchKernelGGL(rebal_coeff_kernel, dim3(gN), dim3(256), 0, s, N, c_prev, c, coeff, sums);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(rebal_scale_sum_kernel, dim3((unsigned)ge), dim3(256), 0, s, num_blocks, cap,
+                     exclude_block, nnz, cols, vals, coeff, sums);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(rebal_norm_kernel, dim3(gN), dim3(256), 0, s, N, c, sums, coeff);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(rebal_rescale_kernel, dim3((unsigned)ge), dim3(256), 0, s, num_blocks, cap,
+                     nnz, cols, vals, coeff);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+size_t lakde_pred_workspace(int64_t N, int64_t M) {
+  int64_t per, ns;
+  pred_geometry(N, M, &per, &ns);
+  return 2 * align256((size_t)ns * M * 4);
+}
+
+int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, int64_t D,
+                       float *out, void *workspace, size_t workspace_bytes,
+                       lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 1 && D <= 63, "D must be in [1, 63]");
+  LAKDE_CHECK_ARG(N >= 1 && M >= 0, "bad size");
+  LAKDE_CHECK_ARG(rec && Y && out && workspace, "null pointer");
+  LAKDE_CHECK_ARG(workspace_bytes >= lakde_pred_workspace(N, M), "workspace too small");
+  if (M == 0) return LAKDE_OK;
+  PParams p;
+  p.N = N;
+  p.M = M;
+  pred_geometry(N, M, &p.per_split, &p.nsplit);
+  p.rec = rec;
+  p.Y = Y;
+  p.part_m = (float *)workspace;
+  p.part_s = (float *)((char *)workspace + align256((size_t)p.nsplit * M * 4));
+  p.out = out;
+  hipStream_t s = as_stream(stream);
+  int rc = dispatch_pred_main(D, p, s);
+  if (rc) { set_error("lakde_pred_density: launch failed"); return rc; }
+  hipLaunchKernelGGL(pred_combine_kernel, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, p,
+                     (float)std::log((double)N));
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+}  // extern "C"

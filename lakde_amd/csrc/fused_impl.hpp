@@ -1,0 +1,243 @@
+// D-templated kernels of the fused E-step and the predictive density.  The
+// launchers are explicitly instantiated per dimension in generated
+// translation units (see lakde_amd/build.py) so they compile in parallel.
+#pragma once
+
+#include "qf.hpp"
+
+namespace lakde {
+
+// Device-side parameters of one E-step launch.
+struct EParams {
+  int64_t N, nb, row0, nchunks;
+  const float *X;
+  const float *rec;
+  float threshold, log_t_lo;
+  const float *c_old;
+  const float *old_colsum;
+  float *c;
+  uint64_t *masks;   // nb x nchunks
+  int32_t *offs;     // nb x nchunks
+  int32_t *totals;   // nb
+  int32_t *rowptr;   // nb + 1
+  int32_t *out_row, *out_col;
+  float *out_val;
+  int64_t cap;
+  int32_t *nnz_dev;
+};
+
+struct PParams {
+  int64_t N, M, per_split, nsplit;
+  const float *rec;
+  const float *Y;
+  float *part_m, *part_s;  // nsplit x M
+  float *out;
+};
+
+constexpr int kEThreads = 256;  // 4 waves; lane = column, wave = kernel quarter
+
+// ---------------------------------------------------------------- E-step
+// One workgroup per chunk of 64 columns (points m) and all nb kernels of the
+// block.  Pass 1 (ACCUM/UPDATE): streaming log-sum-exp over the kernels,
+// merged across the 4 waves in LDS, then the normaliser update.  Pass 2
+// (UPDATE/FIXED): recompute log rho, test the responsibility and write the
+// wave ballot as a 64-bit mask per (kernel, chunk).
+template <int D, int MODE>
+__global__ void __launch_bounds__(kEThreads)
+estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EParams p) {
+  __shared__ float sm_m[4][kWave], sm_s[4][kWave], sm_c[kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  // wave index made explicitly uniform so the record loads below are scalar
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t chunk = blockIdx.x;
+  const int64_t m = chunk * kWave + lane;
+  const bool valid = m < p.N;
+  float xm[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) xm[j] = valid ? X[m * D + j] : 0.f;
+  const int64_t q = (p.nb + 3) / 4;
+  const int64_t nbeg = w * q;
+  const int64_t nend = (nbeg + q) < p.nb ? (nbeg + q) : p.nb;
+  constexpr int64_t R = rec_floats(D);
+  const int64_t diag_n = m - p.row0;
+  float cm = 0.f;
+
+  if (MODE != LAKDE_ESTEP_FIXED) {
+    Lse l;
+    l.init();
+    int64_t n = nbeg;
+    if (D <= 24) {
+      for (; n + 4 <= nend; n += 4) {
+        float x0 = log_rho<D>(rec + (n + 0) * R, xm);
+        float x1 = log_rho<D>(rec + (n + 1) * R, xm);
+        float x2 = log_rho<D>(rec + (n + 2) * R, xm);
+        float x3 = log_rho<D>(rec + (n + 3) * R, xm);
+        if (n + 0 == diag_n) x0 = neg_inf();
+        if (n + 1 == diag_n) x1 = neg_inf();
+        if (n + 2 == diag_n) x2 = neg_inf();
+        if (n + 3 == diag_n) x3 = neg_inf();
+        l.add4(x0, x1, x2, x3);
+      }
+    }
+    for (; n < nend; ++n) {
+      float x = log_rho<D>(rec + n * R, xm);
+      if (n == diag_n) x = neg_inf();
+      l.add1(x);
+    }
+    sm_m[w][lane] = l.m;
+    sm_s[w][lane] = l.s;
+    __syncthreads();
+    if (w == 0) {
+      Lse t;
+      t.init();
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww) t.merge(sm_m[ww][lane], sm_s[ww][lane]);
+      const float lse = t.value();
+      if (valid) {
+        if (MODE == LAKDE_ESTEP_ACCUM) {
+          cm = logaddexp_f(p.c[m], lse);
+        } else {
+          const float co = p.c_old[m];
+          const float oc = logf(p.old_colsum[m]) + co;
+          cm = logaddexp_f(logsubexp_f(co, oc), lse);
+        }
+        p.c[m] = cm;
+      }
+      sm_c[lane] = cm;
+    }
+    __syncthreads();
+    cm = sm_c[lane];
+  } else {
+    cm = valid ? p.c[m] : 0.f;
+  }
+  if (MODE == LAKDE_ESTEP_ACCUM) return;
+
+  for (int64_t n = nbeg; n < nend; ++n) {
+    float x = log_rho<D>(rec + n * R, xm);
+    if (n == diag_n) x = neg_inf();
+    const float d = __fsub_rn(x, cm);
+    float r;
+    const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
+    const uint64_t mask = __ballot(keep);
+    if (lane == 0) p.masks[n * p.nchunks + chunk] = mask;
+  }
+}
+
+// Compaction: one thread per (kernel, chunk) mask; the kept entries are
+// recomputed with the same device functions (identical bits) and written at
+// rowptr[n] + offs[n, chunk] in column order -> canonical CSR.
+template <int D>
+__global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
+  const int64_t nnz = *p.nnz_dev;
+  if (nnz > p.cap) return;
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= p.nb * p.nchunks) return;
+  uint64_t mask = p.masks[idx];
+  if (mask == 0) return;
+  const int64_t n = idx / p.nchunks;
+  const int64_t chunk = idx - n * p.nchunks;
+  constexpr int64_t R = rec_floats(D);
+  const float *rec = p.rec + n * R;
+  int64_t pos = (int64_t)p.rowptr[n] + p.offs[idx];
+  while (mask) {
+    const int b = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    const int64_t m = chunk * kWave + b;
+    float xm[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xm[j] = p.X[m * D + j];
+    const float x = log_rho<D>(rec, xm);
+    const float d = __fsub_rn(x, p.c[m]);
+    float r = 0.f;
+    keep_resp(d, p.threshold, p.log_t_lo, &r);
+    p.out_row[pos] = (int32_t)n;
+    p.out_col[pos] = (int32_t)m;
+    p.out_val[pos] = r;
+    ++pos;
+  }
+}
+
+// ------------------------------------------------------------ predictive
+// grid (column chunks, kernel splits); each wave streams its share of the
+// split's kernels for 64 test points; partial (max, sum) per split.
+template <int D>
+__global__ void __launch_bounds__(kEThreads)
+pred_main_kernel(const float *__restrict__ rec, const float *__restrict__ Y, PParams p) {
+  __shared__ float sm_m[4][kWave], sm_s[4][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t chunk = blockIdx.x;
+  const int64_t split = blockIdx.y;
+  const int64_t m = chunk * kWave + lane;
+  const bool valid = m < p.M;
+  float ym[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) ym[j] = valid ? Y[m * D + j] : 0.f;
+  const int64_t sbeg = split * p.per_split;
+  const int64_t send = (sbeg + p.per_split) < p.N ? (sbeg + p.per_split) : p.N;
+  const int64_t q = (send - sbeg + 3) / 4;
+  const int64_t nbeg = sbeg + w * q;
+  const int64_t nend = (nbeg + q) < send ? (nbeg + q) : send;
+  constexpr int64_t R = rec_floats(D);
+  Lse l;
+  l.init();
+  int64_t n = nbeg;
+  if (D <= 24) {
+    for (; n + 4 <= nend; n += 4) {
+      float x0 = log_pred<D>(rec + (n + 0) * R, ym);
+      float x1 = log_pred<D>(rec + (n + 1) * R, ym);
+      float x2 = log_pred<D>(rec + (n + 2) * R, ym);
+      float x3 = log_pred<D>(rec + (n + 3) * R, ym);
+      l.add4(x0, x1, x2, x3);
+    }
+  }
+  for (; n < nend; ++n) l.add1(log_pred<D>(rec + n * R, ym));
+  sm_m[w][lane] = l.m;
+  sm_s[w][lane] = l.s;
+  __syncthreads();
+  if (w == 0 && valid) {
+    Lse t;
+    t.init();
+#pragma unroll
+    for (int ww = 0; ww < 4; ++ww) t.merge(sm_m[ww][lane], sm_s[ww][lane]);
+    p.part_m[split * p.M + m] = t.m;
+    p.part_s[split * p.M + m] = t.s;
+  }
+}
+
+// ------------------------------------------------------------- launchers
+template <int D>
+int launch_estep_main(const EParams &p, int mode, hipStream_t s) {
+  dim3 grid((unsigned)p.nchunks);
+  if (mode == LAKDE_ESTEP_ACCUM)
+    hipLaunchKernelGGL((estep_main_kernel<D, LAKDE_ESTEP_ACCUM>), grid, dim3(kEThreads), 0, s, p.rec, p.X, p);
+  else if (mode == LAKDE_ESTEP_FIXED)
+    hipLaunchKernelGGL((estep_main_kernel<D, LAKDE_ESTEP_FIXED>), grid, dim3(kEThreads), 0, s, p.rec, p.X, p);
+  else
+    hipLaunchKernelGGL((estep_main_kernel<D, LAKDE_ESTEP_UPDATE>), grid, dim3(kEThreads), 0, s, p.rec, p.X, p);
+  return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+}
+
+template <int D>
+int launch_estep_compact(const EParams &p, hipStream_t s) {
+  const int64_t total = p.nb * p.nchunks;
+  if (total == 0) return LAKDE_OK;
+  hipLaunchKernelGGL((estep_compact_kernel<D>), dim3((unsigned)ceil_div(total, 256)), dim3(256), 0,
+                     s, p);
+  return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+}
+
+template <int D>
+int launch_pred_main(const PParams &p, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(p.M, kWave), (unsigned)p.nsplit);
+  hipLaunchKernelGGL((pred_main_kernel<D>), grid, dim3(kEThreads), 0, s, p.rec, p.Y, p);
+  return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+}
+
+// declarations for the dispatcher (defined by explicit instantiation)
+#define LAKDE_DECL_D(d)                                                   \
+  extern template int launch_estep_main<d>(const EParams &, int, hipStream_t); \
+  extern template int launch_estep_compact<d>(const EParams &, hipStream_t);   \
+  extern template int launch_pred_main<d>(const PParams &, hipStream_t);
+
+}  // namespace lakde

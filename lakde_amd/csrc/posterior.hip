@@ -1,0 +1,242 @@
+// Batched per-point posterior factorisation: one wavefront (64 lanes) per
+// kernel point, D <= 63, everything staged in LDS.
+//
+// Per point n (lakde/local_adaptive_kde.py:83-96, lakde/linalg_util.py:109-170,
+// lakde/local_knn_kde.py:195-207, lakde/local_hierarchical_kde.py:269-271):
+//   kNN prior: factor prior[n] with its persistent jitter (FP32), write the
+//              factor, prior_j = prior[n] + j0 I;  shared prior: prior_j = prior.
+//   W_inv = fl(prior_j * fl(nu0 * fl(a/b))) + S[n], factored with the
+//   persistent jitter schedule (decisions in FP32, LAPACK potrf semantics:
+//   a pivot that is not > 0 fails); a jittered factor is refactored and
+//   inverted in FP64, others in FP32.  Then L^{-1}, logdet_W = 2 sum log diag
+//   accumulated in FP64, the trace term and the packed record.
+#include "common.hpp"
+
+namespace lakde {
+
+constexpr int kPostMaxD = 63;
+
+// digamma for x > 0: upward recurrence to x >= 6, then the asymptotic series
+// psi(x) = ln x - 1/(2x) - sum_k B_2k / (2k x^2k) (Abramowitz & Stegun 6.3.18),
+// truncated after the x^-10 term (error < 1e-11 for x >= 6).
+__device__ double digamma_d(double x) {
+  double r = 0.0;
+  while (x < 6.0) { r -= 1.0 / x; x += 1.0; }
+  double f = 1.0 / (x * x);
+  double t = f * (-1.0 / 12 + f * (1.0 / 120 + f * (-1.0 / 252 + f * (1.0 / 240 + f * (-1.0 / 132)))));
+  return r + log(x) - 0.5 / x + t;
+}
+
+// Lower Cholesky of M (+ jitter on the diagonal, added in FP32) into Lout.
+// Lane i owns row i.  Returns true on success (every pivot > 0).
+template <typename T>
+__device__ bool chol_wave(const float *M, T *Lout, int D, int ld, float jitter) {
+  const int lane = threadIdx.x;
+  for (int j = 0; j < D; ++j) {
+    T s = T(0);
+    if (lane >= j && lane < D) {
+      float mij = M[lane * ld + j];
+      if (lane == j) mij = mij + jitter;
+      T acc = T(mij);
+      for (int k = 0; k < j; ++k) acc = fma(-Lout[lane * ld + k], Lout[j * ld + k], acc);
+      s = acc;
+    }
+    T piv = __shfl(s, j, kWave);
+    if (!(piv > T(0))) {
+      __syncthreads();
+      return false;
+    }
+    T d = sqrt(piv);
+    if (lane == j) Lout[j * ld + j] = d;
+    else if (lane > j && lane < D) Lout[lane * ld + j] = s / d;
+    __syncthreads();
+  }
+  return true;
+}
+
+// X = L^{-1} (lower).  Lane c owns column c; rows run in the same order on
+// every lane so the L reads are LDS broadcasts.
+template <typename T>
+__device__ void tril_inv_wave(const T *L, T *X, int D, int ld) {
+  const int c = threadIdx.x;
+  if (c < D) {
+    for (int i = 0; i < D; ++i) {
+      T acc = (i == c) ? T(1) : T(0);
+      for (int k = 0; k < i; ++k) acc = fma(-L[i * ld + k], X[k * ld + c], acc);
+      X[i * ld + c] = (i < c) ? T(0) : acc / L[i * ld + i];
+    }
+  }
+  __syncthreads();
+}
+
+// Factor with the persistent-jitter schedule of stabilized_cholesky:
+// try the stored jitter; on failure restart from 0 and add eps * 2^k
+// (k = 0..31, accumulated in FP32) until the factorisation succeeds.
+// Returns the final jitter; *ok = false when it never converged.
+__device__ float stabilized_chol_wave(const float *M, float *L, int D, int ld, float jitter_in,
+                                      bool *ok) {
+  if (chol_wave<float>(M, L, D, ld, jitter_in)) { *ok = true; return jitter_in; }
+  const float eps = 1.1920928955078125e-07f;  // FLT_EPSILON
+  float j = 0.f;
+  for (int k = 0; k < 32; ++k) {
+    j = j + ldexpf(eps, k);
+    if (chol_wave<float>(M, L, D, ld, j)) { *ok = true; return j; }
+  }
+  *ok = false;
+  return j;
+}
+
+__global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
+  extern __shared__ float smem[];
+  const int D = (int)a.D;
+  const int ld = D + 1;
+  const int64_t n = blockIdx.x;
+  const int lane = threadIdx.x;
+  float *R0 = smem;               // working matrix
+  float *R1 = R0 + D * ld;
+  float *R2 = R1 + D * ld;
+  float *R3 = R2 + D * ld;
+  const int64_t DD = (int64_t)D * D;
+
+  // ---- prior
+  const float *P = (a.prior_mode == LAKDE_PRIOR_KNN) ? a.prior + n * DD : a.prior;
+  for (int t = lane; t < D * D; t += kWave) R0[(t / D) * ld + (t % D)] = P[t];
+  __syncthreads();
+  if (a.prior_mode == LAKDE_PRIOR_KNN) {
+    bool ok;
+    float j0 = stabilized_chol_wave(R0, R2, D, ld, a.prior_jitter[n], &ok);
+    if (!ok && lane == 0) *a.status = 1;
+    if (lane == 0) a.prior_jitter[n] = j0;
+    double ld0 = 0.0;
+    for (int t = lane; t < D * D; t += kWave) {
+      int i = t / D, k = t % D;
+      a.prior_tril[n * DD + t] = (k <= i) ? R2[i * ld + k] : 0.f;
+    }
+    if (lane < D) ld0 = log((double)R2[lane * ld + lane]);
+    ld0 = wave_sum(ld0);
+    if (lane == 0) a.logdet_prior[n] = (float)(2.0 * ld0);
+    if (lane < D) R0[lane * ld + lane] = R0[lane * ld + lane] + j0;
+    __syncthreads();
+  }
+
+  // ---- W_inv = prior_j * (nu0 * E[tau]) + S
+  const float etau = a.a[n] / a.b[n];
+  const float scale = a.nu0 * etau;
+  const float *S = a.S + n * DD;
+  for (int t = lane; t < D * D; t += kWave) {
+    int i = t / D, k = t % D;
+    R0[i * ld + k] = __fadd_rn(__fmul_rn(R0[i * ld + k], scale), S[t]);
+  }
+  __syncthreads();
+
+  // ---- factor with the persistent jitter (FP32 decisions)
+  bool ok;
+  float jw = stabilized_chol_wave(R0, R2, D, ld, a.W_jitter[n], &ok);
+  if (!ok && lane == 0) *a.status = 1;
+  if (lane == 0) a.W_jitter[n] = jw;
+
+  // Linv ends in R1 (FP32 path) or R2 (FP64 path); R3 is free afterwards.
+  float *Linv;
+  if (jw != 0.f && ok) {
+    // near-singular: refactor fl32(W_inv + j I) and invert in FP64
+    double *L64 = reinterpret_cast<double *>(R2);  // overlays R2|R3
+    double *I64 = reinterpret_cast<double *>(R0);  // overlays R0|R1
+    if (chol_wave<double>(R0, L64, D, ld, jw)) {
+      tril_inv_wave<double>(L64, I64, D, ld);
+      for (int t = lane; t < D * ld; t += kWave) R2[t] = (float)I64[t];
+      __syncthreads();
+      Linv = R2;
+    } else {
+      // FP64 disagrees with the FP32 decision: keep the FP32 factor
+      chol_wave<float>(R0, R2, D, ld, jw);
+      tril_inv_wave<float>(R2, R1, D, ld);
+      Linv = R1;
+    }
+  } else {
+    tril_inv_wave<float>(R2, R1, D, ld);
+    Linv = R1;
+  }
+
+  // ---- outputs: Linv, logdet_W
+  for (int t = lane; t < D * D; t += kWave) {
+    int i = t / D, k = t % D;
+    a.Linv[n * DD + t] = (k <= i) ? Linv[i * ld + k] : 0.f;
+  }
+  double lsum = (lane < D) ? log((double)Linv[lane * ld + lane]) : 0.0;
+  lsum = wave_sum(lsum);
+  const float logdet = (float)(2.0 * lsum);
+
+  // ---- trace = nu * sum_{i,j} (sum_k T[k][i] Linv[j][k])^2
+  float *Tm = R3;
+  const float *Tsrc = (a.prior_mode == LAKDE_PRIOR_KNN) ? a.prior_tril + n * DD : a.prior;
+  __syncthreads();
+  for (int t = lane; t < D * D; t += kWave) Tm[(t / D) * ld + (t % D)] = Tsrc[t];
+  __syncthreads();
+  float tr = 0.f;
+  if (lane < D) {
+    for (int i = 0; i < D; ++i) {
+      float s = 0.f;
+      for (int k = 0; k <= lane; ++k) s = fmaf(Tm[k * ld + i], Linv[lane * ld + k], s);
+      tr = fmaf(s, s, tr);
+    }
+  }
+  tr = wave_sum(tr);
+  const float nu = a.nu[n];
+
+  // ---- packed record
+  const int64_t R = rec_floats(D);
+  float *rec = a.rec + n * R;
+  for (int t = lane; t < (int)rec_tri(D); t += kWave) {
+    int i = 0, r = t;
+    while (r > i) { r -= i + 1; ++i; }
+    rec[t] = Linv[i * ld + r];
+  }
+  if (lane < D) rec[rec_x(D) + lane] = a.Xn[n * D + lane];
+  if (lane == 0) {
+    a.logdet_W[n] = logdet;
+    const float trace = nu * tr;
+    a.trace[n] = trace;
+    if (a.b_out != nullptr) a.b_out[n] = a.b0 + (a.nu0 * trace) / 2.f;
+    // E-step constants (lakde/local_adaptive_kde.py:36-44)
+    const float half_nu = nu / 2.f;
+    double mvd = 0.0;
+    for (int i = 1; i <= D; ++i) mvd += digamma_d((double)(half_nu + (1.f - (float)i) / 2.f));
+    const float eld = ((float)mvd + (float)(D * 0.69314718055994530942)) + logdet;
+    const float c_est = 0.5f * eld;
+    const float e_est = -0.5f * nu;
+    // predictive constants (lakde/local_adaptive_kde.py:209-214)
+    const float h1 = (nu + 1.f) / 2.f;
+    const float h2 = ((nu + 1.f) - (float)D) / 2.f;
+    const float lg1 = (float)lgamma((double)h1);
+    const float lg2 = (float)lgamma((double)h2);
+    const float pc = ((lg1 - lg2) + 0.5f * logdet) - (float)(D / 2.0 * 1.14472988584940017414);
+    float *cst = rec + rec_c(D);
+    cst[0] = c_est;
+    cst[1] = e_est;
+    cst[2] = pc;
+    cst[3] = h1;
+  }
+}
+
+}  // namespace lakde
+
+using namespace lakde;
+
+extern "C" int lakde_posterior(const lakde_posterior_args *args, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr, "null args");
+  const lakde_posterior_args &a = *args;
+  LAKDE_CHECK_ARG(a.D >= 1 && a.D <= kPostMaxD, "D must be in [1, 63]");
+  LAKDE_CHECK_ARG(a.nb >= 0, "negative block size");
+  LAKDE_CHECK_ARG(a.prior && a.S && a.nu && a.a && a.b && a.W_jitter && a.Xn && a.Linv &&
+                      a.logdet_W && a.trace && a.rec && a.status,
+                  "null pointer");
+  LAKDE_CHECK_ARG(a.prior_mode == LAKDE_PRIOR_KNN || a.prior_mode == LAKDE_PRIOR_SHARED,
+                  "bad prior mode");
+  if (a.prior_mode == LAKDE_PRIOR_KNN)
+    LAKDE_CHECK_ARG(a.prior_jitter && a.prior_tril && a.logdet_prior, "kNN prior outputs missing");
+  if (a.nb == 0) return LAKDE_OK;
+  size_t lds = (size_t)4 * a.D * (a.D + 1) * sizeof(float);
+  hipLaunchKernelGGL(posterior_kernel, dim3((unsigned)a.nb), dim3(64), lds, as_stream(stream), a);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}

@@ -1,0 +1,113 @@
+"""LocalKNearestKDE on the fused HIP path (mirrors lakde/local_knn_kde.py:50-372)."""
+
+import math
+from typing import Union
+
+import numpy as np
+import torch
+
+from . import _fused as F
+from ._lib import PRIOR_KNN
+from .local_adaptive_kde import LocallyAdaptiveKDE
+from .utils import compute_knn_graph
+
+
+class LocalKNearestKDE(LocallyAdaptiveKDE):
+    """Locally adaptive KDE with a kNN-covariance prior per point.
+
+    Same constructor, ``fit`` and ``log_pred_density`` as the reference class;
+    data must be float32 on the GPU.
+    """
+
+    def __init__(self, nu_0: Union[torch.Tensor, float] = None,
+                 k_or_knn_graph: Union[torch.Tensor, int] = None, block_size: int = 512,
+                 a_0: Union[torch.Tensor, float] = 1e-6, b_0: Union[torch.Tensor, float] = 1e-6,
+                 verbose=False, logs=False):
+        super().__init__(nu_0, a_0, b_0, block_size, verbose, logs)
+        self.k_or_knn_graph = k_or_knn_graph
+        self.k = None
+        self.knn_graph = None
+        self.logdet_sigma_0 = None
+        self.sigma_0_lambda_mm_trace = None
+        self.sigma_0_jitter = None
+
+    # lakde/local_knn_kde.py:73-138
+    def init_parameters_(self, X, sparsity_threshold=None):
+        self._common_init(X, sparsity_threshold)
+        N, D = X.shape
+        self.sigma_0_jitter = X.new_zeros(N)
+        g = self.k_or_knn_graph
+        if isinstance(g, (int, np.integer)):
+            g = compute_knn_graph(X, int(g), 1024, include_self=False, dtype=torch.int32)
+        elif isinstance(g, np.ndarray):
+            g = torch.from_numpy(g)
+        elif not (torch.is_tensor(g) and g.dtype in (torch.int32, torch.int64)):
+            raise ValueError("k_or_knn_graph type not recognised")
+        self.knn_graph = g.to(X.device).contiguous()
+        del self.k_or_knn_graph
+        self.k = self.knn_graph.size(-1)
+        self._init_arena(X, self.knn_graph, sparsity_threshold)
+        self.nu = X.new_empty(N)
+        self.a = (self.a_0 + self.nu_0 * D / 2).expand(N).contiguous()
+        self.b = self.a.clone()
+        self.rnm_log_consts = X.new_full((N,), -math.inf)
+        self.logdet_sigma_0 = X.new_empty(N)
+        self.logdet_W = X.new_empty(N)
+        self.sigma_0_lambda_mm_trace = X.new_empty(N)
+        self._alloc(X)
+
+    # lakde/local_knn_kde.py:195-207 (prior + its jittered Cholesky happen in
+    # the posterior kernel)
+    def _prior(self, X, b, nb):
+        N = X.size(0)
+        s, e, _ = self._span(b, N)
+        F.knn_prior_cov(X, s, self.knn_graph[s:e], self._prior_buf[:nb])
+        return PRIOR_KNN, self._prior_buf[:nb], self.sigma_0_jitter[s:e], self._prior_tril[:nb]
+
+    # lakde/local_knn_kde.py:140-193
+    def init_summaries_(self, X, sparsity_threshold=None):
+        N = X.size(0)
+        for b in self._blocks(N):
+            s, e, nb = self._span(b, N)
+            self._block_posterior(X, b, self.nu[s:e], self.logdet_W[s:e],
+                                  self.sigma_0_lambda_mm_trace[s:e], self.logdet_sigma_0[s:e],
+                                  update_b=False)
+            self._init_lse_pass(X, b)
+        self.rnm_active_contribs.zero_()
+        for b in self._blocks(N):
+            s, e, nb = self._span(b, N)
+            self._block_posterior(X, b, self._nu_s[:nb], self._logdet_s[:nb],
+                                  self._trace_s[:nb], self._logdet_prior_s[:nb], update_b=False)
+            self._init_fixed_pass(X, b, sparsity_threshold)
+        self._init_rebalance()
+
+    # lakde/local_knn_kde.py:209-239
+    def partial_step_(self, X, block_idx, sparsity_threshold=None):
+        N = X.size(0)
+        s, e, nb = self._span(block_idx, N)
+        # nu, logdet_W, logdet_sigma_0, the trace term and b are written in
+        # place; none of them is read by the E-step of this block
+        self._block_posterior(X, block_idx, self.nu[s:e], self.logdet_W[s:e],
+                              self.sigma_0_lambda_mm_trace[s:e], self.logdet_sigma_0[s:e],
+                              update_b=True)
+        self._update_step(X, block_idx, sparsity_threshold)
+
+    def state_dict(self):
+        d = super().state_dict()
+        d.update({"knn_graph": self.knn_graph, "k": self.k,
+                  "logdet_sigma_0": self.logdet_sigma_0,
+                  "sigma_0_lambda_mm_trace": self.sigma_0_lambda_mm_trace,
+                  "sigma_0_jitter": self.sigma_0_jitter})
+        return d
+
+    def load_state_dict(self, state_dict, X):
+        """Restore a state written by this class or by the reference
+        (lakde/local_knn_kde.py:343-367).  X (the training data, on the GPU)
+        is needed to size the device buffers."""
+        self._load_common(state_dict, X.device)
+        self.knn_graph = state_dict["knn_graph"].to(X.device).contiguous()
+        self.k = state_dict["k"]
+        for k in ("logdet_sigma_0", "sigma_0_lambda_mm_trace", "sigma_0_jitter"):
+            setattr(self, k, state_dict[k].to(X.device))
+        self._load_blocks(list(state_dict["rnm_blocks"]), X)
+        self._alloc(X)

@@ -1,0 +1,32 @@
+"""Initialisation helpers (device-side torch glue, init only)."""
+
+import torch
+
+
+def tensor_as(x, other):
+    """lakde/utils.py:8-11."""
+    if torch.is_tensor(x):
+        return x.to(dtype=other.dtype, device=other.device)
+    return torch.tensor(x, dtype=other.dtype, device=other.device)
+
+
+def cov(X, dtype=torch.float64):
+    """Sample covariance (lakde/utils.py:28-37)."""
+    N = X.size(-2)
+    X_bar = X.mean(dim=-2, dtype=dtype, keepdim=True)
+    Xc = X.to(dtype) - X_bar
+    return (Xc.mT @ Xc) / (N - 1)
+
+
+def compute_knn_graph(X, k, bsize=1024, include_self=False, dtype=torch.int32):
+    """k nearest neighbours by L2 distance, blocked over rows
+    (lakde/utils.py:77-116, the generic path).  Used only at initialisation
+    when the model is given an integer k."""
+    N = X.size(0)
+    out = torch.empty(N, k, dtype=dtype, device=X.device)
+    for i in range(0, N, bsize):
+        d = torch.cdist(X[i:i + bsize], X)
+        if not include_self:
+            torch.diagonal(d, i).fill_(float("inf"))
+        out[i:i + bsize] = d.topk(k, dim=1, largest=False, sorted=True)[1].to(dtype)
+    return out

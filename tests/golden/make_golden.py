@@ -1,0 +1,257 @@
+"""Write golden input/output vectors by running the reference lakde package
+(its pure-PyTorch path, LAKDE_NO_EXT=1, float32 on the CPU).
+
+Run from the repository root, in the development container that has the
+reference checkout:  python tests/golden/make_golden.py [--full]
+
+The reference's lakde/utils.py imports TensorBoard at module load for its
+logging writer; TensorBoard is not installed here, so two empty stand-in
+modules are registered for that import.  The models are built with
+``logs=False``, so the writer is never used and no computation goes through
+the stand-ins.
+
+Each case writes one compressed .npz next to this script holding its inputs,
+its settings and the reference outputs.  The inputs come from
+lakde_amd.synthetic (seeded), with small sizes so the oracle and the GPU path
+can replay them in seconds.
+"""
+
+import argparse
+import os
+import sys
+import time
+import types
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+os.environ["LAKDE_NO_EXT"] = "1"
+_tb = types.ModuleType("torch.utils.tensorboard")
+_tb_summary = types.ModuleType("torch.utils.tensorboard.summary")
+_tb_summary.hparams = None
+_tb_writer = types.ModuleType("torch.utils.tensorboard.writer")
+_tb_writer.SummaryWriter = object
+_tb.summary, _tb.writer = _tb_summary, _tb_writer
+sys.modules["torch.utils.tensorboard"] = _tb
+sys.modules["torch.utils.tensorboard.summary"] = _tb_summary
+sys.modules["torch.utils.tensorboard.writer"] = _tb_writer
+sys.path.insert(0, REF)
+
+import torch  # noqa: E402
+
+import lakde  # noqa: E402
+from lakde import kernels as ref_kernels  # noqa: E402
+from lakde import linalg_util as ref_linalg  # noqa: E402
+from lakde import utils as ref_utils  # noqa: E402
+from lakde_amd import synthetic  # noqa: E402
+
+torch.set_num_threads(8)
+
+
+def _save(name, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
+    print("wrote", path, "%.1f KB" % (os.path.getsize(path) / 1024))
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def case_ops():
+    g = torch.Generator().manual_seed(11)
+    out = {}
+    # chol_quad_form with a transposed (D^2, 1, D)-strided factor, as the models pass it
+    for D in (1, 2, 5, 9):
+        Nb, Mb = 7, 13
+        A = torch.randn(Nb, D, D, generator=g)
+        Winv = A @ A.mT + D * torch.eye(D)
+        W_triu = ref_linalg.tril_inverse(torch.linalg.cholesky(Winv)).mT
+        Xn = torch.randn(Nb, D, generator=g)
+        Xm = torch.randn(Mb, D, generator=g)
+        Q = ref_kernels.chol_quad_form(W_triu, Xn, Xm)
+        out[f"qf{D}_Wtriu"] = _np(W_triu.contiguous())
+        out[f"qf{D}_Xn"], out[f"qf{D}_Xm"], out[f"qf{D}_Q"] = _np(Xn), _np(Xm), _np(Q)
+    # calc_W_inv on a CSR with empty rows
+    N, Mx, D = 9, 40, 4
+    rows = torch.sort(torch.randint(0, N, (60,), generator=g)).values
+    rows = rows[rows != 3]  # row 3 empty
+    col = torch.randint(0, Mx, (rows.numel(),), generator=g, dtype=torch.int64).to(torch.int32)
+    val = torch.rand(rows.numel(), generator=g)
+    Xn = torch.randn(N, D, generator=g)
+    Xm = torch.randn(Mx, D, generator=g)
+    rowptr = ref_kernels.ind2ptr(rows.to(torch.int32), N)
+    S = ref_kernels.calc_W_inv(rowptr, col, val, Xn, Xm)
+    out.update(winv_row=_np(rows.to(torch.int32)), winv_rowptr=_np(rowptr), winv_col=_np(col),
+               winv_val=_np(val), winv_Xn=_np(Xn), winv_Xm=_np(Xm), winv_S=_np(S))
+    # ind2ptr with leading/trailing/inner gaps
+    r = torch.tensor([2, 2, 2, 5, 5, 6, 9, 9, 9, 9], dtype=torch.int32)
+    out["i2p_row"], out["i2p_ptr"] = _np(r), _np(ref_kernels.ind2ptr(r, 12))
+    # special functions and log-sum-exp helpers
+    a = torch.tensor([3.5, 7.25, 40.0, 130.5, 1000.0])
+    for D in (2, 6, 21, 63):
+        out[f"mvdigamma{D}_a"] = _np(a + D)
+        out[f"mvdigamma{D}"] = _np(ref_linalg.mvdigamma((a + D) / 2, D))
+        out[f"mvlgamma{D}"] = _np(ref_linalg.mvlgamma((a + D) / 2, D))
+    x = torch.tensor([0.0, -1.0, -5.0, 3.0, -40.0, -np.inf])
+    y = torch.tensor([-1.0, -1.0, -10.0, 2.999, -39.0, -np.inf])
+    out["lse_a"], out["lse_b"], out["logsubexp"] = _np(x), _np(y), _np(ref_linalg.logsubexp(x, y))
+    # stabilized_cholesky on matrices whose factor/fail decision does not
+    # depend on rounding: PD, exact zero pivots (integer entries, exact
+    # elimination), zero matrix, negative pivot, scaled rank-1 of integers
+    M = torch.zeros(6, 5, 5)
+    M[0] = torch.randn(5, 5, generator=g) @ torch.randn(5, 5, generator=g).mT * 0 + 5 * torch.eye(5)
+    M[0, 0, 1] = M[0, 1, 0] = 1.0
+    M[1] = torch.diag(torch.tensor([2.0, 0.0, 3.0, 1.0, 4.0]))
+    ones = torch.ones(3, 3)
+    M[2, :3, :3] = ones                       # pivot 2 eliminates to exactly 0
+    M[2, 3, 3] = M[2, 4, 4] = 1.0
+    M[3] = 0
+    M[4] = torch.diag(torch.tensor([1.0, 1.0, -3 * 2.0 ** -23, 1.0, 1.0]))
+    M[5] = torch.diag(torch.tensor([1.0, 1.0, 1.0, 1.0, -1e-3]))  # needs k = 13
+    jitter0 = torch.zeros(6)
+    L, jit = ref_linalg.stabilized_cholesky(M.clone(), jitter0.clone())
+    out.update(stab_X=_np(M), stab_L=_np(L), stab_jitter=_np(jit))
+    jit_in = torch.tensor([0.0, 1e-3, 0.0, 0.0, 0.0, 0.0])  # persistent jitter reused
+    L2, jit2 = ref_linalg.stabilized_cholesky(M.clone(), jit_in.clone())
+    out.update(stab2_jitter_in=_np(jit_in), stab2_L=_np(L2), stab2_jitter=_np(jit2))
+    # batched kNN covariance
+    Xk = torch.randn(50, 3, generator=g)
+    graph = ref_utils.compute_knn_graph(Xk, 6, 1024, include_self=False, dtype=torch.int32)
+    covk = ref_utils.batched_knn_cov(Xk, Xk[:20], graph[:20].long(), include_self=False,
+                                     self_center=True)
+    out.update(knn_X=_np(Xk), knn_graph=_np(graph), knn_cov=_np(covk))
+    _save("ops", **out)
+
+
+def _fit_case(name, model_cls, X, Y, graph, block_size, threshold, iterations,
+              store_rnm=True, **kw):
+    Xt = torch.from_numpy(X)
+    Yt = torch.from_numpy(Y)
+    m = model_cls(k_or_knn_graph=torch.from_numpy(graph), block_size=block_size,
+                  verbose=False, logs=False, **kw)
+    t0 = time.time()
+    # the body of AbstractKDE.fit (lakde/kdes.py:46-67) with callbacks=[],
+    # unrolled so the state after init and after every sweep can be recorded
+    m.init_parameters_(Xt, threshold)
+    m.init_summaries_(Xt, threshold)
+    traj = {"init_log_consts": _np(m.rnm_log_consts).copy(), "init_nu": _np(m.nu).copy()}
+    num_blocks = (Xt.size(0) - 1) // m.block_size + 1
+    lc, nus = [], []
+    for _ in range(iterations):
+        for bi in range(num_blocks):
+            m.partial_step_(Xt, bi, sparsity_threshold=threshold)
+            m.iter_steps += 1
+        lc.append(_np(m.rnm_log_consts).copy())
+        nus.append(_np(m.nu).copy())
+    t_fit = time.time() - t0
+    t0 = time.time()
+    lp = m.log_pred_density(Xt, Yt)
+    t_pred = time.time() - t0
+    blocks = list(m.rnm_blocks)
+    rows = np.concatenate([_np(b[0][0]).astype(np.int64) + i * m.block_size
+                           for i, b in enumerate(blocks)])
+    cols = np.concatenate([_np(b[0][1]) for b in blocks]).astype(np.int64)
+    vals = np.concatenate([_np(b[1]) for b in blocks])
+    order = np.lexsort((cols, rows))
+    extra = {}
+    if hasattr(m, "sigma_0_jitter"):
+        extra["sigma_0_jitter"] = _np(m.sigma_0_jitter)
+        extra["logdet_sigma_0"] = _np(m.logdet_sigma_0)
+    if hasattr(m, "W_shared_triu"):
+        extra["W_shared_triu"] = _np(m.W_shared_triu)
+        extra["W_shared_inv_jitter"] = _np(m.W_shared_inv_jitter)
+    _save(name, X=X, Y=Y, graph=graph, block_size=block_size, threshold=threshold,
+          iterations=iterations, log_pred=_np(lp), log_consts=_np(m.rnm_log_consts),
+          nu=_np(m.nu), logdet_W=_np(m.logdet_W), b=_np(m.b), W_inv_jitter=_np(m.W_inv_jitter),
+          traj_log_consts=np.stack(lc) if lc else np.zeros((0, X.shape[0]), np.float32),
+          traj_nu=np.stack(nus) if nus else np.zeros((0, X.shape[0]), np.float32),
+          time_fit_s=t_fit, time_pred_s=t_pred, rnm_nnz=rows.size,
+          rnm_colsum=np.bincount(cols, weights=vals, minlength=X.shape[0]),
+          **(dict(rnm_row=rows[order].astype(np.int32), rnm_col=cols[order].astype(np.int32),
+                  rnm_val=vals[order]) if store_rnm else {}),
+          **traj, **extra)
+    print(f"  {name}: fit {t_fit:.1f}s pred {t_pred:.1f}s mean log p {float(lp.mean()):.6f}")
+
+
+def _ref_graph(X, k, include_self):
+    return _np(ref_utils.compute_knn_graph(torch.from_numpy(X), k, 1024,
+                                           include_self=include_self, dtype=torch.int32))
+
+
+def case_knn_circle_small():
+    X, Y = synthetic.make("circle8", 1000, 200)
+    g = _ref_graph(X, 20, False)
+    _fit_case("knn_circle8_small", lakde.LocalKNearestKDE, X, Y, g, 256, 0.0, 4)
+
+
+def _mix6_small():
+    X, Y = synthetic.make("mix32_d6", 2000, 300)
+    # coarser quantisation of coordinate 1 so that many 64-NN sets share one level
+    for A in (X, Y):
+        A[:, 1] = np.round(A[:, 1] * 1.5) / 1.5
+    return X, Y
+
+
+def case_knn_mix6_small():
+    X, Y = _mix6_small()
+    g = _ref_graph(X, 64, False)
+    _fit_case("knn_mix6_small", lakde.LocalKNearestKDE, X, Y, g, 512, 1 / 64, 4)
+
+
+def case_knn_degenerate_small():
+    # coordinate 1 collapsed onto levels 10 apart: every 32-NN set shares one
+    # level, so the kNN covariances are singular and the jitter path runs
+    X, Y = synthetic.make("mix32_d6", 1500, 200)
+    for A in (X, Y):
+        A[:, 1] = 10 * np.round(A[:, 1] / 2)
+    g = _ref_graph(X, 32, False)
+    _fit_case("knn_degenerate_small", lakde.LocalKNearestKDE, X, Y, g, 512, 1 / 32, 3)
+
+
+def case_hier_latent_small():
+    X, Y = synthetic.make("latent_d43", 1200, 150)
+    g = _ref_graph(X, 32, True)
+    _fit_case("hier_latent_small", lakde.LocalHierarchicalKDE, X, Y, g, 512, 1 / 32, 3)
+
+
+def case_hier_mix6_small():
+    X, Y = synthetic.make("mix32_d6", 1500, 200)
+    g = _ref_graph(X, 32, True)
+    _fit_case("hier_mix6_small", lakde.LocalHierarchicalKDE, X, Y, g, 512, 1 / 32, 4)
+
+
+def case_hier_smooth_small():
+    # the latent-factor data without its two integer columns: well conditioned
+    X, Y = synthetic.make("latent_d43", 1000, 150)
+    X, Y = [np.ascontiguousarray(np.delete(A, [3, 17], axis=1)[:, :12]) for A in (X, Y)]
+    g = _ref_graph(X, 24, True)
+    _fit_case("hier_smooth_small", lakde.LocalHierarchicalKDE, X, Y, g, 256, 1 / 24, 4)
+
+
+def case_knn_circle_full():
+    w = synthetic.WORKLOADS["circle8"]
+    X, Y = synthetic.make("circle8")
+    g = _ref_graph(X, w["k"], False)
+    _fit_case("knn_circle8_full", lakde.LocalKNearestKDE, X, Y, g, 512, 0.0, w["iterations"],
+              store_rnm=False)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--full", action="store_true", help="also run the 5k-point circle fit")
+    ap.add_argument("--only", default=None)
+    args = ap.parse_args()
+    cases = [case_ops, case_knn_circle_small, case_knn_mix6_small, case_knn_degenerate_small,
+             case_hier_latent_small, case_hier_mix6_small, case_hier_smooth_small]
+    if args.full:
+        cases.append(case_knn_circle_full)
+    for c in cases:
+        if args.only and args.only not in c.__name__:
+            continue
+        print(c.__name__)
+        c()

@@ -1,0 +1,266 @@
+"""GPU parity of the reference ops and the fused kernels against the CPU
+oracle (oracle/lakde_np.py) and the reference's golden vectors.
+
+Bars: bit-exact for index work (ind2ptr, CSR indices); float32 results within
+the tolerances written next to each assertion, measured against float64.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import lakde_np as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a, dtype=None):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def K():
+    from lakde_amd import kernels
+    return kernels
+
+
+# ------------------------------------------------------------ ind2ptr
+@pytest.mark.parametrize("in_dt,out_dt", [(torch.int32, torch.int32), (torch.int32, torch.int64),
+                                          (torch.int64, torch.int32), (torch.int64, torch.int64)])
+def test_ind2ptr_bit_exact(K, in_dt, out_dt):
+    rng = np.random.default_rng(1)
+    for M, nnz in [(1, 0), (5, 0), (7, 1), (513, 40_000), (3000, 17)]:
+        row = np.sort(rng.integers(0, M, size=nnz))
+        ind = _t(row, in_dt)
+        out = torch.empty(M + 1, dtype=out_dt, device=DEV)
+        K.ind2ptr(ind, M, out=out)
+        np.testing.assert_array_equal(_np(out), O.ind2ptr(row, M))
+    r = _t(np.array([2, 2, 2, 5, 5, 6, 9, 9, 9, 9]), in_dt)
+    out = K.ind2ptr(r, 12)
+    assert out.dtype == in_dt
+    np.testing.assert_array_equal(_np(out), [0, 0, 0, 3, 3, 3, 5, 6, 6, 6, 10, 10, 10])
+
+
+def test_ind2ptr_golden(K, golden):
+    g = golden("ops")
+    np.testing.assert_array_equal(_np(K.ind2ptr(_t(g["i2p_row"]), 12)), g["i2p_ptr"])
+
+
+# ------------------------------------------------------------ W_inv
+@pytest.mark.parametrize("fp", [torch.float32, torch.float64])
+@pytest.mark.parametrize("D", [1, 2, 6, 21, 43, 63, 80])
+def test_w_inv_matches_oracle(K, fp, D):
+    rng = np.random.default_rng(D)
+    N, M = 37, 300
+    counts = rng.integers(0, 90, size=N)
+    counts[3] = 0
+    counts[10] = 700  # hub row, several staging chunks
+    rowptr = np.concatenate([[0], np.cumsum(counts)])
+    col = rng.integers(0, M, size=rowptr[-1])
+    val = rng.random(rowptr[-1])
+    Xn, Xm = rng.standard_normal((N, D)), rng.standard_normal((M, D))
+    ref = O.calc_W_inv(rowptr, col, val, Xn, Xm, dt=np.float64)
+    for idt in (torch.int32, torch.int64):
+        out = K.calc_W_inv(_t(rowptr, idt), _t(col, idt), _t(val, fp), _t(Xn, fp), _t(Xm, fp))
+        tol = 2e-5 if fp == torch.float32 else 1e-12
+        scale = np.abs(ref).max(axis=(1, 2), keepdims=True) + 1e-30
+        assert np.max(np.abs(_np(out) - ref) / scale) < tol
+        np.testing.assert_array_equal(_np(out), np.swapaxes(_np(out), 1, 2))
+    assert np.all(_np(out)[3] == 0)
+
+
+def test_w_inv_golden(K, golden):
+    g = golden("ops")
+    out = K.calc_W_inv(_t(g["winv_rowptr"]), _t(g["winv_col"]), _t(g["winv_val"]),
+                       _t(g["winv_Xn"]), _t(g["winv_Xm"]))
+    np.testing.assert_allclose(_np(out), g["winv_S"], rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------ chol_quad_form
+def _factors(rng, N, D):
+    A = rng.standard_normal((N, D, D))
+    Winv = A @ np.swapaxes(A, 1, 2) + D * np.eye(D)
+    L = np.linalg.cholesky(Winv)
+    Linv = np.linalg.inv(L)
+    return np.swapaxes(Linv, 1, 2)  # W_triu (upper)
+
+
+@pytest.mark.parametrize("fp", [torch.float32, torch.float64])
+@pytest.mark.parametrize("D", [1, 2, 5, 6, 9, 17, 33, 63, 70])
+def test_chol_quad_form_matches_oracle(K, fp, D):
+    rng = np.random.default_rng(100 + D)
+    N, M = 23, 517
+    Wt = _factors(rng, N, D)
+    Xn, Xm = rng.standard_normal((N, D)), rng.standard_normal((M, D))
+    ref = O.chol_quad_form(Wt, Xn, Xm, dt=np.float64)
+    # the models pass a transposed view of the lower inverse factor
+    W_dev = _t(np.swapaxes(Wt, 1, 2), fp).mT
+    assert not W_dev.is_contiguous()
+    Q = _np(K.chol_quad_form(W_dev, _t(Xn, fp), _t(Xm, fp)))
+    tol = 3e-5 if fp == torch.float32 else 1e-12
+    np.testing.assert_allclose(Q, ref, rtol=tol, atol=tol * ref.mean())
+
+
+def test_chol_quad_form_stride0_and_views(K):
+    rng = np.random.default_rng(3)
+    D, N, M = 4, 9, 50
+    Wt = _factors(rng, 1, D)[0]
+    Xn, Xm = rng.standard_normal((N, D)), rng.standard_normal((M, D))
+    W_dev = _t(Wt).expand(N, D, D)            # stride-0 batch, as the shared models pass it
+    Xm_dev = _t(np.asfortranarray(Xm))        # column-major points
+    Xm_view = torch.as_tensor(Xm.T.copy(), device=DEV, dtype=torch.float32).T
+    Q1 = _np(K.chol_quad_form(W_dev, _t(Xn), Xm_view))
+    Q2 = _np(K.chol_quad_form(W_dev.contiguous(), _t(Xn), Xm_dev.contiguous()))
+    np.testing.assert_array_equal(Q1, Q2)
+    ref = O.chol_quad_form(np.broadcast_to(Wt, (N, D, D)), Xn, Xm, dt=np.float64)
+    np.testing.assert_allclose(Q1, ref, rtol=3e-5)
+    # identity factor -> squared distances; Q = 0 on coincident points
+    I = torch.eye(D, device=DEV).expand(N, D, D)
+    Q = _np(K.chol_quad_form(I, _t(Xn), _t(Xn)))
+    np.testing.assert_allclose(Q, np.sum((Xn[None] - Xn[:, None]) ** 2, -1), rtol=1e-5, atol=1e-6)
+    assert np.all(np.diagonal(Q) == 0)
+
+
+def test_chol_quad_form_golden(K, golden):
+    g = golden("ops")
+    for D in (1, 2, 5, 9):
+        Q = K.chol_quad_form(_t(g[f"qf{D}_Wtriu"]), _t(g[f"qf{D}_Xn"]), _t(g[f"qf{D}_Xm"]))
+        np.testing.assert_allclose(_np(Q), g[f"qf{D}_Q"], rtol=2e-5, atol=1e-6)
+
+
+def test_torch_ops_registered(K):
+    rng = np.random.default_rng(4)
+    r = _t(np.sort(rng.integers(0, 9, 30)), torch.int32)
+    out = torch.empty(10, dtype=torch.int32, device=DEV)
+    torch.ops.lakde.ind2ptr_cuda(r, 9, out)
+    np.testing.assert_array_equal(_np(out), O.ind2ptr(_np(r), 9))
+    Wt = _t(_factors(rng, 3, 2))
+    Xn, Xm = _t(rng.standard_normal((3, 2))), _t(rng.standard_normal((5, 2)))
+    q = torch.empty(3, 5, device=DEV)
+    torch.ops.lakde.chol_quad_form_cuda(Wt, Xn, Xm, q)
+    np.testing.assert_allclose(_np(q), _np(K.chol_quad_form(Wt, Xn, Xm)))
+
+
+# ------------------------------------------------------- fused pieces
+def test_knn_prior_cov_matches_oracle():
+    from lakde_amd import _fused as F
+    g = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__),
+                                           "golden", "ops.npz"))
+    X = g["knn_X"]
+    graph = g["knn_graph"]
+    out = torch.empty(20, 3, 3, device=DEV)
+    F.knn_prior_cov(_t(X), 0, _t(graph[:20]), out)
+    np.testing.assert_allclose(_np(out), g["knn_cov"], rtol=1e-5, atol=1e-6)
+    out2 = torch.empty(10, 3, 3, device=DEV)
+    F.knn_prior_cov(_t(X), 20, _t(graph[20:30], torch.int64), out2)
+    np.testing.assert_allclose(_np(out2), O.batched_knn_cov(X, X[20:30], graph[20:30],
+                                                           np.float32), rtol=1e-5, atol=1e-6)
+
+
+def _run_posterior(prior, S, nu, a, b, nu0, Xn, jit_w, jit_p=None, shared=False):
+    from lakde_amd import _fused as F
+    from lakde_amd._lib import PRIOR_KNN, PRIOR_SHARED
+    nb, D = Xn.shape
+    R = F.record_floats(D)
+    outs = dict(Linv=torch.empty(nb, D, D, device=DEV), logdet=torch.empty(nb, device=DEV),
+                trace=torch.empty(nb, device=DEV), rec=torch.zeros(nb, R, device=DEV),
+                status=torch.zeros(1, dtype=torch.int32, device=DEV))
+    jw = _t(jit_w)
+    if shared:
+        F.posterior(PRIOR_SHARED, _t(prior), None, _t(S), _t(nu), _t(a), _t(b), nu0, jw, _t(Xn),
+                    outs["Linv"], None, outs["logdet"], None, outs["trace"], outs["rec"],
+                    outs["status"])
+    else:
+        jp = _t(jit_p)
+        outs["ptril"] = torch.empty(nb, D, D, device=DEV)
+        outs["ldp"] = torch.empty(nb, device=DEV)
+        F.posterior(PRIOR_KNN, _t(prior), jp, _t(S), _t(nu), _t(a), _t(b), nu0, jw, _t(Xn),
+                    outs["Linv"], outs["ptril"], outs["logdet"], outs["ldp"], outs["trace"],
+                    outs["rec"], outs["status"])
+        outs["jit_p"] = jp
+    outs["jit_w"] = jw
+    return {k: _np(v) for k, v in outs.items()}
+
+
+def test_posterior_kernel_matches_oracle():
+    """Factorisation, jitter schedule, inverse, logdet, trace and the record
+    constants of the posterior kernel against the oracle's float32 path."""
+    rng = np.random.default_rng(11)
+    nb, D = 40, 6
+    A = rng.standard_normal((nb, D, 3)).astype(np.float32)
+    prior = (A @ np.swapaxes(A, 1, 2)).astype(np.float32)      # rank 3: prior needs jitter
+    prior[:10] += np.eye(D, dtype=np.float32)                   # a few full rank
+    B = rng.standard_normal((nb, D, D)).astype(np.float32)
+    S = (B @ np.swapaxes(B, 1, 2) * 0.1).astype(np.float32)
+    S[5] = 0                                                    # W_inv = prior only
+    nu = (D + rng.random(nb) * 30).astype(np.float32)
+    a = np.full(nb, 3.0, np.float32)
+    b = (1 + rng.random(nb)).astype(np.float32)
+    Xn = rng.standard_normal((nb, D)).astype(np.float32)
+    jp0 = np.zeros(nb, np.float32)
+    jw0 = np.zeros(nb, np.float32)
+    nu0 = np.float32(D)
+    out = _run_posterior(prior, S, nu, a, b, nu0, Xn, jw0, jp0)
+    assert out["status"][0] == 0
+    # oracle, float32 decisions
+    Lp, jp = O.stabilized_cholesky(prior, jp0.copy())
+    np.testing.assert_array_equal(out["jit_p"], jp)
+    prior_j = O._add_diag(prior, jp)
+    scale = (nu0 * (a / b).astype(np.float32)).astype(np.float32)
+    Winv = (prior_j * scale[:, None, None]).astype(np.float32) + S
+    L, jw = O.stabilized_cholesky(Winv, jw0.copy())
+    np.testing.assert_array_equal(out["jit_w"], jw)
+    # compare in float64 through the matrices the factors represent
+    Winv_j = O._add_diag(Winv.astype(np.float64), jw.astype(np.float64))
+    Linv = out["Linv"].astype(np.float64)
+    W_dev = np.swapaxes(Linv, 1, 2) @ Linv          # = W (inverse of W_inv + jI)
+    W_ref = np.linalg.inv(Winv_j)
+    rel = np.abs(W_dev - W_ref).max(axis=(1, 2)) / np.abs(W_ref).max(axis=(1, 2))
+    assert rel.max() < 1e-3
+    logdet_ref = -np.linalg.slogdet(Winv_j)[1]
+    np.testing.assert_allclose(out["logdet"], logdet_ref, rtol=1e-4, atol=1e-3)
+    tr_ref = nu * O.chol_mm_trace(Lp.astype(np.float64), np.swapaxes(Linv, 1, 2))
+    np.testing.assert_allclose(out["trace"], tr_ref, rtol=1e-4)
+    # record constants
+    T = D * (D + 1) // 2
+    rec = out["rec"]
+    np.testing.assert_array_equal(rec[:, T:T + D], Xn)
+    eld = O.mvdigamma(nu / 2, D) + np.float32(D * math.log(2)) + out["logdet"]
+    np.testing.assert_allclose(rec[:, T + D], 0.5 * eld, rtol=1e-5, atol=1e-5)
+    np.testing.assert_array_equal(rec[:, T + D + 1], -0.5 * nu)
+    pc = (O.lgamma((nu + 1) / 2) - O.lgamma((nu + 1 - D) / 2) + 0.5 * out["logdet"]
+          - np.float32(D / 2 * math.log(math.pi)))
+    np.testing.assert_allclose(rec[:, T + D + 2], pc, rtol=1e-5, atol=1e-5)
+    # persistent jitter: a second call with the stored jitters keeps them
+    out2 = _run_posterior(prior, S, nu, a, b, nu0, Xn, out["jit_w"], out["jit_p"])
+    np.testing.assert_array_equal(out2["jit_w"], out["jit_w"])
+    np.testing.assert_array_equal(out2["jit_p"], out["jit_p"])
+
+
+def test_posterior_jitter_schedule_exact():
+    """Rounding-independent jitter decisions: exact zero pivot -> eps;
+    a -3 eps pivot -> 7 eps; a -1e-3 pivot -> (2^14 - 1) eps."""
+    D, nb = 3, 3
+    eps = np.finfo(np.float32).eps
+    prior = np.zeros((nb, D, D), np.float32)
+    prior[:] = np.eye(D, dtype=np.float32)
+    S = np.zeros((nb, D, D), np.float32)
+    Winv_diag = np.array([[1, 0, 2], [1, -3 * eps, 2], [1, 1, -1e-3]], np.float32)
+    # W_inv = prior * scale + S with scale = 1 (nu0 = 1, a = b) -> set S = diag - I
+    for i in range(nb):
+        S[i] = np.diag(Winv_diag[i]) - np.eye(D, dtype=np.float32)
+    ones = np.ones(nb, np.float32)
+    out = _run_posterior(prior, S, ones, ones, ones, np.float32(1.0),
+                         np.zeros((nb, D), np.float32), np.zeros(nb, np.float32),
+                         np.zeros(nb, np.float32))
+    np.testing.assert_array_equal(out["jit_p"], 0)
+    np.testing.assert_array_equal(out["jit_w"], np.array([eps, 7 * eps, (2 ** 14 - 1) * eps],
+                                                         np.float32))

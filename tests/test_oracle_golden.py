@@ -1,0 +1,222 @@
+"""Pin the CPU oracle against the reference's own outputs (golden vectors
+written by tests/golden/make_golden.py) and against closed forms."""
+
+import math
+
+import numpy as np
+import pytest
+from scipy import special, stats
+
+from oracle import lakde_np as O
+
+
+# ------------------------------------------------------------- golden ops
+
+def test_ops_quad_form_golden(golden):
+    g = golden("ops")
+    for D in (1, 2, 5, 9):
+        Q = O.chol_quad_form(g[f"qf{D}_Wtriu"], g[f"qf{D}_Xn"], g[f"qf{D}_Xm"])
+        np.testing.assert_allclose(Q, g[f"qf{D}_Q"], rtol=2e-5, atol=1e-6)
+
+
+def test_ops_w_inv_ind2ptr_golden(golden):
+    g = golden("ops")
+    rowptr = O.ind2ptr(g["winv_row"], g["winv_Xn"].shape[0])
+    np.testing.assert_array_equal(rowptr, g["winv_rowptr"])
+    S = O.calc_W_inv(rowptr, g["winv_col"], g["winv_val"], g["winv_Xn"], g["winv_Xm"])
+    np.testing.assert_allclose(S, g["winv_S"], rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(O.ind2ptr(g["i2p_row"], 12), g["i2p_ptr"])
+
+
+def test_ops_special_golden(golden):
+    g = golden("ops")
+    for D in (2, 6, 21, 63):
+        a = g[f"mvdigamma{D}_a"] / 2
+        np.testing.assert_allclose(O.mvdigamma(a, D), g[f"mvdigamma{D}"], rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(O.mvlgamma(a, D), g[f"mvlgamma{D}"], rtol=1e-5, atol=1e-3)
+    # 3 - 2.999 cancels: log1p(-exp(-1e-3)) carries ~1e-4 relative FP32 error
+    np.testing.assert_allclose(O.logsubexp(g["lse_a"], g["lse_b"]), g["logsubexp"], rtol=1e-4)
+
+
+def test_stabilized_cholesky_golden(golden):
+    g = golden("ops")
+    # the jitter decisions are exact; factors are compared through L L^T
+    # because a jittered zero pivot is sqrt(~eps) and carries O(1) relative
+    # rounding in any FP32 implementation
+    X = g["stab_X"]
+    eye = np.eye(5, dtype=np.float32)
+    for jin, jref, Lref in ((np.zeros(6, np.float32), g["stab_jitter"], g["stab_L"]),
+                            (g["stab2_jitter_in"], g["stab2_jitter"], g["stab2_L"])):
+        L, jit = O.stabilized_cholesky(X.copy(), jin.copy())
+        np.testing.assert_array_equal(jit, jref)
+        target = X + jref[:, None, None] * eye
+        np.testing.assert_allclose(L @ np.swapaxes(L, 1, 2), target, atol=2e-6)
+        np.testing.assert_allclose(Lref @ np.swapaxes(Lref, 1, 2), target, atol=2e-6)
+
+
+def test_knn_graph_and_cov_golden(golden):
+    g = golden("ops")
+    graph = O.compute_knn_graph(g["knn_X"], 6, include_self=False)
+    np.testing.assert_array_equal(graph, g["knn_graph"])
+    cov = O.batched_knn_cov(g["knn_X"], g["knn_X"][:20], graph[:20], np.float32)
+    np.testing.assert_allclose(cov, g["knn_cov"], rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------ closed-form pins
+
+def test_quad_form_closed_form():
+    rng = np.random.default_rng(5)
+    D, Nb, Mb = 4, 5, 7
+    A = rng.standard_normal((Nb, D, D))
+    Winv = A @ A.transpose(0, 2, 1) + np.eye(D)
+    W = np.linalg.inv(Winv)
+    U = np.linalg.cholesky(W)                 # W = U U^T with U lower here
+    Ut = np.swapaxes(U, 1, 2)                 # pass an upper factor: W = Ut^T Ut ...
+    Xn, Xm = rng.standard_normal((Nb, D)), rng.standard_normal((Mb, D))
+    # reference convention: W = W_triu W_triu^T with W_triu upper -> use the
+    # upper Cholesky of W, i.e. W_triu = chol(W[::-1,::-1])-style; simplest:
+    # any W_triu with W_triu W_triu^T = W works for the closed form
+    W_triu = U
+    Q = O.chol_quad_form(W_triu, Xn, Xm, dt=np.float64)
+    delta = Xm[None] - Xn[:, None]
+    ref = np.einsum("nmi,nij,nmj->nm", delta, W, delta)
+    np.testing.assert_allclose(Q, ref, rtol=1e-12)
+    # identity factor gives the squared Euclidean distance
+    I = np.broadcast_to(np.eye(D), (Nb, D, D))
+    np.testing.assert_allclose(O.chol_quad_form(I, Xn, Xm, dt=np.float64),
+                               np.sum(delta ** 2, -1), rtol=1e-12)
+    del Ut
+
+
+def test_w_inv_rank_one_and_psd():
+    rng = np.random.default_rng(6)
+    Xn, Xm = rng.standard_normal((3, 3)), rng.standard_normal((10, 3))
+    rowptr = np.array([0, 1, 1, 9])
+    col = np.array([4, 0, 1, 2, 3, 5, 6, 7, 8])
+    val = rng.random(9)
+    S = O.calc_W_inv(rowptr, col, val, Xn, Xm, dt=np.float64)
+    d = Xm[4] - Xn[0]
+    np.testing.assert_allclose(S[0], val[0] * np.outer(d, d), rtol=1e-12)
+    np.testing.assert_array_equal(S[1], 0)
+    assert np.all(np.linalg.eigvalsh(S[2]) > -1e-12)
+
+
+def test_ind2ptr_bruteforce():
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        M = int(rng.integers(1, 30))
+        row = np.sort(rng.integers(0, M, size=int(rng.integers(0, 60)))).astype(np.int32)
+        ptr = O.ind2ptr(row, M)
+        ref = np.array([np.sum(row < i) for i in range(M + 1)], np.int32)
+        np.testing.assert_array_equal(ptr, ref)
+
+
+def test_special_functions_closed_form():
+    a = np.array([5.5, 17.0, 300.25])
+    for D in (1, 3, 8):
+        ref = sum(special.digamma(a + (1 - i) / 2) for i in range(1, D + 1))
+        np.testing.assert_allclose(O.mvdigamma(a, D), ref, rtol=1e-12)
+        np.testing.assert_allclose(O.mvlgamma(a, D), special.multigammaln(a, D), rtol=1e-12)
+
+
+def test_jitter_schedule_closed_form():
+    """A rank-deficient PSD matrix with zero pivot factors at the first
+    jitter, eps; a negative-definite direction of size -3 eps needs
+    (2^{k+1}-1) eps > 3 eps, i.e. k = 1 -> 3 eps fails, k = 2 -> 7 eps."""
+    eps = np.finfo(np.float32).eps
+    X = np.zeros((2, 3, 3), np.float32)
+    X[0] = np.diag([1.0, 0.0, 2.0])
+    X[1] = np.diag([1.0, -3 * eps, 2.0])
+    _, jit = O.stabilized_cholesky(X, np.zeros(2, np.float32))
+    np.testing.assert_array_equal(jit, np.array([eps, 7 * eps], np.float32))
+
+
+def test_predictive_is_student_t_mixture():
+    """log_pred_density = LSE_n log St(y | x_n, ((nu+1-D) W_n)^-1, nu+1-D) - ln N
+    for each posterior kernel (lakde/local_adaptive_kde.py:200-223)."""
+    rng = np.random.default_rng(8)
+    X, Y = rng.standard_normal((40, 2)), rng.standard_normal((6, 2))
+    m = O.LocalKNearestKDE(k_or_knn_graph=5, block_size=16, dtype=np.float64)
+    m.fit(X, iterations=2)
+    lp = m.log_pred_density(X, Y)
+    D = 2
+    comps = []
+    for bi, i in enumerate(range(0, 40, 16)):
+        contrib, nu = m.calc_posterior_contribs(np.asarray(X, np.float64), bi)
+        s0, _ = m.calc_prior_cov(np.asarray(X, np.float64), bi)
+        U = m.calc_posterior(bi, s0, contrib)
+        for n in range(U.shape[0]):
+            W = U[n] @ U[n].T
+            df = nu[n] + 1 - D
+            comps.append(stats.multivariate_t(loc=X[i + n], shape=np.linalg.inv(df * W),
+                                              df=df).logpdf(Y))
+    ref = special.logsumexp(np.stack(comps), axis=0) - math.log(40)
+    np.testing.assert_allclose(lp, ref, rtol=1e-10)
+
+
+def test_predictive_integrates_to_one():
+    """The fitted 2-D density integrates to 1 (the normalisation check of
+    experiments/utils.py:35-45)."""
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((60, 2)).astype(np.float64)
+    m = O.LocalKNearestKDE(k_or_knn_graph=8, block_size=32, dtype=np.float64)
+    m.fit(X, iterations=2)
+    g = np.linspace(-9, 9, 241)
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    p = np.exp(m.log_pred_density(X, np.stack([gx.ravel(), gy.ravel()], 1))).reshape(gx.shape)
+    total = np.trapezoid(np.trapezoid(p, g, axis=1), g)
+    assert abs(total - 1) < 2e-2
+
+
+# -------------------------------------------------- golden fit trajectories
+
+def _fit_oracle(d, cls):
+    m = cls(k_or_knn_graph=d["graph"], block_size=int(d["block_size"]))
+    m.fit(d["X"], iterations=int(d["iterations"]), sparsity_threshold=float(d["threshold"]))
+    return m, m.log_pred_density(d["X"], d["Y"])
+
+
+def _csr_pairs(m):
+    rows = np.concatenate([b[0][0].astype(np.int64) + i * m.block_size
+                           for i, b in enumerate(m.rnm_blocks)])
+    cols = np.concatenate([b[0][1] for b in m.rnm_blocks]).astype(np.int64)
+    vals = np.concatenate([b[1] for b in m.rnm_blocks])
+    return rows, cols, vals
+
+
+@pytest.mark.parametrize("name", ["knn_circle8_small", "knn_mix6_small", "knn_degenerate_small"])
+def test_knn_fit_matches_reference(golden, name):
+    d = golden(name)
+    m, lp = _fit_oracle(d, O.LocalKNearestKDE)
+    np.testing.assert_allclose(lp, d["log_pred"], rtol=1e-5, atol=1e-4)
+    assert abs(lp.mean() - d["log_pred"].mean()) <= 1e-4 * abs(d["log_pred"].mean())
+    np.testing.assert_allclose(m.rnm_log_consts, d["log_consts"], rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(m.nu, d["nu"], rtol=1e-5)
+    np.testing.assert_array_equal(m.W_inv_jitter, d["W_inv_jitter"])
+    np.testing.assert_array_equal(m.sigma_0_jitter, d["sigma_0_jitter"])
+    rows, cols, vals = _csr_pairs(m)
+    np.testing.assert_array_equal(rows, d["rnm_row"])
+    np.testing.assert_array_equal(cols, d["rnm_col"])
+    np.testing.assert_allclose(vals, d["rnm_val"], rtol=1e-4, atol=1e-7)
+
+
+@pytest.mark.parametrize("name,init_rtol", [("hier_smooth_small", 1e-4),
+                                            ("hier_latent_small", 5e-3)])
+def test_hierarchical_fit_matches_reference(golden, name, init_rtol):
+    """The hierarchical model drives many posterior scale matrices to the
+    FP32 jitter threshold (the reference itself jitters ~1/3 of the points),
+    so independent FP32 implementations make different jitter decisions.
+    Pinned: the state after initialisation (1e-4 relative; 5e-3 for the
+    43-D data with integer columns, whose posterior scales are ill
+    conditioned already at initialisation) and the held-out mean log density
+    to 2e-3 relative."""
+    d = golden(name)
+    m = O.LocalHierarchicalKDE(k_or_knn_graph=d["graph"], block_size=int(d["block_size"]))
+    X, t = d["X"], float(d["threshold"])
+    m.init_parameters_(X, t)
+    m.init_summaries_(X, t)
+    np.testing.assert_allclose(m.rnm_log_consts, d["init_log_consts"], rtol=init_rtol)
+    np.testing.assert_allclose(m.nu, d["init_nu"], rtol=1e-5)
+    m.fit(X, iterations=int(d["iterations"]), sparsity_threshold=t)
+    lp = m.log_pred_density(X, d["Y"])
+    assert abs(lp.mean() - d["log_pred"].mean()) <= 2e-3 * abs(d["log_pred"].mean())
