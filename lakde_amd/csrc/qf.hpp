@@ -53,9 +53,12 @@ __device__ __forceinline__ bool keep_resp(float d, float threshold, float log_t_
     *r = expf(d);
     return *r > threshold;
   }
-  if (d <= -110.f) return false;
-  *r = expf(d);
-  return *r != 0.f;
+  // exp(d) rounds to a nonzero float iff exp(d) > 2^-150, i.e. d > -150 ln 2;
+  // below ln(2^-149) the correctly rounded result is the smallest denormal
+  // (device expf returns 0 there, a denormal ulp away)
+  if (d <= -103.972077f) return false;
+  *r = (d < -103.278929f) ? 1.40129846e-45f : expf(d);
+  return true;
 }
 
 // online log-sum-exp state

@@ -218,8 +218,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
     # ------------------------------------------------------------- state
     @property
     def rnm_blocks(self):
-        """Per-block (indices (2, nnz) int32, values), the reference layout."""
-        return [self.arena.block(b) for b in range(self.arena.num_blocks)]
+        """Per-block (indices (2, nnz) int32, values) copies, the reference
+        layout (rows local to the block)."""
+        return [tuple(t.clone() for t in self.arena.block(b))
+                for b in range(self.arena.num_blocks)]
 
     def data_log_likelihood_no_rnm(self, X):
         N, D = X.shape
@@ -241,7 +243,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         for k in ("nu_0", "nu", "a_0", "b_0", "a", "b", "rnm_log_consts", "logdet_W",
                   "rnm_active_contribs", "W_inv_jitter"):
             v = state_dict[k]
-            setattr(self, k, v.to(dev) if torch.is_tensor(v) else v)
+            setattr(self, k, v.to(dev).clone() if torch.is_tensor(v) else v)
         self.a = self.a.contiguous()
         self._nu0 = float(self.nu_0)
         self._b0 = float(self.b_0)

@@ -149,6 +149,6 @@ class LocalHierarchicalKDE(LocallyAdaptiveKDE):
         for k in ("k", "nu_shared_0", "lambda_0_triu", "lambda_0", "nu_shared", "W_shared_triu",
                   "expect_lambda_contribs", "W_shared_inv_jitter"):
             v = state_dict[k]
-            setattr(self, k, v.to(X.device) if torch.is_tensor(v) else v)
+            setattr(self, k, v.to(X.device).clone() if torch.is_tensor(v) else v)
         self._load_blocks(list(state_dict["rnm_blocks"]), X)
         self._alloc(X)

@@ -105,9 +105,9 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
         (lakde/local_knn_kde.py:343-367).  X (the training data, on the GPU)
         is needed to size the device buffers."""
         self._load_common(state_dict, X.device)
-        self.knn_graph = state_dict["knn_graph"].to(X.device).contiguous()
+        self.knn_graph = state_dict["knn_graph"].to(X.device).contiguous().clone()
         self.k = state_dict["k"]
         for k in ("logdet_sigma_0", "sigma_0_lambda_mm_trace", "sigma_0_jitter"):
-            setattr(self, k, state_dict[k].to(X.device))
+            setattr(self, k, state_dict[k].to(X.device).clone())
         self._load_blocks(list(state_dict["rnm_blocks"]), X)
         self._alloc(X)

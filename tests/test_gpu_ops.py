@@ -103,7 +103,7 @@ def test_chol_quad_form_matches_oracle(K, fp, D):
     ref = O.chol_quad_form(Wt, Xn, Xm, dt=np.float64)
     # the models pass a transposed view of the lower inverse factor
     W_dev = _t(np.swapaxes(Wt, 1, 2), fp).mT
-    assert not W_dev.is_contiguous()
+    assert D == 1 or not W_dev.is_contiguous()
     Q = _np(K.chol_quad_form(W_dev, _t(Xn, fp), _t(Xm, fp)))
     tol = 3e-5 if fp == torch.float32 else 1e-12
     np.testing.assert_allclose(Q, ref, rtol=tol, atol=tol * ref.mean())
@@ -112,19 +112,20 @@ def test_chol_quad_form_matches_oracle(K, fp, D):
 def test_chol_quad_form_stride0_and_views(K):
     rng = np.random.default_rng(3)
     D, N, M = 4, 9, 50
+    f32 = torch.float32
     Wt = _factors(rng, 1, D)[0]
     Xn, Xm = rng.standard_normal((N, D)), rng.standard_normal((M, D))
-    W_dev = _t(Wt).expand(N, D, D)            # stride-0 batch, as the shared models pass it
-    Xm_dev = _t(np.asfortranarray(Xm))        # column-major points
-    Xm_view = torch.as_tensor(Xm.T.copy(), device=DEV, dtype=torch.float32).T
-    Q1 = _np(K.chol_quad_form(W_dev, _t(Xn), Xm_view))
-    Q2 = _np(K.chol_quad_form(W_dev.contiguous(), _t(Xn), Xm_dev.contiguous()))
+    W_dev = _t(Wt, f32).expand(N, D, D)       # stride-0 batch, as the shared models pass it
+    Xm_view = torch.as_tensor(Xm.T.copy(), device=DEV, dtype=f32).T   # column-major points
+    assert Xm_view.stride() == (1, M)
+    Q1 = _np(K.chol_quad_form(W_dev, _t(Xn, f32), Xm_view))
+    Q2 = _np(K.chol_quad_form(W_dev.contiguous(), _t(Xn, f32), Xm_view.contiguous()))
     np.testing.assert_array_equal(Q1, Q2)
     ref = O.chol_quad_form(np.broadcast_to(Wt, (N, D, D)), Xn, Xm, dt=np.float64)
     np.testing.assert_allclose(Q1, ref, rtol=3e-5)
     # identity factor -> squared distances; Q = 0 on coincident points
     I = torch.eye(D, device=DEV).expand(N, D, D)
-    Q = _np(K.chol_quad_form(I, _t(Xn), _t(Xn)))
+    Q = _np(K.chol_quad_form(I, _t(Xn, f32), _t(Xn, f32)))
     np.testing.assert_allclose(Q, np.sum((Xn[None] - Xn[:, None]) ** 2, -1), rtol=1e-5, atol=1e-6)
     assert np.all(np.diagonal(Q) == 0)
 
@@ -144,7 +145,7 @@ def test_torch_ops_registered(K):
     np.testing.assert_array_equal(_np(out), O.ind2ptr(_np(r), 9))
     Wt = _t(_factors(rng, 3, 2))
     Xn, Xm = _t(rng.standard_normal((3, 2))), _t(rng.standard_normal((5, 2)))
-    q = torch.empty(3, 5, device=DEV)
+    q = torch.empty(3, 5, device=DEV, dtype=torch.float64)
     torch.ops.lakde.chol_quad_form_cuda(Wt, Xn, Xm, q)
     np.testing.assert_allclose(_np(q), _np(K.chol_quad_form(Wt, Xn, Xm)))
 
@@ -190,9 +191,24 @@ def _run_posterior(prior, S, nu, a, b, nu0, Xn, jit_w, jit_p=None, shared=False)
     return {k: _np(v) for k, v in outs.items()}
 
 
+def _allowed_jitters():
+    eps = np.float32(np.finfo(np.float32).eps)
+    j, out = np.float32(0), {0.0}
+    for k in range(32):
+        j = np.float32(j + np.float32(eps * 2 ** k))
+        out.add(float(j))
+    return out
+
+
 def test_posterior_kernel_matches_oracle():
     """Factorisation, jitter schedule, inverse, logdet, trace and the record
-    constants of the posterior kernel against the oracle's float32 path."""
+    constants of the posterior kernel against the oracle's float32 path.
+
+    Rank-deficient priors (rank 3 of 6) need jitter; whether eps or 3 eps
+    suffices for such a matrix depends on the rounding of the elimination, so
+    there the check is that the kernel's jitter is on the reference schedule
+    and factors the jittered matrix; the posterior (full rank) decisions are
+    compared exactly."""
     rng = np.random.default_rng(11)
     nb, D = 40, 6
     A = rng.standard_normal((nb, D, 3)).astype(np.float32)
@@ -200,7 +216,7 @@ def test_posterior_kernel_matches_oracle():
     prior[:10] += np.eye(D, dtype=np.float32)                   # a few full rank
     B = rng.standard_normal((nb, D, D)).astype(np.float32)
     S = (B @ np.swapaxes(B, 1, 2) * 0.1).astype(np.float32)
-    S[5] = 0                                                    # W_inv = prior only
+    S[5] = 0.01 * np.eye(D, dtype=np.float32)
     nu = (D + rng.random(nb) * 30).astype(np.float32)
     a = np.full(nb, 3.0, np.float32)
     b = (1 + rng.random(nb)).astype(np.float32)
@@ -210,13 +226,16 @@ def test_posterior_kernel_matches_oracle():
     nu0 = np.float32(D)
     out = _run_posterior(prior, S, nu, a, b, nu0, Xn, jw0, jp0)
     assert out["status"][0] == 0
-    # oracle, float32 decisions
-    Lp, jp = O.stabilized_cholesky(prior, jp0.copy())
-    np.testing.assert_array_equal(out["jit_p"], jp)
+    allowed = _allowed_jitters()
+    jp = out["jit_p"]
+    assert all(float(v) in allowed for v in jp)
+    assert np.all(jp[:10] == 0) and np.all(jp[10:] > 0)
+    Lp = out["ptril"].astype(np.float64)
     prior_j = O._add_diag(prior, jp)
+    np.testing.assert_allclose(Lp @ np.swapaxes(Lp, 1, 2), prior_j, atol=2e-5)
     scale = (nu0 * (a / b).astype(np.float32)).astype(np.float32)
     Winv = (prior_j * scale[:, None, None]).astype(np.float32) + S
-    L, jw = O.stabilized_cholesky(Winv, jw0.copy())
+    _, jw = O.stabilized_cholesky(Winv, jw0.copy())
     np.testing.assert_array_equal(out["jit_w"], jw)
     # compare in float64 through the matrices the factors represent
     Winv_j = O._add_diag(Winv.astype(np.float64), jw.astype(np.float64))
@@ -227,22 +246,55 @@ def test_posterior_kernel_matches_oracle():
     assert rel.max() < 1e-3
     logdet_ref = -np.linalg.slogdet(Winv_j)[1]
     np.testing.assert_allclose(out["logdet"], logdet_ref, rtol=1e-4, atol=1e-3)
-    tr_ref = nu * O.chol_mm_trace(Lp.astype(np.float64), np.swapaxes(Linv, 1, 2))
+    # prior logdet: exact against slogdet where the prior is well conditioned;
+    # for the jittered rank-deficient ones (pivots ~ sqrt(eps)) against the
+    # kernel's own factor
+    np.testing.assert_allclose(out["ldp"][:10],
+                               np.linalg.slogdet(prior_j[:10].astype(np.float64))[1],
+                               rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(out["ldp"], 2 * np.log(np.diagonal(Lp, axis1=1, axis2=2)).sum(1),
+                               rtol=1e-5, atol=1e-5)
+    tr_ref = nu * O.chol_mm_trace(Lp, np.swapaxes(Linv, 1, 2))
     np.testing.assert_allclose(out["trace"], tr_ref, rtol=1e-4)
     # record constants
     T = D * (D + 1) // 2
     rec = out["rec"]
     np.testing.assert_array_equal(rec[:, T:T + D], Xn)
+    tri = np.tril_indices(D)
+    np.testing.assert_array_equal(rec[:, :T], out["Linv"][:, tri[0], tri[1]])
     eld = O.mvdigamma(nu / 2, D) + np.float32(D * math.log(2)) + out["logdet"]
     np.testing.assert_allclose(rec[:, T + D], 0.5 * eld, rtol=1e-5, atol=1e-5)
     np.testing.assert_array_equal(rec[:, T + D + 1], -0.5 * nu)
     pc = (O.lgamma((nu + 1) / 2) - O.lgamma((nu + 1 - D) / 2) + 0.5 * out["logdet"]
           - np.float32(D / 2 * math.log(math.pi)))
     np.testing.assert_allclose(rec[:, T + D + 2], pc, rtol=1e-5, atol=1e-5)
+    np.testing.assert_array_equal(rec[:, T + D + 3], (nu + 1) / 2)
     # persistent jitter: a second call with the stored jitters keeps them
     out2 = _run_posterior(prior, S, nu, a, b, nu0, Xn, out["jit_w"], out["jit_p"])
     np.testing.assert_array_equal(out2["jit_w"], out["jit_w"])
     np.testing.assert_array_equal(out2["jit_p"], out["jit_p"])
+
+
+def test_posterior_shared_prior():
+    """Shared-prior mode: one D x D prior, trace against the full matrix
+    (lakde/local_hierarchical_kde.py:289-291)."""
+    rng = np.random.default_rng(12)
+    nb, D = 17, 9
+    C = rng.standard_normal((D, D)).astype(np.float32)
+    prior = (C @ C.T + np.eye(D, dtype=np.float32)).astype(np.float32)
+    B = rng.standard_normal((nb, D, D)).astype(np.float32)
+    S = (B @ np.swapaxes(B, 1, 2)).astype(np.float32)
+    nu = (D + rng.random(nb) * 5).astype(np.float32)
+    ab = np.ones(nb, np.float32)
+    Xn = rng.standard_normal((nb, D)).astype(np.float32)
+    out = _run_posterior(prior, S, nu, ab, ab, np.float32(D), Xn, np.zeros(nb, np.float32),
+                         shared=True)
+    Winv = (prior[None] * np.float32(D)).astype(np.float32) + S
+    Linv = out["Linv"].astype(np.float64)
+    np.testing.assert_allclose(np.swapaxes(Linv, 1, 2) @ Linv, np.linalg.inv(Winv.astype(np.float64)),
+                               rtol=1e-3, atol=1e-6)
+    tr_ref = nu * O.chol_mm_trace(prior.astype(np.float64), np.swapaxes(Linv, 1, 2))
+    np.testing.assert_allclose(out["trace"], tr_ref, rtol=1e-4)
 
 
 def test_posterior_jitter_schedule_exact():
