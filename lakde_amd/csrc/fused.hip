@@ -107,21 +107,25 @@ __global__ void __launch_bounds__(256) rebal_coeff_kernel(int64_t N, const float
   sums[m] = 0.f;
 }
 
-// other blocks: v *= coeff[col]; every entry: sums[col] += v
-__global__ void __launch_bounds__(256) rebal_scale_sum_kernel(int64_t num_blocks, int64_t cap,
-                                                             int64_t exclude, const int32_t *nnz,
-                                                             const int32_t *cols, float *vals,
-                                                             const float *coeff, float *sums) {
-  const int64_t total = num_blocks * cap;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = e / cap;
-    if (e - b * cap >= nnz[b]) continue;
-    const int32_t col = cols[e];
-    float v = vals[e];
-    if (b != exclude) {
+// other blocks: v *= coeff[col]; every entry: sums[col] += v.
+// grid.y = block, grid.x strides over the block's nnz entries.
+__global__ void __launch_bounds__(256) rebal_scale_sum_kernel(int64_t cap, int64_t exclude,
+                                                             const int32_t *__restrict__ nnz,
+                                                             const int32_t *__restrict__ cols,
+                                                             float *__restrict__ vals,
+                                                             const float *__restrict__ coeff,
+                                                             float *__restrict__ sums) {
+  const int64_t b = blockIdx.y;
+  const int32_t n = nnz[b];
+  const int32_t *cb = cols + b * cap;
+  float *vb = vals + b * cap;
+  const bool scale = b != exclude;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int32_t col = cb[e];
+    float v = vb[e];
+    if (scale) {
       v = v * coeff[col];
-      vals[e] = v;
+      vb[e] = v;
     }
     atomicAdd(&sums[col], v);
   }
@@ -138,17 +142,17 @@ __global__ void __launch_bounds__(256) rebal_norm_kernel(int64_t N, float *c, co
   coeff[m] = clamp_nan(1.0f / s, FLT_MIN, FLT_MAX);
 }
 
-__global__ void __launch_bounds__(256) rebal_rescale_kernel(int64_t num_blocks, int64_t cap,
-                                                           const int32_t *nnz,
-                                                           const int32_t *cols, float *vals,
-                                                           const float *coeff) {
-  const int64_t total = num_blocks * cap;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = e / cap;
-    if (e - b * cap >= nnz[b]) continue;
-    vals[e] = vals[e] * coeff[cols[e]];
-  }
+__global__ void __launch_bounds__(256) rebal_rescale_kernel(int64_t cap,
+                                                           const int32_t *__restrict__ nnz,
+                                                           const int32_t *__restrict__ cols,
+                                                           float *__restrict__ vals,
+                                                           const float *__restrict__ coeff) {
+  const int64_t b = blockIdx.y;
+  const int32_t n = nnz[b];
+  const int32_t *cb = cols + b * cap;
+  float *vb = vals + b * cap;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    vb[e] = vb[e] * coeff[cb[e]];
 }
 
 // ----------------------------------------------------------- predictive
@@ -274,19 +278,21 @@ int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_
   LAKDE_CHECK_ARG(N >= 1 && num_blocks >= 1 && cap >= 1, "bad size");
   LAKDE_CHECK_ARG(nnz && cols && vals && c_prev && c && ws, "null pointer");
   hipStream_t s = as_stream(stream);
+  LAKDE_CHECK_ARG(num_blocks <= 65535 && cap < (1LL << 31), "too many blocks or entries");
   float *coeff = ws, *sums = ws + N;
   const unsigned gN = (unsigned)ceil_div(N, 256);
-  int64_t ge = ceil_div(num_blocks * cap, 256);
-  if (ge > 8192) ge = 8192;
+  // x-extent: enough workgroups per block to cover ~all entries in one pass
+  int64_t gx = ceil_div(cap, 256 * 4);
+  if (gx > 64) gx = 64;
+  const dim3 ge((unsigned)gx, (unsigned)num_blocks);
   hipLaunchKernelGGL(rebal_coeff_kernel, dim3(gN), dim3(256), 0, s, N, c_prev, c, coeff, sums);
   LAKDE_LAUNCHED();
-  hipLaunchKernelGGL(rebal_scale_sum_kernel, dim3((unsigned)ge), dim3(256), 0, s, num_blocks, cap,
-                     exclude_block, nnz, cols, vals, coeff, sums);
+  hipLaunchKernelGGL(rebal_scale_sum_kernel, ge, dim3(256), 0, s, cap, exclude_block, nnz, cols,
+                     vals, coeff, sums);
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(rebal_norm_kernel, dim3(gN), dim3(256), 0, s, N, c, sums, coeff);
   LAKDE_LAUNCHED();
-  hipLaunchKernelGGL(rebal_rescale_kernel, dim3((unsigned)ge), dim3(256), 0, s, num_blocks, cap,
-                     nnz, cols, vals, coeff);
+  hipLaunchKernelGGL(rebal_rescale_kernel, ge, dim3(256), 0, s, cap, nnz, cols, vals, coeff);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }

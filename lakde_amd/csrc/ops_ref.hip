@@ -97,10 +97,11 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
     for (int64_t c0 = beg; c0 < end; c0 += E) {
       const int64_t ne = (end - c0) < E ? (end - c0) : E;
       __syncthreads();
-      for (int64_t t = tid; t < ne * D; t += kOuterThreads) {
-        int64_t e = t / D, d = t - e * D;
-        int64_t cidx = (int64_t)col[c0 + e];
-        stage[e * D + d] = Xm[cidx * D + d] - xn_s[d];
+      const int Di = (int)D;
+      for (int t = tid; t < (int)ne * Di; t += kOuterThreads) {
+        const int e = t / Di, d = t - e * Di;
+        const int64_t cidx = (int64_t)col[c0 + e];
+        stage[t] = Xm[cidx * D + d] - xn_s[d];
       }
       for (int64_t e = tid; e < ne; e += kOuterThreads)
         wts[e] = GRAPH ? T(1) : value[c0 + e];
