@@ -209,6 +209,33 @@ int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, i
                        float *out, void *workspace, size_t workspace_bytes,
                        lakde_stream_t stream);
 
+/* ------------------------------------------------------------------ *
+ * Precision form (8 <= D <= 63): BF16 matrix cores, split precision    *
+ * ------------------------------------------------------------------ */
+
+/* Bytes of one kernel's bf16 fragment block (Linv hi/lo, FP32 centre),
+ * or -1 when D is outside [8, 63]. */
+int64_t lakde_bf16_frag_bytes(int64_t D);
+
+/* Fragment blocks of nb kernels from their FP32 records: Linv split into
+ * bf16 hi + lo (zero padded) and the centre x_n in FP32. */
+int lakde_pack_frag_bf16(const float *rec, int64_t nb, int64_t D, void *frag,
+                         lakde_stream_t stream);
+
+/* lakde_estep_main on the matrix cores: y = Linv (x_m - x_n) via bf16x3 MFMA
+ * (hi*hi + hi*lo + lo*hi of the split factor and the split difference, FP32
+ * accumulate).  Same outputs as lakde_estep_main; follow with
+ * lakde_estep_finish, whose compaction recomputes every kept responsibility
+ * exactly in FP32. */
+int lakde_estep_main_bf16(const lakde_estep_args *args, const void *frag,
+                          lakde_stream_t stream);
+
+/* lakde_pred_density with the quadratic forms on the matrix cores
+ * (frag: N fragment blocks; Y: M x D float32 test points). */
+int lakde_pred_density_bf16(const float *rec, const void *frag, int64_t N, const float *Y,
+                            int64_t M, int64_t D, float *out, void *workspace,
+                            size_t workspace_bytes, lakde_stream_t stream);
+
 /* fill helper (device memset replacement that is safe under graph capture) */
 int lakde_fill_f32(float *p, int64_t n, float v, lakde_stream_t stream);
 

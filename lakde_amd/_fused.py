@@ -100,7 +100,10 @@ class EstepWorkspace:
 
 
 def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
-          arena=None, b=None, compact_only=False, timer=None):
+          arena=None, b=None, compact_only=False, timer=None, bf16=None):
+    """One E-step launch sequence.  ``bf16=frag`` (the block's fragment
+    blocks) runs the main (quadratic form / LSE / ballot) kernel on the
+    matrix cores."""
     N, D = X.shape
     nb = rec_blk.size(0)
     if arena is not None:
@@ -115,11 +118,17 @@ def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
     st = stream_of(X)
     if compact_only:
         check(lib.lakde_estep_compact(args, st), "estep_compact")
-    elif timer is not None:
-        timer(lambda: check(lib.lakde_estep_main(args, st), "estep_main"))
-        check(lib.lakde_estep_finish(args, st), "estep_finish")
+        return
+    if bf16 is not None:
+        main = lambda: check(lib.lakde_estep_main_bf16(args, ptr(bf16), st),  # noqa: E731
+                             "estep_main_bf16")
     else:
-        check(lib.lakde_estep(args, st), "estep")
+        main = lambda: check(lib.lakde_estep_main(args, st), "estep_main")  # noqa: E731
+    if timer is not None:
+        timer(main)
+    else:
+        main()
+    check(lib.lakde_estep_finish(args, st), "estep_finish")
 
 
 def rebalance(arena, exclude_block, c_prev, c, ws):
@@ -139,6 +148,33 @@ def pred_density(rec, Y, out):
     check(lib.lakde_pred_density(ptr(rec), N, ptr(Y), M, D, ptr(out), ptr(ws), nbytes,
                                  stream_of(Y)),
           "pred_density")
+    return out
+
+
+def bf16_frag_bytes(D):
+    return int(_lib.load().lakde_bf16_frag_bytes(D))
+
+
+def bf16_kp(D):
+    return (D + 15) // 16 * 16
+
+
+def pack_frag_bf16(rec, D, out):
+    nb = rec.size(0)
+    check(_lib.load().lakde_pack_frag_bf16(ptr(rec), nb, D, ptr(out), stream_of(rec)),
+          "pack_frag_bf16")
+    return out
+
+
+def pred_density_bf16(rec, frag, Y, out):
+    N = rec.size(0)
+    M, D = Y.shape
+    lib = _lib.load()
+    nbytes = int(lib.lakde_pred_workspace(N, M))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=Y.device)
+    check(lib.lakde_pred_density_bf16(ptr(rec), ptr(frag), N, ptr(Y), M, D, ptr(out), ptr(ws),
+                                      nbytes, stream_of(Y)),
+          "pred_density_bf16")
     return out
 
 

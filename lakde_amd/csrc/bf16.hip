@@ -1,0 +1,324 @@
+// Precision form of the quadratic forms: y = Linv (x_m - x_n) on the gfx950
+// BF16 matrix cores (v_mfma_f32_32x32x16_bf16) with FP32 accumulation.
+//
+// Split precision: both operands are pairs of bf16 values (hi = bf16(v),
+// lo = bf16(v - hi)) and each K-step issues hi*hi + hi*lo + lo*hi (bf16x3,
+// ~2^-17 relative per product).  The B operand is the difference
+// x_m - x_n itself, formed in FP32 and split on the vector ALUs per kernel
+// (hidden behind the matrix-core work), so there is no cancellation between
+// large terms: the error is relative to |Linv (x_m - x_n)|, as in the FP32
+// path.  The E-step kernels write the same outputs (normalisers, ballot
+// masks) as the FP32 kernels; the compaction that follows recomputes every
+// kept responsibility exactly in FP32 from the FP32 records, so stored
+// values carry no bf16 error.
+//
+// Layouts (ISA of v_mfma_f32_32x32x16_bf16): A and B fragments hold 8 bf16 per
+// lane, row/column = lane % 32, K = 16 * step + 8 * (lane / 32) + [0, 8);
+// C holds 16 floats per lane, column = lane % 32, row = 8 * (r / 4) +
+// 4 * (lane / 32) + r % 4 for register r.
+#include "fused_impl.hpp"
+
+namespace lakde {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ constexpr int bf_kp(int D) { return (D + 15) / 16 * 16; }
+__host__ __device__ constexpr int bf_dp(int D) { return (D + 31) / 32 * 32; }
+// per-kernel fragment block: L_hi [Dp x Kp] bf16, L_lo [Dp x Kp] bf16, x_n [Kp] f32
+__host__ __device__ constexpr int64_t bf_frag_bytes(int D) {
+  return (int64_t)bf_dp(D) * bf_kp(D) * 2 * 2 + (int64_t)bf_kp(D) * 4;
+}
+
+// ------------------------------------------------------------- packing
+// Per kernel n of the records: dense Linv (zero padded) split into bf16
+// hi/lo, and the centre x_n (zero padded) in FP32.
+__global__ void __launch_bounds__(256) pack_frag_kernel(const float *__restrict__ rec, int64_t R,
+                                                       int D, uint8_t *__restrict__ frag) {
+  const int64_t n = blockIdx.x;
+  const int Kp = bf_kp(D), Dp = bf_dp(D);
+  const float *r = rec + n * R;
+  uint8_t *f = frag + n * bf_frag_bytes(D);
+  __bf16 *hi = (__bf16 *)f;
+  __bf16 *lo = hi + Dp * Kp;
+  float *xn = (float *)(lo + Dp * Kp);
+  for (int t = threadIdx.x; t < Dp * Kp; t += blockDim.x) {
+    const int i = t / Kp, j = t - i * Kp;
+    const float v = (i < D && j <= i) ? r[i * (i + 1) / 2 + j] : 0.f;
+    const __bf16 h = (__bf16)v;
+    hi[t] = h;
+    lo[t] = (__bf16)(v - (float)h);
+  }
+  for (int j = threadIdx.x; j < Kp; j += blockDim.x) xn[j] = j < D ? r[rec_x(D) + j] : 0.f;
+}
+
+// --------------------------------------------------- MFMA quadratic forms
+// One wave owns 64 consecutive columns (two 32-column MFMA tiles); a
+// workgroup of 4 waves owns 256 columns and walks the kernels in lockstep,
+// double-buffering each kernel's fragment block through LDS.  After the
+// MFMAs, lane l < 32 reduces tile 0's column l and lane l >= 32 tile 1's
+// column l - 32, so every lane ends with Q for exactly one column (m =
+// wave_base + lane) — the same lane -> column map as the FP32 kernels.
+template <int RT, int KS>
+struct BfTile {
+  static constexpr int Kp = KS * 16;
+  static constexpr int Dp = RT * 32;
+  static constexpr int FB = Dp * Kp * 2 * 2 + Kp * 4;
+  float xf[2][KS][8];  // this lane's point coordinates for its B fragments
+
+  __device__ void load_points(const float *__restrict__ X, int64_t N, int D, int64_t base,
+                              int lane) {
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int64_t col = base + ct * 32 + (lane & 31);
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int k = s * 16 + 8 * (lane >> 5) + e;
+          xf[ct][s][e] = (col < N && k < D) ? X[col * D + k] : 0.f;
+        }
+    }
+  }
+
+  // Q for this lane's column from one kernel's fragment block in LDS
+  __device__ __forceinline__ float quad(const uint8_t *fb, int lane) const {
+    const __bf16 *hi = reinterpret_cast<const __bf16 *>(fb);
+    const __bf16 *lo = hi + Dp * Kp;
+    const float *xn = reinterpret_cast<const float *>(lo + Dp * Kp);
+    bf16x8 dh[2][KS], dl[2][KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      float c[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) c[e] = xn[s * 16 + 8 * (lane >> 5) + e];
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float d = xf[ct][s][e] - c[e];
+          const __bf16 h = (__bf16)d;
+          dh[ct][s][e] = h;
+          dl[ct][s][e] = (__bf16)(d - (float)h);
+        }
+    }
+    float q0 = 0.f, q1 = 0.f;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int off = (rt * 32 + (lane & 31)) * Kp + s * 16 + 8 * (lane >> 5);
+        const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(hi + off);
+        const bf16x8 al = *reinterpret_cast<const bf16x8 *>(lo + off);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dh[0][s], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dh[1][s], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dl[0][s], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dl[1][s], acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, dh[0][s], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, dh[1][s], acc1, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        q0 = fmaf(acc0[r], acc0[r], q0);
+        q1 = fmaf(acc1[r], acc1[r], q1);
+      }
+    }
+    const bool upper = lane >= 32;
+    const float send = upper ? q0 : q1;
+    const float recv = __shfl_xor(send, 32, kWave);
+    return (upper ? q1 : q0) + recv;
+  }
+};
+
+// cooperative copy of one fragment block (FB bytes, 16-byte granules) into LDS
+template <int FB>
+__device__ __forceinline__ void stage_frag(const uint8_t *__restrict__ src, uint8_t *dst) {
+  const int4 *s = reinterpret_cast<const int4 *>(src);
+  int4 *d = reinterpret_cast<int4 *>(dst);
+  for (int t = threadIdx.x; t < FB / 16; t += blockDim.x) d[t] = s[t];
+}
+
+template <int RT, int KS, int MODE>
+__global__ void __launch_bounds__(256)
+estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
+                  const float *__restrict__ rec, int64_t R, int Dreal, EParams p) {
+  using T = BfTile<RT, KS>;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2][T::FB];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t base = (int64_t)blockIdx.x * 256 + w * kWave;
+  const int64_t m = base + lane;
+  const bool valid = m < p.N;
+  const int64_t chunk = base / kWave;
+  T tile;
+  tile.load_points(X, p.N, Dreal, base, lane);
+  const int64_t diag_n = m - p.row0;
+  const int64_t nb = p.nb;
+  const int cofs = (int)rec_c(Dreal);
+  float cm = 0.f;
+
+  if (MODE != LAKDE_ESTEP_FIXED) {
+    Lse l;
+    l.init();
+    stage_frag<T::FB>(frag, lds[0]);
+    __syncthreads();
+    for (int64_t n = 0; n < nb; ++n) {
+      if (n + 1 < nb) stage_frag<T::FB>(frag + (n + 1) * T::FB, lds[(n + 1) & 1]);
+      const float q = tile.quad(lds[n & 1], lane);
+      const float *rc = rec + n * R + cofs;
+      float x = __fadd_rn(__fmul_rn(q, rc[1]), rc[0]);
+      if (n == diag_n) x = neg_inf();
+      l.add1(x);
+      __syncthreads();
+    }
+    const float lse = l.value();
+    if (valid) {
+      if (MODE == LAKDE_ESTEP_ACCUM) {
+        cm = logaddexp_f(p.c[m], lse);
+      } else {
+        const float co = p.c_old[m];
+        const float oc = logf(p.old_colsum[m]) + co;
+        cm = logaddexp_f(logsubexp_f(co, oc), lse);
+      }
+      p.c[m] = cm;
+    }
+  } else {
+    cm = valid ? p.c[m] : 0.f;
+  }
+  if (MODE == LAKDE_ESTEP_ACCUM) return;
+
+  stage_frag<T::FB>(frag, lds[0]);
+  __syncthreads();
+  for (int64_t n = 0; n < nb; ++n) {
+    if (n + 1 < nb) stage_frag<T::FB>(frag + (n + 1) * T::FB, lds[(n + 1) & 1]);
+    const float q = tile.quad(lds[n & 1], lane);
+    const float *rc = rec + n * R + cofs;
+    float x = __fadd_rn(__fmul_rn(q, rc[1]), rc[0]);
+    if (n == diag_n) x = neg_inf();
+    const float d = __fsub_rn(x, cm);
+    float r;
+    const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
+    const uint64_t mask = __ballot(keep);
+    if (lane == 0 && chunk < p.nchunks) p.masks[n * p.nchunks + chunk] = mask;
+    __syncthreads();
+  }
+}
+
+template <int RT, int KS>
+__global__ void __launch_bounds__(256)
+pred_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ Y,
+                 const float *__restrict__ rec, int64_t R, int Dreal, PParams p) {
+  using T = BfTile<RT, KS>;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2][T::FB];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t base = (int64_t)blockIdx.x * 256 + w * kWave;
+  const int64_t m = base + lane;
+  const int64_t split = blockIdx.y;
+  T tile;
+  tile.load_points(Y, p.M, Dreal, base, lane);
+  const int64_t sbeg = split * p.per_split;
+  const int64_t send = (sbeg + p.per_split) < p.N ? (sbeg + p.per_split) : p.N;
+  const int cofs = (int)rec_c(Dreal);
+  Lse l;
+  l.init();
+  stage_frag<T::FB>(frag + sbeg * T::FB, lds[0]);
+  __syncthreads();
+  for (int64_t n = sbeg; n < send; ++n) {
+    const int64_t k = n - sbeg;
+    if (n + 1 < send) stage_frag<T::FB>(frag + (n + 1) * T::FB, lds[(k + 1) & 1]);
+    const float q = tile.quad(lds[k & 1], lane);
+    const float *rc = rec + n * R + cofs;
+    l.add1(__fsub_rn(rc[2], __fmul_rn(rc[3], log1pf(q))));
+    __syncthreads();
+  }
+  if (m < p.M) {
+    p.part_m[split * p.M + m] = l.m;
+    p.part_s[split * p.M + m] = l.s;
+  }
+}
+
+#define LAKDE_BF_SHAPES(X) X(1, 1) X(1, 2) X(2, 3) X(2, 4)
+
+static bool bf_shape(int D, int *RT, int *KS) {
+  if (D < 8 || D > 63) return false;
+  *KS = bf_kp(D) / 16;
+  *RT = bf_dp(D) / 32;
+  // instantiated (RT, KS) pairs: D <= 16 -> (1,1); <= 32 -> (1,2); <= 48 -> (2,3); <= 64 -> (2,4)
+  return (*RT == 1 && (*KS == 1 || *KS == 2)) || (*RT == 2 && (*KS == 3 || *KS == 4));
+}
+
+}  // namespace lakde
+
+using namespace lakde;
+
+extern "C" {
+
+int64_t lakde_bf16_frag_bytes(int64_t D) { return (D >= 8 && D <= 63) ? bf_frag_bytes((int)D) : -1; }
+
+int lakde_pack_frag_bf16(const float *rec, int64_t nb, int64_t D, void *frag,
+                         lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 8 && D <= 63, "the bf16 form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(rec && frag && nb >= 0, "bad args");
+  if (nb == 0) return LAKDE_OK;
+  hipLaunchKernelGGL(pack_frag_kernel, dim3((unsigned)nb), dim3(256), 0, as_stream(stream), rec,
+                     rec_floats(D), (int)D, (uint8_t *)frag);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int lakde_estep_main_bf16(const lakde_estep_args *args, const void *frag,
+                          lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr && frag, "null args");
+  int RT, KS;
+  LAKDE_CHECK_ARG(bf_shape((int)args->D, &RT, &KS), "the bf16 form needs 8 <= D <= 63");
+  EParams p;
+  int rc = estep_fill_params(*args, &p);
+  if (rc) return rc;
+  const dim3 grid((unsigned)ceil_div(p.N, 256));
+  hipStream_t s = as_stream(stream);
+  const int64_t R = rec_floats(args->D);
+  const int Dr = (int)args->D;
+#define LAUNCH(rt, ks)                                                                           \
+  if (RT == rt && KS == ks) {                                                                    \
+    if (args->mode == LAKDE_ESTEP_ACCUM)                                                         \
+      hipLaunchKernelGGL((estep_bf16_kernel<rt, ks, LAKDE_ESTEP_ACCUM>), grid, dim3(256), 0, s,  \
+                         (const uint8_t *)frag, p.X, p.rec, R, Dr, p);         \
+    else if (args->mode == LAKDE_ESTEP_FIXED)                                                    \
+      hipLaunchKernelGGL((estep_bf16_kernel<rt, ks, LAKDE_ESTEP_FIXED>), grid, dim3(256), 0, s,  \
+                         (const uint8_t *)frag, p.X, p.rec, R, Dr, p);         \
+    else                                                                                         \
+      hipLaunchKernelGGL((estep_bf16_kernel<rt, ks, LAKDE_ESTEP_UPDATE>), grid, dim3(256), 0, s, \
+                         (const uint8_t *)frag, p.X, p.rec, R, Dr, p);         \
+  }
+  LAKDE_BF_SHAPES(LAUNCH)
+#undef LAUNCH
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int lakde_pred_density_bf16(const float *rec, const void *frag, int64_t N, const float *Y,
+                            int64_t M, int64_t D, float *out, void *workspace,
+                            size_t workspace_bytes, lakde_stream_t stream) {
+  int RT, KS;
+  LAKDE_CHECK_ARG(bf_shape((int)D, &RT, &KS), "the bf16 form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(rec && frag && Y && out && workspace && N >= 1 && M >= 0, "bad args");
+  PParams p;
+  int rc = pred_fill_params(rec, N, M, out, workspace, workspace_bytes, &p);
+  if (rc) return rc;
+  if (M == 0) return LAKDE_OK;
+  const dim3 grid((unsigned)ceil_div(M, 256), (unsigned)p.nsplit);
+  hipStream_t s = as_stream(stream);
+  const int64_t R = rec_floats(D);
+#define LAUNCH(rt, ks)                                                                       \
+  if (RT == rt && KS == ks)                                                                  \
+    hipLaunchKernelGGL((pred_bf16_kernel<rt, ks>), grid, dim3(256), 0, s, (const uint8_t *)frag, \
+                       Y, rec, R, (int)D, p);
+  LAKDE_BF_SHAPES(LAUNCH)
+#undef LAUNCH
+  LAKDE_LAUNCHED();
+  return pred_combine(p, stream);
+}
+
+}  // extern "C"

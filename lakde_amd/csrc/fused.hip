@@ -5,6 +5,9 @@
 
 #include "fused_impl.hpp"
 
+extern "C" size_t lakde_estep_workspace(int64_t N, int64_t nb);
+extern "C" size_t lakde_pred_workspace(int64_t N, int64_t M);
+
 namespace lakde {
 
 LAKDE_FOR_EACH_D(LAKDE_DECL_D)
@@ -177,19 +180,7 @@ static void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsp
   *nsplit = ceil_div(N, *per_split);
 }
 
-}  // namespace lakde
-
-using namespace lakde;
-
-extern "C" {
-
-size_t lakde_estep_workspace(int64_t N, int64_t nb) {
-  const int64_t nchunks = ceil_div(N, kWave);
-  return align256((size_t)nb * nchunks * 8) + align256((size_t)nb * nchunks * 4) +
-         align256((size_t)nb * 4);
-}
-
-static int fill_eparams(const lakde_estep_args &a, EParams *p) {
+int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   LAKDE_CHECK_ARG(a.D >= 1 && a.D <= 63, "D must be in [1, 63]");
   LAKDE_CHECK_ARG(a.N >= 1 && a.nb >= 1 && a.row0 >= 0 && a.row0 + a.nb <= a.N, "bad block");
   LAKDE_CHECK_ARG(a.N < (1LL << 31), "N must fit int32 column indices");
@@ -230,10 +221,46 @@ static int fill_eparams(const lakde_estep_args &a, EParams *p) {
   return LAKDE_OK;
 }
 
+
+int pred_fill_params(const float *rec, int64_t N, int64_t M, float *out, void *workspace,
+                     size_t workspace_bytes, PParams *p) {
+  LAKDE_CHECK_ARG(N >= 1 && M >= 0, "bad size");
+  LAKDE_CHECK_ARG(rec && out && workspace, "null pointer");
+  LAKDE_CHECK_ARG(workspace_bytes >= lakde_pred_workspace(N, M), "workspace too small");
+  p->N = N;
+  p->M = M;
+  pred_geometry(N, M, &p->per_split, &p->nsplit);
+  p->rec = rec;
+  p->Y = nullptr;
+  p->part_m = (float *)workspace;
+  p->part_s = (float *)((char *)workspace + align256((size_t)p->nsplit * M * 4));
+  p->out = out;
+  return LAKDE_OK;
+}
+
+int pred_combine(const PParams &p, lakde_stream_t stream) {
+  hipLaunchKernelGGL(pred_combine_kernel, dim3((unsigned)ceil_div(p.M, 256)), dim3(256), 0,
+                     as_stream(stream), p, (float)std::log((double)p.N));
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+}  // namespace lakde
+
+using namespace lakde;
+
+extern "C" {
+
+size_t lakde_estep_workspace(int64_t N, int64_t nb) {
+  const int64_t nchunks = ceil_div(N, kWave);
+  return align256((size_t)nb * nchunks * 8) + align256((size_t)nb * nchunks * 4) +
+         align256((size_t)nb * 4);
+}
+
 int lakde_estep_main(const lakde_estep_args *args, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(args != nullptr, "null args");
   EParams p;
-  int rc = fill_eparams(*args, &p);
+  int rc = estep_fill_params(*args, &p);
   if (rc) return rc;
   rc = dispatch_estep_main(args->D, p, args->mode, as_stream(stream));
   if (rc) set_error("lakde_estep_main: launch failed");
@@ -244,7 +271,7 @@ int lakde_estep_finish(const lakde_estep_args *args, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(args != nullptr, "null args");
   if (args->mode == LAKDE_ESTEP_ACCUM) return LAKDE_OK;
   EParams p;
-  int rc = fill_eparams(*args, &p);
+  int rc = estep_fill_params(*args, &p);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(estep_scan_rows_kernel, dim3((unsigned)p.nb), dim3(256), 0, s, p);
@@ -265,7 +292,7 @@ int lakde_estep(const lakde_estep_args *args, lakde_stream_t stream) {
 int lakde_estep_compact(const lakde_estep_args *args, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(args != nullptr && args->mode != LAKDE_ESTEP_ACCUM, "bad args");
   EParams p;
-  int rc = fill_eparams(*args, &p);
+  int rc = estep_fill_params(*args, &p);
   if (rc) return rc;
   rc = dispatch_estep_compact(args->D, p, as_stream(stream));
   if (rc) set_error("lakde_estep_compact: launch failed");
@@ -307,26 +334,15 @@ int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, i
                        float *out, void *workspace, size_t workspace_bytes,
                        lakde_stream_t stream) {
   LAKDE_CHECK_ARG(D >= 1 && D <= 63, "D must be in [1, 63]");
-  LAKDE_CHECK_ARG(N >= 1 && M >= 0, "bad size");
-  LAKDE_CHECK_ARG(rec && Y && out && workspace, "null pointer");
-  LAKDE_CHECK_ARG(workspace_bytes >= lakde_pred_workspace(N, M), "workspace too small");
-  if (M == 0) return LAKDE_OK;
+  LAKDE_CHECK_ARG(Y != nullptr, "null pointer");
   PParams p;
-  p.N = N;
-  p.M = M;
-  pred_geometry(N, M, &p.per_split, &p.nsplit);
-  p.rec = rec;
+  int rc = pred_fill_params(rec, N, M, out, workspace, workspace_bytes, &p);
+  if (rc) return rc;
+  if (M == 0) return LAKDE_OK;
   p.Y = Y;
-  p.part_m = (float *)workspace;
-  p.part_s = (float *)((char *)workspace + align256((size_t)p.nsplit * M * 4));
-  p.out = out;
-  hipStream_t s = as_stream(stream);
-  int rc = dispatch_pred_main(D, p, s);
+  rc = dispatch_pred_main(D, p, as_stream(stream));
   if (rc) { set_error("lakde_pred_density: launch failed"); return rc; }
-  hipLaunchKernelGGL(pred_combine_kernel, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, p,
-                     (float)std::log((double)N));
-  LAKDE_LAUNCHED();
-  return LAKDE_OK;
+  return pred_combine(p, stream);
 }
 
 }  // extern "C"

@@ -234,6 +234,12 @@ int launch_pred_main(const PParams &p, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
 }
 
+// shared host helpers (fused.hip)
+int estep_fill_params(const lakde_estep_args &a, EParams *p);
+int pred_fill_params(const float *rec, int64_t N, int64_t M, float *out, void *workspace,
+                     size_t workspace_bytes, PParams *p);
+int pred_combine(const PParams &p, lakde_stream_t stream);
+
 // declarations for the dispatcher (defined by explicit instantiation)
 #define LAKDE_DECL_D(d)                                                   \
   extern template int launch_estep_main<d>(const EParams &, int, hipStream_t); \

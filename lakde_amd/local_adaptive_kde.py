@@ -25,8 +25,16 @@ from .utils import tensor_as
 
 
 class LocallyAdaptiveKDE(AbstractKDE):
-    def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs):
+    """``precision``: "fp32" (VALU quadratic forms) or "bf16" (the precision
+    form: quadratic forms on the BF16 matrix cores with bf16x3 split
+    precision, kept responsibilities recomputed exactly in FP32; 8 <= D <= 63).
+    """
+
+    def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32"):
         super().__init__(block_size, verbose, logs)
+        if precision not in ("fp32", "bf16"):
+            raise ValueError("precision must be 'fp32' or 'bf16'")
+        self.precision = precision
         self.nu_0 = nu_0
         self.a_0 = a_0
         self.b_0 = b_0
@@ -75,6 +83,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         bs = self.block_size
         dev = X.device
         self._R = F.record_floats(D)
+        self.rec_D = D
         self.rec = torch.zeros(N, self._R, device=dev)
         self._S = torch.empty(bs, D, D, device=dev)
         self._prior_buf = torch.empty(bs, D, D, device=dev)
@@ -88,6 +97,11 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._c_new = torch.empty(N, device=dev)
         self._rebal_ws = torch.empty(2 * N, device=dev)
         self._ews = F.EstepWorkspace(N, bs, dev)
+        if self.precision == "bf16":
+            if not 8 <= D <= 63:
+                raise ValueError("the bf16 precision form needs 8 <= D <= 63")
+            self._fb = F.bf16_frag_bytes(D)
+            self._frag_blk = torch.empty(bs * self._fb, dtype=torch.uint8, device=dev)
 
     def _init_arena(self, X, graph, sparsity_threshold):
         """Initial responsibilities 1/init_k on the kNN graph, laid out per
@@ -156,7 +170,8 @@ class LocallyAdaptiveKDE(AbstractKDE):
         s, e, nb = self._span(b, N)
         c = self._c_new if mode == ESTEP_UPDATE else self.rnm_log_consts
         F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, c_old=c_old,
-                old_colsum=old_colsum, arena=self.arena, b=b, timer=self._estep_timer)
+                old_colsum=old_colsum, arena=self.arena, b=b, timer=self._estep_timer,
+                bf16=self._bf16_operands(b, N))
         nnz = self._sync(b)
         if nnz > self.arena.cap:
             self.arena.grow(nnz)
@@ -166,12 +181,21 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self.arena.nnz[b] = nnz
         return old, nnz
 
+    def _bf16_operands(self, b, N):
+        if self.precision != "bf16":
+            return None
+        s, e, nb = self._span(b, N)
+        frag = self._frag_blk[:nb * self._fb]
+        F.pack_frag_bf16(self.rec[s:e], self.rec_D, frag)
+        return frag
+
     # ------------------------------------------------------------- E-step
     def _init_lse_pass(self, X, b):
         """lakde/local_knn_kde.py:157-166: c = logaddexp(c, LSE over block b)."""
         N = X.size(0)
         s, e, nb = self._span(b, N)
-        F.estep(X, self.rec[s:e], s, ESTEP_ACCUM, 0.0, self.rnm_log_consts, self._ews)
+        F.estep(X, self.rec[s:e], s, ESTEP_ACCUM, 0.0, self.rnm_log_consts, self._ews,
+                bf16=self._bf16_operands(b, N))
 
     def _init_fixed_pass(self, X, b, sparsity_threshold):
         """partial_expectation_step(update_normalization=False)."""
@@ -213,6 +237,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
                                   self._trace_s[:nb], self._logdet_prior_s[:nb], update_b=False)
         self._sync(None)
         out = torch.empty(Y.size(0), device=Y.device)
+        if self.precision == "bf16":
+            frag = torch.empty(N * self._fb, dtype=torch.uint8, device=Y.device)
+            F.pack_frag_bf16(self.rec, D, frag)
+            return F.pred_density_bf16(self.rec, frag, Y, out)
         return F.pred_density(self.rec, Y, out)
 
     # ------------------------------------------------------------- state
