@@ -30,6 +30,7 @@ def _parse():
     ap.add_argument("--workload", default="mix32_d6")
     ap.add_argument("--block-size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision-form", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     return ap.parse_args()
 
@@ -85,6 +86,90 @@ def cpu_baseline(X, graph, block_size, threshold, seconds):
     return {"value": 1.0 / (dt * nb), "unit": "VI iters/sec", "cores": 1, "kind": "port",
             "sample": f"{steps} partial steps of the full workload (block size {block_size}), "
                       f"{dt:.3f} s each, extrapolated to {nb} steps per VI iteration"}
+
+
+def precision_form(seconds_budget=None):
+    """The precision comparison on the full-scale d=63 workload
+    (mix128_d63: N=1,000,000, D=63, k=128, threshold 1/128, block size 512):
+    initialise once (BF16 form), then time partial steps of the FP32 and the
+    BF16 E-step from the same state and extrapolate to a VI iteration
+    (num_blocks partial steps); predictive density on 5,000 held-out points
+    with both forms on the same posterior."""
+    import torch
+
+    from lakde_amd import LocalKNearestKDE, synthetic
+    from lakde_amd.utils import compute_knn_graph
+
+    w = synthetic.WORKLOADS["mix128_d63"]
+    Xh, Yh = synthetic.make("mix128_d63", n_test=5000)
+    N, D = Xh.shape
+    dev = torch.device("cuda:0")
+    X = torch.from_numpy(Xh).to(dev)
+    Y = torch.from_numpy(Yh).to(dev)
+    t = w["threshold"]
+    t0 = time.perf_counter()
+    graph = compute_knn_graph(X, w["k"], 1024, include_self=False)
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t0
+    print(f"# precision form: mix128_d63 N={N} D={D}, knn graph {t_graph:.1f}s", file=sys.stderr,
+          flush=True)
+    model = LocalKNearestKDE(k_or_knn_graph=graph, block_size=512, precision="bf16")
+    t0 = time.perf_counter()
+    model.init_parameters_(X, t)
+    model.init_summaries_(X, t)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    nblocks = (N - 1) // model.block_size + 1
+    print(f"# precision form: init (bf16) {t_init:.1f}s, {nblocks} blocks", file=sys.stderr,
+          flush=True)
+    res = {"workload": f"mix128_d63: LocalKNearestKDE, N={N}, D={D}, k={w['k']}, "
+                       f"threshold=1/128, block_size=512",
+           "init_s_bf16": round(t_init, 2), "num_blocks": nblocks}
+    steps = 6
+    for prec in ("fp32", "bf16"):
+        model.precision = prec
+        timer = KernelTimer()
+        model._estep_timer = timer
+        model.partial_step_(X, 0, sparsity_threshold=t)  # warm-up
+        torch.cuda.synchronize()
+        timer.pairs.clear()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            model.partial_step_(X, 1 + i * (nblocks // steps), sparsity_threshold=t)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        k_ms, _ = timer.mean_ms()
+        model._estep_timer = None
+        pairs = 512 * N
+        res[prec] = {"partial_step_ms": round(dt * 1e3, 2),
+                     "vi_iters_per_s_extrapolated": round(1.0 / (dt * nblocks), 5),
+                     "estep_main_ms": round(k_ms, 3),
+                     "estep_algorithmic_tflops": round(pairs * D * (D + 1) / (k_ms * 1e-3) / 1e12, 2)}
+        print(f"# precision form {prec}: {dt * 1e3:.1f} ms/partial step, E-step main {k_ms:.2f} ms",
+              file=sys.stderr, flush=True)
+    # executed matrix-core work of the bf16 E-step main kernel: two passes of
+    # 3 (bf16x3) x 2 x Dp x Kp flop per pair
+    dp, kp = 64, 64
+    mfma_flop = 2 * 3 * 2 * dp * kp * 512 * N
+    res["bf16"]["mfma_executed_tflops"] = round(mfma_flop / (res["bf16"]["estep_main_ms"] * 1e-3) / 1e12, 1)
+    res["bf16"]["mfma_frac_of_2500_tflops"] = round(res["bf16"]["mfma_executed_tflops"] / 2500.0, 4)
+    res["speedup_bf16_over_fp32"] = round(res["fp32"]["partial_step_ms"] / res["bf16"]["partial_step_ms"], 2)
+    # predictive density, both forms, same posterior
+    lps = {}
+    for prec in ("fp32", "bf16"):
+        model.precision = prec
+        model.log_pred_density(X, Y[:256])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lps[prec] = model.log_pred_density(X, Y)
+        torch.cuda.synchronize()
+        res[prec]["test_pts_per_s"] = round(Y.size(0) / (time.perf_counter() - t0), 1)
+    d = (lps["bf16"] - lps["fp32"]).abs()
+    res["pred_abs_err_bf16_vs_fp32_nats"] = {"mean": float(d.mean()), "max": float(d.max())}
+    res["mean_log_density_fp32"] = float(lps["fp32"].mean())
+    del model, X, Y, graph
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -171,6 +256,14 @@ def main():
                 "traffic": None, "kernel": f"estep_main_kernel<{D},UPDATE>",
                 "kernel_ms": round(k_ms, 5), "pairs_per_launch": int(pairs_per_launch)}
 
+    prec = None
+    if not args.no_precision_form:
+        try:
+            prec = precision_form()
+        except Exception as e:  # reported, never fatal to the headline line
+            print(f"# precision form failed: {e!r}", file=sys.stderr, flush=True)
+            prec = {"error": repr(e)}
+
     cpu = None
     if not args.no_cpu_baseline:
         g = graph.cpu().numpy()
@@ -199,6 +292,7 @@ def main():
         "init_s": round(t_init, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "precision_form": prec,
     }
     print(json.dumps(out), flush=True)
 
