@@ -236,6 +236,32 @@ int lakde_pred_density_bf16(const float *rec, const void *frag, int64_t N, const
                             int64_t M, int64_t D, float *out, void *workspace,
                             size_t workspace_bytes, lakde_stream_t stream);
 
+/* ------------------------------------------------------------------ *
+ * Differentiation form                                                *
+ * ------------------------------------------------------------------ */
+
+/* Records from explicit factors: Linv (N x D x D lower), centres Xc (N x D),
+ * nu (N); the constants are derived as in lakde_posterior. */
+int lakde_pack_records(const float *Linv, const float *Xc, const float *nu, int64_t N,
+                       int64_t D, float *rec, lakde_stream_t stream);
+
+/* Backward of lakde_pred_density (lakde/local_adaptive_kde.py:200-223 has
+ * none; this is the gradient of its result).  Inputs: the records, Y, the
+ * forward output `out` (M) and the upstream gradient g (M).  The mixture
+ * weights w_nm are recomputed from out (nothing of size N x M is stored).
+ * Outputs: dY (M x D) = d(sum_m g_m out_m)/dY, and per kernel
+ *   S (N x D x D) = sum_m a_nm d d^T,  s1 (N x D) = sum_m a_nm d,
+ *   beta (N) = sum_m g_m w_nm,  tau (N) = sum_m g_m w_nm log1p(Q_nm),
+ * with d = y_m - x_n and a_nm = -g_m w_nm (nu_n + 1) / 2 / (1 + Q_nm), from
+ * which d/dLinv = tril(2 Linv S) + diag(beta / diag(Linv)),
+ * d/dXc = -2 Linv^T Linv s1 and d/dnu = (beta (psi(h) - psi(h - D/2)) - tau) / 2,
+ * h = (nu + 1) / 2. */
+size_t lakde_pred_backward_workspace(int64_t N, int64_t M, int64_t D);
+int lakde_pred_backward(const float *rec, int64_t N, const float *Y, int64_t M, int64_t D,
+                        const float *out, const float *g, float *dY, float *S, float *s1,
+                        float *beta, float *tau, void *workspace, size_t workspace_bytes,
+                        lakde_stream_t stream);
+
 /* fill helper (device memset replacement that is safe under graph capture) */
 int lakde_fill_f32(float *p, int64_t n, float v, lakde_stream_t stream);
 

@@ -71,6 +71,10 @@ _SIGNATURES = {
     "lakde_pack_frag_bf16": (_int, [_vp, _i64, _i64, _vp, _vp]),
     "lakde_estep_main_bf16": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp]),
     "lakde_pred_density_bf16": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "lakde_pack_records": (_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp]),
+    "lakde_pred_backward_workspace": (_sz, [_i64, _i64, _i64]),
+    "lakde_pred_backward": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, _sz, _vp]),
 }
 
 EXPORTS = tuple(_SIGNATURES)

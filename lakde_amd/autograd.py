@@ -1,0 +1,182 @@
+"""Differentiation form: a differentiable log_pred_density.
+
+``mixture_log_density(Y, Linv, Xc, nu)`` is the Student-t mixture of
+lakde/local_adaptive_kde.py:200-223 given explicit posterior factors
+(Linv = L^{-1}, lower, i.e. the reference's W_triu transposed), as a
+``torch.autograd.Function`` whose forward is the streaming predictive kernel
+and whose backward is the hand-written HIP backward (gradients w.r.t. Y, Linv,
+Xc and nu; the mixture weights are recomputed from the saved per-point
+log-sum-exp, nothing of size N x M is kept).
+
+``log_pred_density(model, X, Y)`` builds the factors from a fitted model so
+that gradients flow to ``Y`` and, for LocalHierarchicalKDE, to the shared
+prior factor: the factors are the HIP posterior kernel's, and the chain
+W_inv = prior (nu0 E[tau]) + S_n -> Cholesky -> inverse is differentiated
+with the analytic Cholesky VJP in float64 (S_n, the responsibility scatter,
+comes from the HIP accumulation kernel and is held fixed, as are the
+persistent jitters).
+"""
+
+import torch
+
+from . import _fused as F
+from . import _lib
+from ._lib import check, ptr, require_gpu, stream_of
+
+
+def _pack_records(Linv, Xc, nu):
+    N, D, _ = Linv.shape
+    rec = torch.empty(N, F.record_floats(D), device=Linv.device)
+    check(_lib.load().lakde_pack_records(ptr(Linv), ptr(Xc), ptr(nu), N, D, ptr(rec),
+                                         stream_of(Linv)), "pack_records")
+    return rec
+
+
+def _backward(rec, Y, out, g):
+    N = rec.size(0)
+    M, D = Y.shape
+    dev = Y.device
+    lib = _lib.load()
+    nbytes = int(lib.lakde_pred_backward_workspace(N, M, D))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    dY = torch.empty(M, D, device=dev)
+    S = torch.empty(N, D, D, device=dev)
+    s1 = torch.empty(N, D, device=dev)
+    beta = torch.empty(N, device=dev)
+    tau = torch.empty(N, device=dev)
+    check(lib.lakde_pred_backward(ptr(rec), N, ptr(Y), M, D, ptr(out), ptr(g), ptr(dY), ptr(S),
+                                  ptr(s1), ptr(beta), ptr(tau), ptr(ws), nbytes, stream_of(Y)),
+          "pred_backward")
+    return dY, S, s1, beta, tau
+
+
+class MixtureLogDensity(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, Y, Linv, Xc, nu):
+        require_gpu(Y, Linv, Xc, nu)
+        Y = Y.detach().float().contiguous()
+        Linv = Linv.detach().float().contiguous()
+        Xc = Xc.detach().float().contiguous()
+        nu = nu.detach().float().contiguous()
+        rec = _pack_records(Linv, Xc, nu)
+        out = F.pred_density(rec, Y, torch.empty(Y.size(0), device=Y.device))
+        ctx.save_for_backward(Y, Linv, nu, rec, out)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        Y, Linv, nu, rec, out = ctx.saved_tensors
+        D = Y.size(1)
+        dY, S, s1, beta, tau = _backward(rec, Y, out, g.detach().float().contiguous())
+        dLinv = dXc = dnu = None
+        if ctx.needs_input_grad[1]:
+            diag = torch.diagonal(Linv, dim1=-2, dim2=-1)
+            dLinv = torch.tril(2 * torch.bmm(Linv, S)) + torch.diag_embed(beta[:, None] / diag)
+        if ctx.needs_input_grad[2]:
+            dXc = -2 * torch.bmm(Linv.mT.contiguous(), torch.bmm(Linv, s1[..., None]))[..., 0]
+        if ctx.needs_input_grad[3]:
+            h = (nu + 1) / 2
+            dnu = 0.5 * (beta * (torch.digamma(h) - torch.digamma(h - D / 2)) - tau)
+        return (dY if ctx.needs_input_grad[0] else None), dLinv, dXc, dnu
+
+
+def mixture_log_density(Y, Linv, Xc, nu):
+    """log p(y) = -ln N + LSE_n log St(y | x_n, Linv_n, nu_n) for explicit
+    factors; differentiable w.r.t. all four arguments."""
+    return MixtureLogDensity.apply(Y, Linv, Xc, nu)
+
+
+class FactorInverse(torch.autograd.Function):
+    """Linv = chol(A)^{-1} with the forward value supplied (the HIP posterior
+    kernel's factor, so the differentiable path evaluates exactly the model's
+    posterior) and the analytic VJP in float64: with L = Linv^{-1},
+    gL = -tril(Linv^T gLinv Linv^T), P = Phi(L^T gL) (lower triangle, half
+    diagonal), gA = sym(Linv^T P Linv) (the Cholesky VJP of Murray, 2016,
+    "Differentiation of the Cholesky decomposition")."""
+
+    @staticmethod
+    def forward(ctx, A, Linv_value):
+        ctx.save_for_backward(Linv_value)
+        return Linv_value.clone()
+
+    @staticmethod
+    def backward(ctx, gLinv):
+        (Linv,) = ctx.saved_tensors
+        Li = Linv.double()
+        D = Li.size(-1)
+        eye = torch.eye(D, dtype=torch.float64, device=Li.device).expand_as(Li)
+        L = torch.linalg.solve_triangular(Li, eye, upper=False)
+        gL = -torch.tril(Li.mT @ gLinv.double() @ Li.mT)
+        P = torch.tril(L.mT @ gL)
+        P = P - 0.5 * torch.diag_embed(torch.diagonal(P, dim1=-2, dim2=-1))
+        S = Li.mT @ P @ Li
+        return 0.5 * (S + S.mT), None
+
+
+def _scatter(model, X, b):
+    """Posterior contributions S_n, nu_n of block b from the HIP kernel."""
+    N, D = X.shape
+    s, e, nb = model._span(b, N)
+    arena = model.arena
+    S = torch.empty(nb, D, D, device=X.device)
+    nu = torch.empty(nb, device=X.device)
+    F.posterior_contribs(arena.rowptr[b, :nb + 1], arena.col[b], arena.val[b], X[s:e], X,
+                         model._nu0, S, nu)
+    return S, nu
+
+
+def log_pred_density(model, X, Y, prior_factor=None):
+    """Differentiable held-out log density of a fitted model.
+
+    For LocalHierarchicalKDE, ``prior_factor`` (D x D, lower triangular,
+    float64) is the shared prior factor W_shared_triu^T; pass a tensor with
+    ``requires_grad=True`` to differentiate through it (default: the model's
+    current factor, no gradient).  For LocalKNearestKDE the kNN prior is data
+    and only Y (and the factors returned by ``posterior_factors``) carry
+    gradients.
+    """
+    require_gpu(X, Y)
+    N, D = X.shape
+    from .local_hierarchical_kde import LocalHierarchicalKDE
+    if isinstance(model, LocalHierarchicalKDE):
+        if prior_factor is None:
+            prior_factor = model.W_shared_triu.T
+        T = torch.sqrt(model.nu_shared) * prior_factor.T       # W_shared_triu scaled
+        sigma_bar = T @ T.T                                    # lakde/local_hierarchical_kde.py:269-271
+        Linv_val, nu = posterior_factors(model, X)
+        Linvs = []
+        for b in model._blocks(N):
+            s, e, nb = model._span(b, N)
+            S, _ = _scatter(model, X, b)
+            scale = model._nu0 * (model.a[s:e] / model.b[s:e])
+            # W_inv as the posterior kernel forms it (plus the persistent
+            # jitter); its factor is the kernel's, its gradient analytic
+            W_inv = sigma_bar[None] * scale.double()[:, None, None] + S.double()
+            jit = model.W_inv_jitter[s:e].double()
+            W_inv = W_inv + torch.diag_embed(jit[:, None].expand(nb, D))
+            Linvs.append(FactorInverse.apply(W_inv, Linv_val[s:e]))
+        Linv = torch.cat(Linvs)
+    else:
+        Linv, nu = posterior_factors(model, X)
+    return mixture_log_density(Y, Linv, X, nu)
+
+
+def posterior_factors(model, X):
+    """(Linv (N x D x D), nu (N)) of every kernel from the HIP posterior kernel
+    (the persistent jitters are updated as by log_pred_density)."""
+    N, D = X.shape
+    Linv = torch.empty(N, D, D, device=X.device)
+    nu = torch.empty(N, device=X.device)
+    for b in model._blocks(N):
+        s, e, nb = model._span(b, N)
+        model._block_posterior(X, b, model._nu_s[:nb], model._logdet_s[:nb], model._trace_s[:nb],
+                               model._logdet_prior_s[:nb] if hasattr(model, "sigma_0_jitter")
+                               else None, update_b=False)
+        Linv[s:e] = model._Linv[:nb]
+        nu[s:e] = model._nu_s[:nb]
+    model._sync(None)
+    return Linv, nu
+
+
+__all__ = ["MixtureLogDensity", "FactorInverse", "mixture_log_density", "log_pred_density",
+           "posterior_factors"]

@@ -33,7 +33,7 @@ CFLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["ops_ref.hip", "posterior.hip", "fused.hip", "bf16.hip"]
+SOURCES = ["ops_ref.hip", "posterior.hip", "fused.hip", "bf16.hip", "backward.hip"]
 D_GROUPS = [(1, 6), (7, 12), (13, 18), (19, 24), (25, 32), (33, 40), (41, 48), (49, 56),
             (57, 63)]
 
@@ -46,6 +46,8 @@ def _gen_inst(lo, hi):
             f"template int launch_estep_main<{d}>(const EParams &, int, hipStream_t);",
             f"template int launch_estep_compact<{d}>(const EParams &, hipStream_t);",
             f"template int launch_pred_main<{d}>(const PParams &, hipStream_t);",
+            f"template int launch_bwd_y<{d}>(const PParams &, const float *, const float *, "
+            "float *, hipStream_t);",
         ]
     lines.append("}")
     text = "\n".join(lines) + "\n"

@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256) pred_combine_kernel(PParams p, float log_
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-static void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit) {
+void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit) {
   const int64_t chunks = ceil_div(M, kWave);
   int64_t ns = ceil_div(16384, chunks * 4);
   const int64_t max_ns = ceil_div(N, 256);

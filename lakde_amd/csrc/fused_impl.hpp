@@ -205,6 +205,76 @@ pred_main_kernel(const float *__restrict__ rec, const float *__restrict__ Y, PPa
   }
 }
 
+// -------------------------------------------------- backward w.r.t. y
+// d log p(y_m) / d y_m = sum_n w_nm (-(nu_n+1)/(1+Q_nm)) Linv_n^T z_nm with
+// z = Linv (y - x_n) and w_nm = exp(lp_nm - LSE_m) recomputed from the saved
+// per-point log-sum-exp; scaled by the upstream gradient g_m.  Same grid as
+// the predictive kernel; per-split partial gradients are summed afterwards.
+template <int D>
+__global__ void __launch_bounds__(kEThreads)
+bwd_y_kernel(const float *__restrict__ rec, const float *__restrict__ Y,
+             const float *__restrict__ lse, const float *__restrict__ g, PParams p,
+             float *__restrict__ part) {
+  __shared__ float sm_g[4][kWave][D];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t m = (int64_t)blockIdx.x * kWave + lane;
+  const int64_t split = blockIdx.y;
+  const bool valid = m < p.M;
+  float ym[D], gy[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    ym[j] = valid ? Y[m * D + j] : 0.f;
+    gy[j] = 0.f;
+  }
+  const float l_m = valid ? lse[m] : 0.f;
+  const float g_m = valid ? g[m] : 0.f;
+  const int64_t sbeg = split * p.per_split;
+  const int64_t send = (sbeg + p.per_split) < p.N ? (sbeg + p.per_split) : p.N;
+  const int64_t q = (send - sbeg + 3) / 4;
+  const int64_t nbeg = sbeg + w * q;
+  const int64_t nend = (nbeg + q) < send ? (nbeg + q) : send;
+  constexpr int64_t R = rec_floats(D);
+  for (int64_t n = nbeg; n < nend; ++n) {
+    const float *r = rec + n * R;
+    float d[D], z[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) d[j] = ym[j] - r[rec_x(D) + j];
+    float Q = 0.f;
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float y = 0.f;
+#pragma unroll
+      for (int j = 0; j <= i; ++j) y = fmaf(r[t + j], d[j], y);
+      t += i + 1;
+      z[i] = y;
+      Q = fmaf(y, y, Q);
+    }
+    const float h = r[rec_c(D) + 3];
+    const float lp = __fsub_rn(r[rec_c(D) + 2], __fmul_rn(h, log1pf(Q)));
+    const float wgt = expf(lp - l_m);
+    const float a = -2.f * g_m * wgt * h / (1.f + Q);
+    t = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const float az = a * z[i];
+#pragma unroll
+      for (int j = 0; j <= i; ++j) gy[j] = fmaf(r[t + j], az, gy[j]);
+      t += i + 1;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) sm_g[w][lane][j] = gy[j];
+  __syncthreads();
+  if (w == 0 && valid) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      part[(split * p.M + m) * D + j] =
+          (sm_g[0][lane][j] + sm_g[1][lane][j]) + (sm_g[2][lane][j] + sm_g[3][lane][j]);
+  }
+}
+
 // ------------------------------------------------------------- launchers
 template <int D>
 int launch_estep_main(const EParams &p, int mode, hipStream_t s) {
@@ -234,7 +304,16 @@ int launch_pred_main(const PParams &p, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
 }
 
+template <int D>
+int launch_bwd_y(const PParams &p, const float *lse, const float *g, float *part,
+                 hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(p.M, kWave), (unsigned)p.nsplit);
+  hipLaunchKernelGGL((bwd_y_kernel<D>), grid, dim3(kEThreads), 0, s, p.rec, p.Y, lse, g, p, part);
+  return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+}
+
 // shared host helpers (fused.hip)
+void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit);
 int estep_fill_params(const lakde_estep_args &a, EParams *p);
 int pred_fill_params(const float *rec, int64_t N, int64_t M, float *out, void *workspace,
                      size_t workspace_bytes, PParams *p);
@@ -244,6 +323,8 @@ int pred_combine(const PParams &p, lakde_stream_t stream);
 #define LAKDE_DECL_D(d)                                                   \
   extern template int launch_estep_main<d>(const EParams &, int, hipStream_t); \
   extern template int launch_estep_compact<d>(const EParams &, hipStream_t);   \
-  extern template int launch_pred_main<d>(const PParams &, hipStream_t);
+  extern template int launch_pred_main<d>(const PParams &, hipStream_t);        \
+  extern template int launch_bwd_y<d>(const PParams &, const float *, const float *, float *, \
+                                      hipStream_t);
 
 }  // namespace lakde

@@ -10,22 +10,11 @@
 //   a pivot that is not > 0 fails); a jittered factor is refactored and
 //   inverted in FP64, others in FP32.  Then L^{-1}, logdet_W = 2 sum log diag
 //   accumulated in FP64, the trace term and the packed record.
-#include "common.hpp"
+#include "record.hpp"
 
 namespace lakde {
 
 constexpr int kPostMaxD = 63;
-
-// digamma for x > 0: upward recurrence to x >= 6, then the asymptotic series
-// psi(x) = ln x - 1/(2x) - sum_k B_2k / (2k x^2k) (Abramowitz & Stegun 6.3.18),
-// truncated after the x^-10 term (error < 1e-11 for x >= 6).
-__device__ double digamma_d(double x) {
-  double r = 0.0;
-  while (x < 6.0) { r -= 1.0 / x; x += 1.0; }
-  double f = 1.0 / (x * x);
-  double t = f * (-1.0 / 12 + f * (1.0 / 120 + f * (-1.0 / 252 + f * (1.0 / 240 + f * (-1.0 / 132)))));
-  return r + log(x) - 0.5 / x + t;
-}
 
 // Lower Cholesky of M (+ jitter on the diagonal, added in FP32) into Lout.
 // Lane i owns row i.  Returns true on success (every pivot > 0).
@@ -197,24 +186,7 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
     const float trace = nu * tr;
     a.trace[n] = trace;
     if (a.b_out != nullptr) a.b_out[n] = a.b0 + (a.nu0 * trace) / 2.f;
-    // E-step constants (lakde/local_adaptive_kde.py:36-44)
-    const float half_nu = nu / 2.f;
-    double mvd = 0.0;
-    for (int i = 1; i <= D; ++i) mvd += digamma_d((double)(half_nu + (1.f - (float)i) / 2.f));
-    const float eld = ((float)mvd + (float)(D * 0.69314718055994530942)) + logdet;
-    const float c_est = 0.5f * eld;
-    const float e_est = -0.5f * nu;
-    // predictive constants (lakde/local_adaptive_kde.py:209-214)
-    const float h1 = (nu + 1.f) / 2.f;
-    const float h2 = ((nu + 1.f) - (float)D) / 2.f;
-    const float lg1 = (float)lgamma((double)h1);
-    const float lg2 = (float)lgamma((double)h2);
-    const float pc = ((lg1 - lg2) + 0.5f * logdet) - (float)(D / 2.0 * 1.14472988584940017414);
-    float *cst = rec + rec_c(D);
-    cst[0] = c_est;
-    cst[1] = e_est;
-    cst[2] = pc;
-    cst[3] = h1;
+    record_consts(rec + rec_c(D), nu, logdet, D);
   }
 }
 
