@@ -106,11 +106,14 @@ int lakde_knn_prior_cov(const float *X, int64_t N, int64_t D, int64_t row0, int6
                         lakde_stream_t stream);
 
 /* S[n] as lakde_w_inv on one block's CSR plus nu[n] = nu0 + sum_e value[e]
- * (lakde/local_adaptive_kde.py:66-81). rowptr/col int32/int64. */
+ * (lakde/local_adaptive_kde.py:66-81). rowptr/col int32/int64.  With
+ * colscale (N floats, or NULL) the responsibility of entry e is
+ * value[e] * colscale[col[e]] (stored values plus per-column factors, see
+ * lakde_rebalance_lazy). */
 int lakde_posterior_contribs(const void *rowptr, int rowptr_dtype, const void *col,
-                             int col_dtype, const float *value, const float *Xn,
-                             const float *Xm, int64_t nb, int64_t M, int64_t D, float nu0,
-                             float *S, float *nu, lakde_stream_t stream);
+                             int col_dtype, const float *value, const float *colscale,
+                             const float *Xn, const float *Xm, int64_t nb, int64_t M, int64_t D,
+                             float nu0, float *S, float *nu, lakde_stream_t stream);
 
 /* Batched posterior factorisation (lakde/local_adaptive_kde.py:83-96,
  * lakde/linalg_util.py:30-37,49-52,109-170, lakde/local_knn_kde.py:195-207,220-222,
@@ -200,6 +203,46 @@ int lakde_estep_compact(const lakde_estep_args *args, lakde_stream_t stream);
 int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_block,
                     const int32_t *nnz, const int32_t *cols, float *vals,
                     const float *c_prev, float *c, float *ws, lakde_stream_t stream);
+
+/* The same rebalance with the responsibilities stored unscaled (lazy
+ * rebalance): the value of entry e is val[e] * G[col[e]] with per-column
+ * factors G (float) and stored column totals U (double).  A partial step on
+ * block i then touches only block i's entries and O(N) column state instead
+ * of every block:
+ *   lakde_rebalance_lazy_begin (before lakde_estep, ESTEP_UPDATE):
+ *     old_colsum[m] += sum over block i of val*G (the true old column sums
+ *     the E-step reads), U_old[m] += sum of block i's stored values,
+ *     *unsafe = 0;
+ *   lakde_rebalance_lazy_check (after the E-step wrote c):
+ *     *unsafe = 1 if any G[m] clamp(exp(c_prev - c)) lies outside
+ *     [2^-60, 2^60] or is NaN; the stored form cannot represent that step
+ *     (float under/overflow) and the caller must instead fold
+ *     (lakde_rebalance_fold), run lakde_rebalance, fold again and clear
+ *     old_colsum and U_old;
+ *   lakde_rebalance_lazy (otherwise; block i now holds true values r):
+ *     coeff = clamp(exp(c_prev - c), tiny, max), val = r / (G coeff),
+ *     s = max(coeff G (U - U_old) + sum r, eps), c += log s,
+ *     G *= coeff clamp(1/s, tiny, max), U += sum val - U_old,
+ *     and old_colsum, U_old are cleared again.
+ * old_colsum (N floats) and U_old (N doubles) are zero between steps; ws is
+ * 2*N doubles, zero on entry and left zero.  With col, val, nnz_dev NULL no
+ * block was updated (the initial normalisation, c_prev == c).  c_prev may
+ * alias c. */
+int lakde_rebalance_lazy_begin(int64_t N, const int32_t *col, const float *val,
+                               const int32_t *nnz_dev, const float *G, float *old_colsum,
+                               double *U_old, int32_t *unsafe, lakde_stream_t stream);
+int lakde_rebalance_lazy_check(int64_t N, const float *c_prev, const float *c, const float *G,
+                               int32_t *unsafe, lakde_stream_t stream);
+int lakde_rebalance_lazy(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
+                         const float *c_prev, float *c, float *G, double *U, double *U_old,
+                         float *old_colsum, double *ws, lakde_stream_t stream);
+
+/* Fold the column factors into the stored values of every block of the
+ * arena (val *= G[col]), then G = 1 and U = the exact stored column sums.
+ * Bounds the drift of U; afterwards the arena holds true values. */
+int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32_t *nnz,
+                         const int32_t *cols, float *vals, float *G, double *U,
+                         lakde_stream_t stream);
 
 /* Streaming predictive density (lakde/local_adaptive_kde.py:200-223):
  * out[m] = -log N + LSE_n [coeff_n - (nu_n+1)/2 log1p(Q_nm)] over all N

@@ -21,7 +21,9 @@ class Arena:
     within a row.  This is the data of the reference's per-block
     ``rnm_blocks[b] = (indices (2, nnz) int32, values)``
     (lakde/local_adaptive_kde.py:145-153) with the row pointer kept alongside.
-    ``nnz_dev`` carries one extra slot used as the posterior status flag.
+    ``nnz_dev`` carries two extra slots: the posterior status flag and the
+    lazy rebalance's unsafe-step flag, read with the entry counts in one
+    copy.
     """
 
     def __init__(self, num_blocks, block_size, cap, device):
@@ -34,12 +36,16 @@ class Arena:
         self.col = torch.zeros(num_blocks, self.cap, dtype=i32, device=device)
         self.val = torch.zeros(num_blocks, self.cap, dtype=torch.float32, device=device)
         self.rowptr = torch.zeros(num_blocks, block_size + 1, dtype=i32, device=device)
-        self.nnz_dev = torch.zeros(num_blocks + 1, dtype=i32, device=device)
+        self.nnz_dev = torch.zeros(num_blocks + 2, dtype=i32, device=device)
         self.nnz = [0] * num_blocks
 
     @property
     def status_ptr(self):
-        return self.nnz_dev[self.num_blocks:]
+        return self.nnz_dev[self.num_blocks:self.num_blocks + 1]
+
+    @property
+    def unsafe_ptr(self):
+        return self.nnz_dev[self.num_blocks + 1:]
 
     def grow(self, need):
         new_cap = max(int(need * 1.25), int(self.cap * 1.5)) + 255
@@ -69,10 +75,11 @@ def knn_prior_cov(X, row0, graph_blk, out):
     return out
 
 
-def posterior_contribs(rowptr, col, val, Xn, X, nu0, S, nu):
+def posterior_contribs(rowptr, col, val, Xn, X, nu0, S, nu, colscale=None):
     nb, D = Xn.shape
     check(_lib.load().lakde_posterior_contribs(ptr(rowptr), idx_code(rowptr.dtype), ptr(col),
-                                               idx_code(col.dtype), ptr(val), ptr(Xn), ptr(X),
+                                               idx_code(col.dtype), ptr(val), ptr(colscale),
+                                               ptr(Xn), ptr(X),
                                                nb, X.size(0), D, float(nu0), ptr(S), ptr(nu),
                                                stream_of(X)),
           "posterior_contribs")
@@ -137,6 +144,66 @@ def rebalance(arena, exclude_block, c_prev, c, ws):
                                       ptr(arena.nnz_dev), ptr(arena.col), ptr(arena.val),
                                       ptr(c_prev), ptr(c), ptr(ws), stream_of(c)),
           "rebalance")
+
+
+class LazyRebalance:
+    """Column state of the lazy rebalance: the arena holds stored values u
+    whose true values are u * G[col]; U (float64) are the stored column
+    totals.  ``old_colsum`` (the true old column sums of the updated block,
+    read by the E-step), ``U_old`` and the workspace stay zero between
+    steps."""
+
+    def __init__(self, N, device):
+        self.N = N
+        self.G = torch.ones(N, device=device)
+        self.U = torch.zeros(N, dtype=torch.float64, device=device)
+        self.U_old = torch.zeros(N, dtype=torch.float64, device=device)
+        self.old_colsum = torch.zeros(N, device=device)
+        self.ws = torch.zeros(2 * N, dtype=torch.float64, device=device)
+        self._exact_ws = None
+        self.dirty = False  # G != 1 somewhere
+        self.exact_steps = 0
+
+    def begin(self, arena, b):
+        check(_lib.load().lakde_rebalance_lazy_begin(
+            self.N, ptr(arena.col[b]), ptr(arena.val[b]), ptr(arena.nnz_dev[b:b + 1]),
+            ptr(self.G), ptr(self.old_colsum), ptr(self.U_old), ptr(arena.unsafe_ptr),
+            stream_of(self.G)),
+            "rebalance_lazy_begin")
+
+    def check(self, arena, c_prev, c):
+        check(_lib.load().lakde_rebalance_lazy_check(
+            self.N, ptr(c_prev), ptr(c), ptr(self.G), ptr(arena.unsafe_ptr), stream_of(c)),
+            "rebalance_lazy_check")
+
+    def exact_step(self, arena, b, c_prev, c):
+        """A step the stored form cannot represent: fold, rebalance the true
+        values exactly, recompute U and clear the per-step sums."""
+        self.fold(arena)
+        if self._exact_ws is None:
+            self._exact_ws = torch.empty(2 * self.N, device=self.G.device)
+        rebalance(arena, b, c_prev, c, self._exact_ws)
+        self.fold(arena)
+        self.U_old.zero_()
+        self.old_colsum.zero_()
+
+    def end(self, arena, b, c_prev, c):
+        if b is None:
+            blk = (ptr(None), ptr(None), ptr(None))
+        else:
+            blk = (ptr(arena.col[b]), ptr(arena.val[b]), ptr(arena.nnz_dev[b:b + 1]))
+        check(_lib.load().lakde_rebalance_lazy(
+            self.N, blk[0], blk[1], blk[2], ptr(c_prev), ptr(c), ptr(self.G), ptr(self.U),
+            ptr(self.U_old), ptr(self.old_colsum), ptr(self.ws), stream_of(c)),
+            "rebalance_lazy")
+        self.dirty = True
+
+    def fold(self, arena):
+        check(_lib.load().lakde_rebalance_fold(
+            self.N, arena.num_blocks, arena.cap, ptr(arena.nnz_dev), ptr(arena.col),
+            ptr(arena.val), ptr(self.G), ptr(self.U), stream_of(self.G)),
+            "rebalance_fold")
+        self.dirty = False
 
 
 def pred_density(rec, Y, out):

@@ -54,8 +54,8 @@ _SIGNATURES = {
                                     _vp, _i64, _i64, _i64, _i64, _i64, _int, _vp]),
     "lakde_record_floats": (_i64, [_i64]),
     "lakde_knn_prior_cov": (_int, [_vp, _i64, _i64, _i64, _i64, _vp, _int, _i64, _vp, _vp]),
-    "lakde_posterior_contribs": (_int, [_vp, _int, _vp, _int, _vp, _vp, _vp, _i64, _i64, _i64,
-                                        _f32, _vp, _vp, _vp]),
+    "lakde_posterior_contribs": (_int, [_vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _i64, _i64,
+                                        _i64, _f32, _vp, _vp, _vp]),
     "lakde_posterior": (_int, [ctypes.POINTER(PosteriorArgs), _vp]),
     "lakde_colsum": (_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
     "lakde_estep_workspace": (_sz, [_i64, _i64]),
@@ -64,6 +64,11 @@ _SIGNATURES = {
     "lakde_estep_main": (_int, [ctypes.POINTER(EstepArgs), _vp]),
     "lakde_estep_finish": (_int, [ctypes.POINTER(EstepArgs), _vp]),
     "lakde_rebalance": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lakde_rebalance_lazy_begin": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lakde_rebalance_lazy_check": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "lakde_rebalance_lazy": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _vp]),
+    "lakde_rebalance_fold": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lakde_pred_workspace": (_sz, [_i64, _i64]),
     "lakde_pred_density": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "lakde_fill_f32": (_int, [_vp, _i64, _f32, _vp]),

@@ -121,7 +121,7 @@ def _scatter(model, X, b):
     S = torch.empty(nb, D, D, device=X.device)
     nu = torch.empty(nb, device=X.device)
     F.posterior_contribs(arena.rowptr[b, :nb + 1], arena.col[b], arena.val[b], X[s:e], X,
-                         model._nu0, S, nu)
+                         model._nu0, S, nu, colscale=model._colscale())
     return S, nu
 
 

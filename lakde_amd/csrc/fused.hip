@@ -158,6 +158,126 @@ __global__ void __launch_bounds__(256) rebal_rescale_kernel(int64_t cap,
     vb[e] = vb[e] * coeff[cb[e]];
 }
 
+// ------------------------------------------------------- lazy rebalance
+// Stored value u_e, true value u_e * G[col_e]; U[m] = sum of stored values
+// of column m (double, so that a column that empties reads exactly zero up
+// to 1e-16 and is clamped to eps like the exact rebalance).
+
+// before the E-step: true column sums of the block (for the normaliser
+// update) and its stored column sums
+__global__ void __launch_bounds__(256) lazy_begin_kernel(const int32_t *__restrict__ cols,
+                                                        const float *__restrict__ vals,
+                                                        const int32_t *__restrict__ nnz_dev,
+                                                        const float *__restrict__ G,
+                                                        float *__restrict__ old_colsum,
+                                                        double *__restrict__ Uold,
+                                                        int32_t *__restrict__ unsafe) {
+  const int32_t n = *nnz_dev;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *unsafe = 0;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int32_t col = cols[e];
+    const float u = vals[e];
+    atomicAdd(&old_colsum[col], u * G[col]);
+    atomicAdd(&Uold[col], (double)u);
+  }
+}
+
+// after the E-step, before the host reads the entry count: flag columns
+// where G coeff leaves [2^-60, 2^60] (or is NaN), for which the stored
+// representation would under/overflow; the caller then rebalances exactly
+__global__ void __launch_bounds__(256) lazy_check_kernel(int64_t N, const float *__restrict__ c_prev,
+                                                        const float *__restrict__ c,
+                                                        const float *__restrict__ G,
+                                                        int32_t *__restrict__ unsafe) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (m < N) {
+    const float gc = G[m] * clamp_nan(expf(c_prev[m] - c[m]), FLT_MIN, FLT_MAX);
+    bad = !(gc >= 0x1p-60f && gc <= 0x1p60f);
+  }
+  if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(unsafe, 1);
+}
+
+// after the E-step: the block's true values r -> stored u = r / (G coeff),
+// with column sums of both
+__global__ void __launch_bounds__(256) lazy_block_kernel(const int32_t *__restrict__ cols,
+                                                        float *__restrict__ vals,
+                                                        const int32_t *__restrict__ nnz_dev,
+                                                        const float *__restrict__ c_prev,
+                                                        const float *__restrict__ c,
+                                                        const float *__restrict__ G,
+                                                        double *__restrict__ Rnew,
+                                                        double *__restrict__ Unew) {
+  const int32_t n = *nnz_dev;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int32_t col = cols[e];
+    const float r = vals[e];
+    const float coeff = clamp_nan(expf(c_prev[col] - c[col]), FLT_MIN, FLT_MAX);
+    const float u = r / (G[col] * coeff);
+    vals[e] = u;
+    atomicAdd(&Rnew[col], (double)r);
+    atomicAdd(&Unew[col], (double)u);
+  }
+}
+
+// per column: s = max(coeff G (U - Uold) + Rnew, eps), c += log s,
+// G *= coeff clamp(1/s), U += Unew - Uold; clears the per-step sums
+__global__ void __launch_bounds__(256) lazy_norm_kernel(int64_t N, const float *__restrict__ c_prev,
+                                                       float *__restrict__ c,
+                                                       float *__restrict__ G,
+                                                       double *__restrict__ U,
+                                                       double *__restrict__ Uold,
+                                                       float *__restrict__ old_colsum,
+                                                       double *__restrict__ Rnew,
+                                                       double *__restrict__ Unew) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  const float cp = c_prev[m], cn = c[m];
+  const float coeff = clamp_nan(expf(cp - cn), FLT_MIN, FLT_MAX);
+  const float g = G[m];
+  const double um = U[m] - Uold[m];
+  float s = (float)((double)(coeff * g) * um + Rnew[m]);
+  s = (s != s) ? s : fmaxf(s, FLT_EPSILON);
+  c[m] = cn + logf(s);
+  const float coeff2 = clamp_nan(1.0f / s, FLT_MIN, FLT_MAX);
+  G[m] = g * coeff * coeff2;
+  U[m] = um + Unew[m];
+  Uold[m] = 0.0;
+  old_colsum[m] = 0.f;
+  Rnew[m] = 0.0;
+  Unew[m] = 0.0;
+}
+
+// u *= G[col] over every block, U = exact stored column sums
+__global__ void __launch_bounds__(256) fold_values_kernel(int64_t cap,
+                                                         const int32_t *__restrict__ nnz,
+                                                         const int32_t *__restrict__ cols,
+                                                         float *__restrict__ vals,
+                                                         const float *__restrict__ G,
+                                                         double *__restrict__ U) {
+  const int64_t b = blockIdx.y;
+  const int32_t n = nnz[b];
+  const int32_t *cb = cols + b * cap;
+  float *vb = vals + b * cap;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int32_t col = cb[e];
+    const float v = vb[e] * G[col];
+    vb[e] = v;
+    atomicAdd(&U[col], (double)v);
+  }
+}
+
+__global__ void __launch_bounds__(256) fold_reset_kernel(int64_t N, float *__restrict__ G) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  G[m] = 1.f;
+}
+
+__global__ void __launch_bounds__(256) zero_f64_kernel(double *p, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 // ----------------------------------------------------------- predictive
 __global__ void __launch_bounds__(256) pred_combine_kernel(PParams p, float log_n) {
   const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -320,6 +440,64 @@ int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_
   hipLaunchKernelGGL(rebal_norm_kernel, dim3(gN), dim3(256), 0, s, N, c, sums, coeff);
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(rebal_rescale_kernel, ge, dim3(256), 0, s, cap, nnz, cols, vals, coeff);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int lakde_rebalance_lazy_begin(int64_t N, const int32_t *col, const float *val,
+                               const int32_t *nnz_dev, const float *G, float *old_colsum,
+                               double *U_old, int32_t *unsafe, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(N >= 1 && col && val && nnz_dev && G && old_colsum && U_old && unsafe,
+                  "bad args");
+  hipLaunchKernelGGL(lazy_begin_kernel, dim3(256), dim3(256), 0, as_stream(stream), col, val,
+                     nnz_dev, G, old_colsum, U_old, unsafe);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int lakde_rebalance_lazy_check(int64_t N, const float *c_prev, const float *c, const float *G,
+                               int32_t *unsafe, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(N >= 1 && c_prev && c && G && unsafe, "bad args");
+  hipLaunchKernelGGL(lazy_check_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0,
+                     as_stream(stream), N, c_prev, c, G, unsafe);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int lakde_rebalance_lazy(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
+                         const float *c_prev, float *c, float *G, double *U, double *U_old,
+                         float *old_colsum, double *ws, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(N >= 1 && c_prev && c && G && U && U_old && old_colsum && ws, "bad args");
+  LAKDE_CHECK_ARG((col && val && nnz_dev) || (!col && !val && !nnz_dev),
+                  "col, val and nnz_dev must all be given or all be null");
+  hipStream_t s = as_stream(stream);
+  double *Rnew = ws, *Unew = ws + N;
+  if (col) {
+    hipLaunchKernelGGL(lazy_block_kernel, dim3(256), dim3(256), 0, s, col, val, nnz_dev, c_prev,
+                       c, G, Rnew, Unew);
+    LAKDE_LAUNCHED();
+  }
+  hipLaunchKernelGGL(lazy_norm_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, N,
+                     c_prev, c, G, U, U_old, old_colsum, Rnew, Unew);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32_t *nnz,
+                         const int32_t *cols, float *vals, float *G, double *U,
+                         lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(N >= 1 && num_blocks >= 1 && num_blocks <= 65535 && cap >= 1, "bad size");
+  LAKDE_CHECK_ARG(nnz && cols && vals && G && U, "null pointer");
+  hipStream_t s = as_stream(stream);
+  const unsigned gN = (unsigned)ceil_div(N, 256);
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(gN), dim3(256), 0, s, U, N);
+  LAKDE_LAUNCHED();
+  int64_t gx = ceil_div(cap, 256 * 4);
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(fold_values_kernel, dim3((unsigned)gx, (unsigned)num_blocks), dim3(256), 0, s,
+                     cap, nnz, cols, vals, G, U);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(fold_reset_kernel, dim3(gN), dim3(256), 0, s, N, G);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }

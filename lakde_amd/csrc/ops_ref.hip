@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kOuterThreads)
 outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
                  const T *__restrict__ value, const T *__restrict__ Xn,
                  const T *__restrict__ Xm, int64_t D, int64_t K, T *__restrict__ out,
-                 T *__restrict__ nu_out, T nu0) {
+                 T *__restrict__ nu_out, T nu0, const T *__restrict__ colscale) {
   __shared__ T stage[kStageElems];
   __shared__ T wts[64];
   __shared__ T xn_s[512];
@@ -103,8 +103,11 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
         const int64_t cidx = (int64_t)col[c0 + e];
         stage[t] = Xm[cidx * D + d] - xn_s[d];
       }
-      for (int64_t e = tid; e < ne; e += kOuterThreads)
-        wts[e] = GRAPH ? T(1) : value[c0 + e];
+      for (int64_t e = tid; e < ne; e += kOuterThreads) {
+        T wv = GRAPH ? T(1) : value[c0 + e];
+        if (!GRAPH && colscale != nullptr) wv = wv * colscale[(int64_t)col[c0 + e]];
+        wts[e] = wv;
+      }
       __syncthreads();
       if (active) {
         for (int64_t e = g; e < ne; e += G) {
@@ -139,7 +142,8 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
   // nu = nu0 + sum of the row's weights, accumulated in entry order
   if (!GRAPH && nu_out != nullptr && tid == 0) {
     T s = nu0;
-    for (int64_t e = beg; e < end; ++e) s += value[e];
+    for (int64_t e = beg; e < end; ++e)
+      s += colscale != nullptr ? value[e] * colscale[(int64_t)col[e]] : value[e];
     nu_out[n] = s;
   }
 }
@@ -147,11 +151,11 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
 template <typename T, typename RP, typename CI, bool GRAPH>
 static int launch_outer(const void *rowptr, const void *col, const void *value, const void *Xn,
                         const void *Xm, int64_t N, int64_t D, int64_t K, void *out, void *nu,
-                        T nu0, hipStream_t s) {
+                        T nu0, hipStream_t s, const T *colscale = nullptr) {
   if (N == 0) return LAKDE_OK;
   hipLaunchKernelGGL((outer_sum_kernel<T, RP, CI, GRAPH>), dim3((unsigned)N), dim3(kOuterThreads),
                      0, s, (const RP *)rowptr, (const CI *)col, (const T *)value, (const T *)Xn,
-                     (const T *)Xm, D, K, (T *)out, (T *)nu, nu0);
+                     (const T *)Xm, D, K, (T *)out, (T *)nu, nu0, colscale);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }
@@ -159,15 +163,16 @@ static int launch_outer(const void *rowptr, const void *col, const void *value, 
 template <typename T>
 static int dispatch_outer_idx(int rp_dt, int col_dt, const void *rowptr, const void *col,
                               const void *value, const void *Xn, const void *Xm, int64_t N,
-                              int64_t D, void *out, void *nu, T nu0, hipStream_t s) {
+                              int64_t D, void *out, void *nu, T nu0, hipStream_t s,
+                              const T *cs = nullptr) {
   if (rp_dt == LAKDE_I32 && col_dt == LAKDE_I32)
-    return launch_outer<T, int32_t, int32_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s);
+    return launch_outer<T, int32_t, int32_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   if (rp_dt == LAKDE_I32 && col_dt == LAKDE_I64)
-    return launch_outer<T, int32_t, int64_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s);
+    return launch_outer<T, int32_t, int64_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   if (rp_dt == LAKDE_I64 && col_dt == LAKDE_I32)
-    return launch_outer<T, int64_t, int32_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s);
+    return launch_outer<T, int64_t, int32_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   if (rp_dt == LAKDE_I64 && col_dt == LAKDE_I64)
-    return launch_outer<T, int64_t, int64_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s);
+    return launch_outer<T, int64_t, int64_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   set_error("unsupported index dtype combination (int32/int64 only)");
   return LAKDE_EDTYPE;
 }
@@ -383,13 +388,13 @@ int lakde_knn_prior_cov(const float *X, int64_t N, int64_t D, int64_t row0, int6
 }
 
 int lakde_posterior_contribs(const void *rowptr, int rowptr_dtype, const void *col, int col_dtype,
-                             const float *value, const float *Xn, const float *Xm, int64_t nb,
-                             int64_t M, int64_t D, float nu0, float *S, float *nu,
-                             lakde_stream_t stream) {
+                             const float *value, const float *colscale, const float *Xn,
+                             const float *Xm, int64_t nb, int64_t M, int64_t D, float nu0,
+                             float *S, float *nu, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(rowptr && col && value && Xn && Xm && S && nu, "null pointer");
   LAKDE_CHECK_ARG(nb >= 0 && M >= 0 && D >= 1 && D <= 512, "bad size");
   return dispatch_outer_idx<float>(rowptr_dtype, col_dtype, rowptr, col, value, Xn, Xm, nb, D, S,
-                                   nu, nu0, as_stream(stream));
+                                   nu, nu0, as_stream(stream), colscale);
 }
 
 int lakde_fill_f32(float *p, int64_t n, float v, lakde_stream_t stream) {

@@ -6,12 +6,20 @@ Per partial step of kernel block b the device work is:
   prior (kNN covariance or the shared prior)           lakde_knn_prior_cov
   posterior contributions S_n, nu_n                    lakde_posterior_contribs
   factorisation, L^{-1}, logdet, trace, tau update     lakde_posterior
-  old column sums of block b                           lakde_colsum
+  old column sums of block b                           lakde_rebalance_lazy_begin
   quad forms + LSE + renormalise + compaction to CSR   lakde_estep
-  rebalance of every block                             lakde_rebalance
+  rebalance                                            lakde_rebalance_lazy
 
 with one host synchronisation (the new block's entry count and the
 stabilisation status).  Responsibilities live in an ``Arena``.
+
+The rebalance (lakde/local_adaptive_kde.py:156-181) rescales every column
+of every block by per-column factors.  The default ``rebalance="lazy"``
+keeps those factors as a per-column vector G (true value = stored value *
+G[col]) so that a partial step touches only the updated block's entries,
+and folds G into the values once per sweep and whenever the values are
+handed out; ``rebalance="exact"`` rescales the whole arena every step
+(lakde_colsum + lakde_rebalance), as the reference does.
 """
 
 import math
@@ -30,11 +38,17 @@ class LocallyAdaptiveKDE(AbstractKDE):
     precision, kept responsibilities recomputed exactly in FP32; 8 <= D <= 63).
     """
 
-    def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32"):
+    def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32",
+                 rebalance="lazy"):
         super().__init__(block_size, verbose, logs)
         if precision not in ("fp32", "bf16"):
             raise ValueError("precision must be 'fp32' or 'bf16'")
+        if rebalance not in ("lazy", "exact"):
+            raise ValueError("rebalance must be 'lazy' or 'exact'")
         self.precision = precision
+        self.rebalance = rebalance
+        self._lazy = None
+        self._unsafe_step = False
         self.nu_0 = nu_0
         self.a_0 = a_0
         self.b_0 = b_0
@@ -93,9 +107,12 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._logdet_s = torch.empty(bs, device=dev)
         self._logdet_prior_s = torch.empty(bs, device=dev)
         self._trace_s = torch.empty(bs, device=dev)
-        self._colsum = torch.empty(N, device=dev)
         self._c_new = torch.empty(N, device=dev)
-        self._rebal_ws = torch.empty(2 * N, device=dev)
+        if self.rebalance == "lazy":
+            self._lazy = F.LazyRebalance(N, dev)
+        else:
+            self._colsum = torch.empty(N, device=dev)
+            self._rebal_ws = torch.empty(2 * N, device=dev)
         self._ews = F.EstepWorkspace(N, bs, dev)
         if self.precision == "bf16":
             if not 8 <= D <= 63:
@@ -152,7 +169,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         arena = self.arena
         S = self._S[:nb]
         F.posterior_contribs(arena.rowptr[b, :nb + 1], arena.col[b], arena.val[b], X[s:e], X,
-                             self._nu0, S, nu_out)
+                             self._nu0, S, nu_out, colscale=self._colscale())
         mode, prior, prior_jitter, prior_tril = self._prior(X, b, nb)
         F.posterior(mode, prior, prior_jitter, S, nu_out, self.a[s:e], self.b[s:e], self._nu0,
                     self.W_inv_jitter[s:e], X[s:e], self._Linv[:nb], prior_tril, logdet_out,
@@ -161,8 +178,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
 
     def _sync(self, b):
         flags = self.arena.nnz_dev.cpu()
-        if int(flags[-1]) != 0:
+        nblk = self.arena.num_blocks
+        if int(flags[nblk]) != 0:
             raise RuntimeError("Stabilization did not converge")
+        self._unsafe_step = int(flags[nblk + 1]) != 0
         return int(flags[b]) if b is not None else None
 
     def _estep_into_arena(self, X, b, mode, threshold, c_old=None, old_colsum=None):
@@ -172,6 +191,8 @@ class LocallyAdaptiveKDE(AbstractKDE):
         F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, c_old=c_old,
                 old_colsum=old_colsum, arena=self.arena, b=b, timer=self._estep_timer,
                 bf16=self._bf16_operands(b, N))
+        if mode == ESTEP_UPDATE and self._lazy is not None:
+            self._lazy.check(self.arena, c_old, c)
         nnz = self._sync(b)
         if nnz > self.arena.cap:
             self.arena.grow(nnz)
@@ -207,15 +228,50 @@ class LocallyAdaptiveKDE(AbstractKDE):
         (lakde/local_adaptive_kde.py:98-181)."""
         N = X.size(0)
         arena = self.arena
-        F.colsum(arena.col[b], arena.val[b], arena.nnz_dev[b:b + 1], N, self._colsum)
+        lazy = self._lazy
+        if lazy is not None:
+            if b == 0 and lazy.dirty:
+                lazy.fold(arena)  # once per sweep: bounds the drift of G and U
+            lazy.begin(arena, b)
+            old_colsum = lazy.old_colsum
+        else:
+            F.colsum(arena.col[b], arena.val[b], arena.nnz_dev[b:b + 1], N, self._colsum)
+            old_colsum = self._colsum
         old, new = self._estep_into_arena(X, b, ESTEP_UPDATE, sparsity_threshold,
-                                          c_old=self.rnm_log_consts, old_colsum=self._colsum)
+                                          c_old=self.rnm_log_consts, old_colsum=old_colsum)
         self.rnm_active_contribs += new - old
-        F.rebalance(arena, b, self.rnm_log_consts, self._c_new, self._rebal_ws)
+        if lazy is not None:
+            if self._unsafe_step:
+                lazy.exact_step(arena, b, self.rnm_log_consts, self._c_new)
+                lazy.exact_steps += 1
+            else:
+                lazy.end(arena, b, self.rnm_log_consts, self._c_new)
+        else:
+            F.rebalance(arena, b, self.rnm_log_consts, self._c_new, self._rebal_ws)
         self.rnm_log_consts, self._c_new = self._c_new, self.rnm_log_consts
 
     def _init_rebalance(self):
-        F.rebalance(self.arena, -1, self.rnm_log_consts, self.rnm_log_consts, self._rebal_ws)
+        if self._lazy is not None:
+            self._lazy.fold(self.arena)  # G = 1, U = exact column sums
+            self._lazy.end(self.arena, None, self.rnm_log_consts, self.rnm_log_consts)
+        else:
+            F.rebalance(self.arena, -1, self.rnm_log_consts, self.rnm_log_consts,
+                        self._rebal_ws)
+
+    def _colscale(self):
+        """Per-column factors of the stored values (None when they are 1)."""
+        return self._lazy.G if self._lazy is not None and self._lazy.dirty else None
+
+    def materialize(self):
+        """Fold pending lazy-rebalance factors into the stored values, so
+        that the arena holds the true responsibilities."""
+        if self._lazy is not None and self._lazy.dirty:
+            self._lazy.fold(self.arena)
+
+    def fit(self, X, iterations=200, callbacks=None, sparsity_threshold=None, dataset_label=None):
+        super().fit(X, iterations, callbacks, sparsity_threshold, dataset_label)
+        self.materialize()
+        return self
 
     # ---------------------------------------------------------- prediction
     def log_pred_density(self, X, Y):
@@ -248,6 +304,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
     def rnm_blocks(self):
         """Per-block (indices (2, nnz) int32, values) copies, the reference
         layout (rows local to the block)."""
+        self.materialize()
         return [tuple(t.clone() for t in self.arena.block(b))
                 for b in range(self.arena.num_blocks)]
 
@@ -297,3 +354,8 @@ class LocallyAdaptiveKDE(AbstractKDE):
             ind2ptr(arena.row[b, :n], nb, out=arena.rowptr[b, :nb + 1])
         arena.nnz_dev[:nblocks] = torch.tensor(arena.nnz, dtype=torch.int32, device=X.device)
         self.arena = arena
+
+    def _finish_load(self, X):
+        self._alloc(X)
+        if self._lazy is not None:
+            self._lazy.fold(self.arena)  # G = 1, U = exact column sums

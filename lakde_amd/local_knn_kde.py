@@ -22,8 +22,8 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
     def __init__(self, nu_0: Union[torch.Tensor, float] = None,
                  k_or_knn_graph: Union[torch.Tensor, int] = None, block_size: int = 512,
                  a_0: Union[torch.Tensor, float] = 1e-6, b_0: Union[torch.Tensor, float] = 1e-6,
-                 verbose=False, logs=False, precision="fp32"):
-        super().__init__(nu_0, a_0, b_0, block_size, verbose, logs, precision)
+                 verbose=False, logs=False, precision="fp32", rebalance="lazy"):
+        super().__init__(nu_0, a_0, b_0, block_size, verbose, logs, precision, rebalance)
         self.k_or_knn_graph = k_or_knn_graph
         self.k = None
         self.knn_graph = None
@@ -110,4 +110,4 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
         for k in ("logdet_sigma_0", "sigma_0_lambda_mm_trace", "sigma_0_jitter"):
             setattr(self, k, state_dict[k].to(X.device).clone())
         self._load_blocks(list(state_dict["rnm_blocks"]), X)
-        self._alloc(X)
+        self._finish_load(X)

@@ -26,9 +26,9 @@ def _t(a, dtype=None):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
 
 
-def _fit(d, cls):
+def _fit(d, cls, rebalance="lazy"):
     X, Y = _t(d["X"]), _t(d["Y"])
-    m = cls(k_or_knn_graph=_t(d["graph"]), block_size=int(d["block_size"]))
+    m = cls(k_or_knn_graph=_t(d["graph"]), block_size=int(d["block_size"]), rebalance=rebalance)
     m.fit(X, iterations=int(d["iterations"]), sparsity_threshold=float(d["threshold"]))
     lp = m.log_pred_density(X, Y).cpu().numpy()
     return m, lp
@@ -61,12 +61,14 @@ def _check_index_sets(rows, cols, vals, d, threshold):
     return len(only_dev) + len(only_ref)
 
 
+@pytest.mark.parametrize("rebalance", ["lazy", "exact"])
 @pytest.mark.parametrize("name", ["knn_circle8_small", "knn_mix6_small", "knn_degenerate_small"])
-def test_knn_model_matches_reference(golden, name):
+def test_knn_model_matches_reference(golden, name, rebalance):
     from lakde_amd import LocalKNearestKDE
     d = golden(name)
-    print(f"[{name}] N={d['X'].shape[0]} D={d['X'].shape[1]} bs={int(d['block_size'])}", flush=True)
-    m, lp = _fit(d, LocalKNearestKDE)
+    print(f"[{name}] N={d['X'].shape[0]} D={d['X'].shape[1]} bs={int(d['block_size'])} "
+          f"rebalance={rebalance}", flush=True)
+    m, lp = _fit(d, LocalKNearestKDE, rebalance)
     ref = d["log_pred"]
     rel = abs(lp.mean() - ref.mean()) / abs(ref.mean())
     assert rel <= 1e-4, rel
@@ -83,25 +85,28 @@ def test_knn_model_matches_reference(golden, name):
     _check_index_sets(rows, cols, vals, d, float(d["threshold"]))
 
 
-def test_knn_circle_full_matches_reference(golden):
+@pytest.mark.parametrize("rebalance", ["lazy", "exact"])
+def test_knn_circle_full_matches_reference(golden, rebalance):
     """The 5,000-point, 50-iteration circle workload against the reference's
     own CPU run."""
     from lakde_amd import LocalKNearestKDE
     d = golden("knn_circle8_full")
-    m, lp = _fit(d, LocalKNearestKDE)
+    m, lp = _fit(d, LocalKNearestKDE, rebalance)
     ref = d["log_pred"]
     rel = abs(lp.mean() - ref.mean()) / abs(ref.mean())
-    print(f"circle8 full: mean log p {lp.mean():.6f} (reference {ref.mean():.6f}), rel {rel:.2e}")
+    print(f"circle8 full ({rebalance}): mean log p {lp.mean():.6f} (reference {ref.mean():.6f}), "
+          f"rel {rel:.2e}, nnz {m.arena.total_nnz()} (reference {int(d['rnm_nnz'])})")
     assert rel <= 1e-4
     np.testing.assert_allclose(lp, ref, rtol=1e-3, atol=1e-3)
     assert abs(m.arena.total_nnz() - int(d["rnm_nnz"])) <= 1e-4 * int(d["rnm_nnz"])
 
 
+@pytest.mark.parametrize("rebalance", ["lazy", "exact"])
 @pytest.mark.parametrize("name", ["hier_smooth_small", "hier_latent_small"])
-def test_hierarchical_model_matches_reference(golden, name):
+def test_hierarchical_model_matches_reference(golden, name, rebalance):
     from lakde_amd import LocalHierarchicalKDE
     d = golden(name)
-    m, lp = _fit(d, LocalHierarchicalKDE)
+    m, lp = _fit(d, LocalHierarchicalKDE, rebalance)
     ref = d["log_pred"]
     rel = abs(lp.mean() - ref.mean()) / abs(ref.mean())
     print(f"[{name}] mean log p {lp.mean():.6f} (reference {ref.mean():.6f}) rel {rel:.2e}")
@@ -151,4 +156,52 @@ def test_state_dict_round_trip(golden):
     m2.fit(X, iterations=1, sparsity_threshold=float(d["threshold"]))
     a = m.log_pred_density(X, Y).cpu().numpy()
     b = m2.log_pred_density(X, Y).cpu().numpy()
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5)
+
+
+def test_lazy_rebalance_matches_exact_mid_sweep():
+    """The lazy rebalance (per-column factors, folded once per sweep) against
+    the exact whole-arena rebalance on the same inputs, read mid-sweep (the
+    responsibilities handed out fold the pending factors) and after the fit:
+    values within 1e-4 relative on the common entries, log normalisers within
+    1e-5 relative, index sets equal up to a few threshold-crossing entries."""
+    from lakde_amd import LocalKNearestKDE
+    from lakde_amd import synthetic
+    X, Y = synthetic.make("mix32_d6", 4000, 300)
+    graph = O.compute_knn_graph(X, 32)
+    t = 1 / 32
+    Xd = _t(X)
+    snaps = {}
+
+    def grab(tag):
+        def cb(X_, model, step):
+            if step == 5:
+                snaps[tag] = (_pairs(model), model.rnm_log_consts.cpu().numpy())
+        return cb
+
+    ms = {}
+    for reb in ("lazy", "exact"):
+        m = LocalKNearestKDE(k_or_knn_graph=_t(graph), block_size=512, rebalance=reb)
+        m.fit(Xd, iterations=3, sparsity_threshold=t, callbacks=grab(reb))
+        ms[reb] = m
+    for stage in ("mid", "end"):
+        if stage == "mid":
+            (pl, cl), (pe, ce) = snaps["lazy"], snaps["exact"]
+        else:
+            pl, cl = _pairs(ms["lazy"]), ms["lazy"].rnm_log_consts.cpu().numpy()
+            pe, ce = _pairs(ms["exact"]), ms["exact"].rnm_log_consts.cpu().numpy()
+        np.testing.assert_allclose(cl, ce, rtol=1e-5, atol=1e-5)
+        N = X.shape[0]
+        kl = pl[0] * N + pl[1]
+        ke = pe[0] * N + pe[1]
+        common, il, ie = np.intersect1d(kl, ke, return_indices=True)
+        np.testing.assert_allclose(pl[2][il], pe[2][ie], rtol=1e-4, atol=1e-9)
+        # a float32 rounding difference can move an entry across the
+        # threshold at its E-step; such entries are rare
+        ndiff = kl.size + ke.size - 2 * common.size
+        assert ndiff <= 1e-4 * common.size + 5, (stage, ndiff, common.size)
+        print(f"[{stage}] common {common.size}, only lazy {kl.size - common.size}, "
+              f"only exact {ke.size - common.size}")
+    a = ms["lazy"].log_pred_density(Xd, _t(Y)).cpu().numpy()
+    b = ms["exact"].log_pred_density(Xd, _t(Y)).cpu().numpy()
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5)
