@@ -36,16 +36,27 @@ def _parse():
 
 
 class KernelTimer:
-    """HIP-event timing of one kernel launch site on the stream it runs on."""
+    """HIP-event timing of one kernel launch site on the stream it runs on.
+    Events are created up front (``reserve``) so that the timed region pays
+    only for the two records per launch."""
 
     def __init__(self):
         import torch
         self.torch = torch
+        self.pool = []
         self.pairs = []
 
+    def reserve(self, n):
+        E = self.torch.cuda.Event
+        self.pool = [(E(enable_timing=True), E(enable_timing=True)) for _ in range(n)]
+        self.pool.reverse()
+
     def __call__(self, fn):
-        ev0 = self.torch.cuda.Event(enable_timing=True)
-        ev1 = self.torch.cuda.Event(enable_timing=True)
+        if self.pool:
+            ev0, ev1 = self.pool.pop()
+        else:
+            ev0 = self.torch.cuda.Event(enable_timing=True)
+            ev1 = self.torch.cuda.Event(enable_timing=True)
         ev0.record()
         fn()
         ev1.record()
@@ -221,6 +232,7 @@ def main():
         print(f"# warmup sweep {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
 
     timer = KernelTimer()
+    timer.reserve(args.steps * nblocks)
     model._estep_timer = timer
     torch.cuda.synchronize()
     t0 = time.perf_counter()

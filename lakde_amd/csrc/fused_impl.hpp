@@ -40,15 +40,31 @@ constexpr int kEThreads = 256;  // 4 waves; lane = column, wave = kernel quarter
 // One workgroup per chunk of 64 columns (points m) and all nb kernels of the
 // block.  Pass 1 (ACCUM/UPDATE): streaming log-sum-exp over the kernels,
 // merged across the 4 waves in LDS, then the normaliser update.  Pass 2
-// (UPDATE/FIXED): recompute log rho, test the responsibility and write the
-// wave ballot as a 64-bit mask per (kernel, chunk).
+// (UPDATE/FIXED): test every responsibility against the final normaliser
+// and write the wave ballot as a 64-bit mask per (kernel, chunk).
+//
+// UPDATE mode does not recompute the quadratic forms in pass 2: the final
+// normaliser c = logaddexp(lb, lse) is bounded below, already in pass 1, by
+// B = max(lb, running max of the wave's log rho) (lb = logsubexp(c_old,
+// log old_colsum + c_old) is known up front, the log-sum-exp is at least
+// the largest term, and rounding is monotone), so a pair with
+// fl(log rho - B) <= log_t_lo can never pass keep_resp.  The other pairs
+// (typically a few per column and block) are queued per lane in LDS with
+// their log rho and tested exactly in pass 2; a wave in which some lane
+// queued more than kEQueue pairs falls back to recomputing all of its pairs.
+constexpr int kEQueue = 8;
+
 template <int D, int MODE>
 __global__ void __launch_bounds__(kEThreads)
 estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EParams p) {
   __shared__ float sm_m[4][kWave], sm_s[4][kWave], sm_c[kWave];
-  const int lane = threadIdx.x & (kWave - 1);
+  __shared__ int32_t q_n[MODE == LAKDE_ESTEP_UPDATE ? kEQueue : 1][kEThreads];
+  __shared__ float q_x[MODE == LAKDE_ESTEP_UPDATE ? kEQueue : 1][kEThreads];
+  __shared__ uint64_t sm_win[4][kWave];
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
   // wave index made explicitly uniform so the record loads below are scalar
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int64_t chunk = blockIdx.x;
   const int64_t m = chunk * kWave + lane;
   const bool valid = m < p.N;
@@ -61,8 +77,23 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
   constexpr int64_t R = rec_floats(D);
   const int64_t diag_n = m - p.row0;
   float cm = 0.f;
+  int cnt = 0;  // UPDATE: queued candidates of this lane (may exceed kEQueue)
 
   if (MODE != LAKDE_ESTEP_FIXED) {
+    float lb = neg_inf();
+    if (MODE == LAKDE_ESTEP_UPDATE && valid) {
+      const float co = p.c_old[m];
+      lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
+    }
+    auto cand = [&](float x, int64_t n, float B) {
+      if (MODE == LAKDE_ESTEP_UPDATE && valid && __fsub_rn(x, B) > p.log_t_lo) {
+        if (cnt < kEQueue) {
+          q_n[cnt][tid] = (int32_t)(n - nbeg);
+          q_x[cnt][tid] = x;
+        }
+        ++cnt;
+      }
+    };
     Lse l;
     l.init();
     int64_t n = nbeg;
@@ -77,12 +108,20 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
         if (n + 2 == diag_n) x2 = neg_inf();
         if (n + 3 == diag_n) x3 = neg_inf();
         l.add4(x0, x1, x2, x3);
+        if (MODE == LAKDE_ESTEP_UPDATE) {
+          const float B = fmaxf(lb, l.m);
+          cand(x0, n + 0, B);
+          cand(x1, n + 1, B);
+          cand(x2, n + 2, B);
+          cand(x3, n + 3, B);
+        }
       }
     }
     for (; n < nend; ++n) {
       float x = log_rho<D>(rec + n * R, xm);
       if (n == diag_n) x = neg_inf();
       l.add1(x);
+      if (MODE == LAKDE_ESTEP_UPDATE) cand(x, n, fmaxf(lb, l.m));
     }
     sm_m[w][lane] = l.m;
     sm_s[w][lane] = l.s;
@@ -94,13 +133,8 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
       for (int ww = 0; ww < 4; ++ww) t.merge(sm_m[ww][lane], sm_s[ww][lane]);
       const float lse = t.value();
       if (valid) {
-        if (MODE == LAKDE_ESTEP_ACCUM) {
-          cm = logaddexp_f(p.c[m], lse);
-        } else {
-          const float co = p.c_old[m];
-          const float oc = logf(p.old_colsum[m]) + co;
-          cm = logaddexp_f(logsubexp_f(co, oc), lse);
-        }
+        if (MODE == LAKDE_ESTEP_ACCUM) cm = logaddexp_f(p.c[m], lse);
+        else cm = logaddexp_f(lb, lse);
         p.c[m] = cm;
       }
       sm_c[lane] = cm;
@@ -111,6 +145,28 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
     cm = valid ? p.c[m] : 0.f;
   }
   if (MODE == LAKDE_ESTEP_ACCUM) return;
+
+  if (MODE == LAKDE_ESTEP_UPDATE && __ballot(cnt > kEQueue) == 0) {
+    // exact test of the queued candidates, 64 kernels at a time through a
+    // per-wave LDS window of masks (LDS operations of one wave are in order)
+    int ptr = 0;
+    for (int64_t wb = 0; wb < nend - nbeg; wb += kWave) {
+      sm_win[w][lane] = 0;
+      __builtin_amdgcn_wave_barrier();
+      while (ptr < cnt && q_n[ptr][tid] < wb + kWave) {
+        const float d = __fsub_rn(q_x[ptr][tid], cm);
+        float r;
+        if (keep_resp(d, p.threshold, p.log_t_lo, &r))
+          atomicOr((unsigned long long *)&sm_win[w][q_n[ptr][tid] - wb], 1ull << lane);
+        ++ptr;
+      }
+      __builtin_amdgcn_wave_barrier();
+      const int64_t n = nbeg + wb + lane;
+      if (n < nend) p.masks[n * p.nchunks + chunk] = sm_win[w][lane];
+      __builtin_amdgcn_wave_barrier();
+    }
+    return;
+  }
 
   for (int64_t n = nbeg; n < nend; ++n) {
     float x = log_rho<D>(rec + n * R, xm);
