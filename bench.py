@@ -36,9 +36,10 @@ def _parse():
 
 
 class KernelTimer:
-    """HIP-event timing of one kernel launch site on the stream it runs on.
-    Events are created up front (``reserve``) so that the timed region pays
-    only for the two records per launch."""
+    """HIP-event timing of the E-step main kernel on the stream it runs on:
+    the step driver records a pair of events around its launch.  Events are
+    created (and recorded once, which creates the HIP event) before the
+    timed region by ``reserve``."""
 
     def __init__(self):
         import torch
@@ -46,21 +47,20 @@ class KernelTimer:
         self.pool = []
         self.pairs = []
 
-    def reserve(self, n):
-        E = self.torch.cuda.Event
-        self.pool = [(E(enable_timing=True), E(enable_timing=True)) for _ in range(n)]
-        self.pool.reverse()
+    def _new(self):
+        ev = self.torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return ev
 
-    def __call__(self, fn):
-        if self.pool:
-            ev0, ev1 = self.pool.pop()
-        else:
-            ev0 = self.torch.cuda.Event(enable_timing=True)
-            ev1 = self.torch.cuda.Event(enable_timing=True)
-        ev0.record()
-        fn()
-        ev1.record()
-        self.pairs.append((ev0, ev1))
+    def reserve(self, n):
+        self.pool = [(self._new(), self._new()) for _ in range(n)]
+        self.pool.reverse()
+        self.torch.cuda.synchronize()
+
+    def pair(self):
+        p = self.pool.pop() if self.pool else (self._new(), self._new())
+        self.pairs.append(p)
+        return p
 
     def mean_ms(self):
         self.torch.cuda.synchronize()
@@ -140,6 +140,7 @@ def precision_form(seconds_budget=None):
     for prec in ("fp32", "bf16"):
         model.precision = prec
         timer = KernelTimer()
+        timer.reserve(steps + 1)
         model._estep_timer = timer
         model.partial_step_(X, 0, sparsity_threshold=t)  # warm-up
         torch.cuda.synchronize()

@@ -3,8 +3,8 @@
  *
  * Plain pointers, sizes and strides; no framework types.  Every entry point
  * enqueues work on `stream` (a hipStream_t; NULL = the null stream), never
- * synchronises the host, never allocates device memory, and returns a status
- * code (LAKDE_OK on success).  lakde_last_error() gives a message for the
+ * allocates device memory, and returns a status code (LAKDE_OK on success).
+ * Only lakde_partial_step synchronises the host (once, on `stream`).  lakde_last_error() gives a message for the
  * most recent failure on the calling thread.
  *
  * Device pointers are expected for every array argument.  Arguments are
@@ -39,6 +39,8 @@ typedef void *lakde_stream_t; /* hipStream_t */
 #define LAKDE_EINVAL 1      /* bad argument (size, stride, null pointer) */
 #define LAKDE_EDTYPE 2      /* unsupported dtype combination */
 #define LAKDE_EHIP 3        /* a HIP runtime call failed */
+#define LAKDE_ECAPACITY 4   /* lakde_partial_step: the block's entries exceed cap */
+#define LAKDE_ESTABILIZE 5  /* lakde_partial_step: a stabilisation did not converge */
 
 /* dtype codes */
 #define LAKDE_I32 0
@@ -243,6 +245,63 @@ int lakde_rebalance_lazy(int64_t N, const int32_t *col, float *val, const int32_
 int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32_t *nnz,
                          const int32_t *cols, float *vals, float *G, double *U,
                          lakde_stream_t stream);
+
+/* ------------------------------------------------------------------ *
+ * Partial-step driver                                                 *
+ * ------------------------------------------------------------------ */
+
+/* One partial step of kernel block `block` (lakde/local_knn_kde.py:209-239 and
+ * lakde/local_hierarchical_kde.py:273-311 up to the shared-prior update, i.e.
+ * lakde/local_adaptive_kde.py:66-181): every launch of the step from one
+ * call, then one copy of the arena counts and flags into host_flags and one
+ * stream synchronisation, then the rebalance.
+ *
+ *   knn_graph != NULL: lakde_knn_prior_cov into post.prior first;
+ *   lakde_posterior_contribs of the block's current entries (estep.rowptr,
+ *     estep.out_col, estep.out_val; colscale G) into post.S, post.nu;
+ *   lakde_posterior(post);
+ *   G != NULL (lazy): [fold_first: lakde_rebalance_fold] and
+ *     lakde_rebalance_lazy_begin; else lakde_colsum into estep.old_colsum;
+ *   bf16_frag != NULL: lakde_pack_frag_bf16 of estep.rec and
+ *     lakde_estep_main_bf16, else lakde_estep_main (ev_main[0], ev_main[1]:
+ *     optional hipEvent_t recorded around it); lakde_estep_finish;
+ *   lazy: lakde_rebalance_lazy_check;
+ *   host_flags[0 .. num_blocks+1] <- nnz_all (per-block counts, status,
+ *     unsafe flag); synchronise;
+ *   status set: LAKDE_ESTABILIZE; host_flags[block] > cap: LAKDE_ECAPACITY
+ *     (the caller grows the arena, updates the pointers and calls
+ *     lakde_partial_step_resume, which compacts again and rebalances);
+ *   otherwise the rebalance: lazy (lakde_rebalance_lazy, or for an unsafe
+ *   step fold + lakde_rebalance + fold) or lakde_rebalance(exclude block).
+ *
+ * estep must be an ESTEP_UPDATE step whose count, status and entry arrays are
+ * the block's slots of the arena: estep.nnz_dev == nnz_all + block,
+ * post.status == nnz_all + num_blocks, the unsafe flag is
+ * nnz_all[num_blocks + 1], estep.out_col/out_val == cols_all/vals_all +
+ * block * cap.  host_flags: num_blocks + 2 ints of pinned host memory. */
+typedef struct {
+  lakde_posterior_args post;
+  lakde_estep_args estep;
+  const void *knn_graph;   /* nb x K rows of the block, or NULL */
+  int knn_graph_dtype;
+  int64_t knn_k;
+  float nu0;
+  int64_t block, num_blocks, cap;
+  int32_t *nnz_all;        /* num_blocks + 2 */
+  int32_t *cols_all;       /* num_blocks x cap */
+  float *vals_all;         /* num_blocks x cap */
+  float *G;                /* N, lazy rebalance factors, or NULL (exact) */
+  double *U, *U_old;       /* N each (lazy) */
+  double *lazy_ws;         /* 2 N (lazy) */
+  float *rebal_ws;         /* 2 N: exact rebalance and the lazy fallback */
+  int fold_first;          /* lazy: fold G into the values before the step */
+  void *bf16_frag;         /* nb * lakde_bf16_frag_bytes(D) bytes, or NULL */
+  int32_t *host_flags;     /* pinned host, num_blocks + 2 */
+  void *ev_main[2];
+} lakde_step_args;
+
+int lakde_partial_step(const lakde_step_args *args, lakde_stream_t stream);
+int lakde_partial_step_resume(const lakde_step_args *args, lakde_stream_t stream);
 
 /* Streaming predictive density (lakde/local_adaptive_kde.py:200-223):
  * out[m] = -log N + LSE_n [coeff_n - (nu_n+1)/2 log1p(Q_nm)] over all N

@@ -85,14 +85,18 @@ def posterior_contribs(rowptr, col, val, Xn, X, nu0, S, nu, colscale=None):
           "posterior_contribs")
 
 
-def posterior(prior_mode, prior, prior_jitter, S, nu, a, b, nu0, W_jitter, Xn, Linv,
-              prior_tril, logdet_W, logdet_prior, trace, rec, status, b0=0.0, b_out=None):
+def posterior_args(prior_mode, prior, prior_jitter, S, nu, a, b, nu0, W_jitter, Xn, Linv,
+                   prior_tril, logdet_W, logdet_prior, trace, rec, status, b0=0.0, b_out=None):
     nb, D = Xn.shape
-    args = PosteriorArgs(nb, D, prior_mode, ptr(prior), ptr(prior_jitter), ptr(S), ptr(nu),
+    return PosteriorArgs(nb, D, prior_mode, ptr(prior), ptr(prior_jitter), ptr(S), ptr(nu),
                          ptr(a), ptr(b), float(nu0), ptr(W_jitter), ptr(Xn), ptr(Linv),
                          ptr(prior_tril), ptr(logdet_W), ptr(logdet_prior), ptr(trace), ptr(rec),
                          ptr(status), float(b0), ptr(b_out))
-    check(_lib.load().lakde_posterior(args, stream_of(Xn)), "posterior")
+
+
+def posterior(*args, **kw):
+    a = posterior_args(*args, **kw)
+    check(_lib.load().lakde_posterior(a, stream_of(args[9])), "posterior")
 
 
 def colsum(col, val, nnz_dev, N, out):
@@ -106,11 +110,8 @@ class EstepWorkspace:
         self.buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
 
 
-def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
-          arena=None, b=None, compact_only=False, timer=None, bf16=None):
-    """One E-step launch sequence.  ``bf16=frag`` (the block's fragment
-    blocks) runs the main (quadratic form / LSE / ballot) kernel on the
-    matrix cores."""
+def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
+               arena=None, b=None):
     N, D = X.shape
     nb = rec_blk.size(0)
     if arena is not None:
@@ -118,23 +119,26 @@ def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
                arena.cap, ptr(arena.nnz_dev[b:b + 1]))
     else:
         out = (ptr(None), ptr(None), ptr(None), ptr(None), 0, ptr(None))
-    args = EstepArgs(N, D, nb, row0, ptr(X), ptr(rec_blk), mode, float(threshold or 0.0),
+    return EstepArgs(N, D, nb, row0, ptr(X), ptr(rec_blk), mode, float(threshold or 0.0),
                      ptr(c_old), ptr(old_colsum), ptr(c), out[0], out[1], out[2], out[3], out[4],
                      out[5], ptr(ws.buf), ws.buf.numel())
+
+
+def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
+          arena=None, b=None, compact_only=False, bf16=None):
+    """One E-step launch sequence.  ``bf16=frag`` (the block's fragment
+    blocks) runs the main (quadratic form / LSE / ballot) kernel on the
+    matrix cores."""
+    args = estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old, old_colsum, arena, b)
     lib = _lib.load()
     st = stream_of(X)
     if compact_only:
         check(lib.lakde_estep_compact(args, st), "estep_compact")
         return
     if bf16 is not None:
-        main = lambda: check(lib.lakde_estep_main_bf16(args, ptr(bf16), st),  # noqa: E731
-                             "estep_main_bf16")
+        check(lib.lakde_estep_main_bf16(args, ptr(bf16), st), "estep_main_bf16")
     else:
-        main = lambda: check(lib.lakde_estep_main(args, st), "estep_main")  # noqa: E731
-    if timer is not None:
-        timer(main)
-    else:
-        main()
+        check(lib.lakde_estep_main(args, st), "estep_main")
     check(lib.lakde_estep_finish(args, st), "estep_finish")
 
 
@@ -151,7 +155,8 @@ class LazyRebalance:
     whose true values are u * G[col]; U (float64) are the stored column
     totals.  ``old_colsum`` (the true old column sums of the updated block,
     read by the E-step), ``U_old`` and the workspace stay zero between
-    steps."""
+    steps; ``exact_ws`` is the exact rebalance's workspace for the steps the
+    stored form cannot represent."""
 
     def __init__(self, N, device):
         self.N = N
@@ -160,32 +165,9 @@ class LazyRebalance:
         self.U_old = torch.zeros(N, dtype=torch.float64, device=device)
         self.old_colsum = torch.zeros(N, device=device)
         self.ws = torch.zeros(2 * N, dtype=torch.float64, device=device)
-        self._exact_ws = None
+        self.exact_ws = torch.empty(2 * N, device=device)
         self.dirty = False  # G != 1 somewhere
         self.exact_steps = 0
-
-    def begin(self, arena, b):
-        check(_lib.load().lakde_rebalance_lazy_begin(
-            self.N, ptr(arena.col[b]), ptr(arena.val[b]), ptr(arena.nnz_dev[b:b + 1]),
-            ptr(self.G), ptr(self.old_colsum), ptr(self.U_old), ptr(arena.unsafe_ptr),
-            stream_of(self.G)),
-            "rebalance_lazy_begin")
-
-    def check(self, arena, c_prev, c):
-        check(_lib.load().lakde_rebalance_lazy_check(
-            self.N, ptr(c_prev), ptr(c), ptr(self.G), ptr(arena.unsafe_ptr), stream_of(c)),
-            "rebalance_lazy_check")
-
-    def exact_step(self, arena, b, c_prev, c):
-        """A step the stored form cannot represent: fold, rebalance the true
-        values exactly, recompute U and clear the per-step sums."""
-        self.fold(arena)
-        if self._exact_ws is None:
-            self._exact_ws = torch.empty(2 * self.N, device=self.G.device)
-        rebalance(arena, b, c_prev, c, self._exact_ws)
-        self.fold(arena)
-        self.U_old.zero_()
-        self.old_colsum.zero_()
 
     def end(self, arena, b, c_prev, c):
         if b is None:

@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LAKDE_AMD_LIB", os.path.join(_HERE, "liblakde_amd.so"))
 
 LAKDE_I32, LAKDE_I64, LAKDE_F32, LAKDE_F64 = 0, 1, 2, 3
+LAKDE_ECAPACITY, LAKDE_ESTABILIZE = 4, 5
 ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE = 0, 1, 2
 PRIOR_KNN, PRIOR_SHARED = 0, 1
 
@@ -45,6 +46,17 @@ class EstepArgs(ctypes.Structure):
 
 
 # name -> (restype, argtypes)
+class StepArgs(ctypes.Structure):
+    _fields_ = [
+        ("post", PosteriorArgs), ("estep", EstepArgs),
+        ("knn_graph", _vp), ("knn_graph_dtype", _int), ("knn_k", _i64), ("nu0", _f32),
+        ("block", _i64), ("num_blocks", _i64), ("cap", _i64),
+        ("nnz_all", _vp), ("cols_all", _vp), ("vals_all", _vp),
+        ("G", _vp), ("U", _vp), ("U_old", _vp), ("lazy_ws", _vp), ("rebal_ws", _vp),
+        ("fold_first", _int), ("bf16_frag", _vp), ("host_flags", _vp), ("ev_main", _vp * 2),
+    ]
+
+
 _SIGNATURES = {
     "lakde_version": (_int, []),
     "lakde_last_error": (ctypes.c_char_p, []),
@@ -69,6 +81,8 @@ _SIGNATURES = {
     "lakde_rebalance_lazy": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _vp]),
     "lakde_rebalance_fold": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lakde_partial_step": (_int, [ctypes.POINTER(StepArgs), _vp]),
+    "lakde_partial_step_resume": (_int, [ctypes.POINTER(StepArgs), _vp]),
     "lakde_pred_workspace": (_sz, [_i64, _i64]),
     "lakde_pred_density": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "lakde_fill_f32": (_int, [_vp, _i64, _f32, _vp]),

@@ -1,0 +1,143 @@
+// Host-side driver of one partial step (block b): every launch of the step
+// from one C call, with the single host synchronisation it needs (the new
+// block's entry count, the stabilisation status and the lazy rebalance's
+// unsafe-step flag, copied together into pinned host memory).
+//
+// Device work, in stream order (lakde/local_knn_kde.py:209-239,
+// lakde/local_adaptive_kde.py:66-181):
+//   kNN prior covariances of the block          lakde_knn_prior_cov (optional)
+//   posterior contributions S_n, nu_n           lakde_posterior_contribs
+//   factorisation, records, tau update          lakde_posterior
+//   old column sums (+ stored sums, lazy)       lakde_rebalance_lazy_begin / lakde_colsum
+//   E-step main (+ optional events), finish     lakde_estep_main[_bf16], lakde_estep_finish
+//   unsafe-step test (lazy)                     lakde_rebalance_lazy_check
+//   -- copy of the arena counts and flags, stream synchronise --
+//   rebalance                                   lakde_rebalance_lazy / exact fallback /
+//                                               lakde_rebalance
+#include "common.hpp"
+
+using namespace lakde;
+
+namespace {
+
+int step_rebalance(const lakde_step_args *a, hipStream_t s) {
+  const lakde_estep_args &e = a->estep;
+  const int64_t N = e.N;
+  int rc;
+  if (a->G == nullptr) {
+    return lakde_rebalance(N, a->num_blocks, a->cap, a->block, a->nnz_all, a->cols_all,
+                           a->vals_all, e.c_old, e.c, a->rebal_ws, s);
+  }
+  const int32_t unsafe = a->host_flags[a->num_blocks + 1];
+  if (!unsafe) {
+    return lakde_rebalance_lazy(N, e.out_col, e.out_val, e.nnz_dev, e.c_old, e.c, a->G, a->U,
+                                a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws, s);
+  }
+  // the stored form cannot represent this step: fold, rebalance the true
+  // values exactly, recompute the totals, clear the per-step sums
+  rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all, a->G,
+                            a->U, s);
+  if (rc) return rc;
+  rc = lakde_rebalance(N, a->num_blocks, a->cap, a->block, a->nnz_all, a->cols_all, a->vals_all,
+                       e.c_old, e.c, a->rebal_ws, s);
+  if (rc) return rc;
+  rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all, a->G,
+                            a->U, s);
+  if (rc) return rc;
+  LAKDE_HIP(hipMemsetAsync(a->U_old, 0, (size_t)N * sizeof(double), s));
+  LAKDE_HIP(hipMemsetAsync(const_cast<float *>(e.old_colsum), 0, (size_t)N * sizeof(float), s));
+  return LAKDE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(a != nullptr, "null args");
+  const lakde_estep_args &e = a->estep;
+  const lakde_posterior_args &p = a->post;
+  LAKDE_CHECK_ARG(e.mode == LAKDE_ESTEP_UPDATE, "the step runs the UPDATE E-step");
+  LAKDE_CHECK_ARG(p.nb == e.nb && p.D == e.D && p.nb >= 1, "posterior and E-step sizes differ");
+  LAKDE_CHECK_ARG(a->block >= 0 && a->block < a->num_blocks, "block out of range");
+  LAKDE_CHECK_ARG(a->host_flags && a->nnz_all && a->cols_all && a->vals_all, "null pointer");
+  LAKDE_CHECK_ARG(e.nnz_dev == a->nnz_all + a->block &&
+                      (int32_t *)p.status == a->nnz_all + a->num_blocks,
+                  "the block count and status must be slots of nnz_all");
+  LAKDE_CHECK_ARG(e.out_col == a->cols_all + a->block * a->cap &&
+                      e.out_val == a->vals_all + a->block * a->cap && e.cap == a->cap,
+                  "the block's entries must be its arena slots");
+  LAKDE_CHECK_ARG(a->G != nullptr || a->rebal_ws != nullptr, "no rebalance workspace");
+  LAKDE_CHECK_ARG(a->G == nullptr || (a->U && a->U_old && a->lazy_ws && a->rebal_ws),
+                  "lazy rebalance state incomplete");
+  hipStream_t s = as_stream(stream);
+  const int64_t N = e.N, D = e.D, nb = e.nb;
+  int32_t *unsafe = a->nnz_all + a->num_blocks + 1;
+  int rc;
+
+  if (a->knn_graph != nullptr) {
+    rc = lakde_knn_prior_cov(e.X, N, D, e.row0, nb, a->knn_graph, a->knn_graph_dtype, a->knn_k,
+                             const_cast<float *>(p.prior), s);
+    if (rc) return rc;
+  }
+  rc = lakde_posterior_contribs(e.rowptr, LAKDE_I32, e.out_col, LAKDE_I32, e.out_val, a->G,
+                                e.X + e.row0 * D, e.X, nb, N, D, a->nu0,
+                                const_cast<float *>(p.S), const_cast<float *>(p.nu), s);
+  if (rc) return rc;
+  rc = lakde_posterior(&p, s);
+  if (rc) return rc;
+  if (a->G != nullptr) {
+    if (a->fold_first) {
+      rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all,
+                                a->G, a->U, s);
+      if (rc) return rc;
+    }
+    rc = lakde_rebalance_lazy_begin(N, e.out_col, e.out_val, e.nnz_dev, a->G,
+                                    const_cast<float *>(e.old_colsum), a->U_old, unsafe, s);
+  } else {
+    rc = lakde_colsum(e.out_col, e.out_val, e.nnz_dev, N, const_cast<float *>(e.old_colsum), s);
+  }
+  if (rc) return rc;
+  if (a->bf16_frag != nullptr) {
+    rc = lakde_pack_frag_bf16(e.rec, nb, D, a->bf16_frag, s);
+    if (rc) return rc;
+  }
+  if (a->ev_main[0]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[0], s));
+  rc = a->bf16_frag != nullptr ? lakde_estep_main_bf16(&e, a->bf16_frag, s)
+                               : lakde_estep_main(&e, s);
+  if (rc) return rc;
+  if (a->ev_main[1]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[1], s));
+  rc = lakde_estep_finish(&e, s);
+  if (rc) return rc;
+  if (a->G != nullptr) {
+    rc = lakde_rebalance_lazy_check(N, e.c_old, e.c, a->G, unsafe, s);
+    if (rc) return rc;
+  }
+  LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all, (size_t)(a->num_blocks + 2) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, s));
+  LAKDE_HIP(hipStreamSynchronize(s));
+  if (a->host_flags[a->num_blocks] != 0) {
+    set_error("lakde_partial_step: Stabilization did not converge");
+    return LAKDE_ESTABILIZE;
+  }
+  if (a->host_flags[a->block] > a->cap) {
+    set_error("lakde_partial_step: the block's entries exceed the arena capacity");
+    return LAKDE_ECAPACITY;
+  }
+  return step_rebalance(a, s);
+}
+
+int lakde_partial_step_resume(const lakde_step_args *a, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(a != nullptr, "null args");
+  LAKDE_CHECK_ARG(a->host_flags[a->block] <= a->cap, "capacity still too small");
+  LAKDE_CHECK_ARG(a->estep.out_col == a->cols_all + a->block * a->cap &&
+                      a->estep.out_val == a->vals_all + a->block * a->cap &&
+                      a->estep.cap == a->cap,
+                  "the block's entries must be its arena slots");
+  hipStream_t s = as_stream(stream);
+  int rc = lakde_estep_compact(&a->estep, s);
+  if (rc) return rc;
+  return step_rebalance(a, s);
+}
+
+}  // extern "C"

@@ -1,17 +1,19 @@
 """Locally adaptive KDE core on the fused HIP path
 (mirrors lakde/local_adaptive_kde.py:47-314).
 
-Per partial step of kernel block b the device work is:
+A partial step of kernel block b is one call of the native step driver
+(lakde_partial_step), which launches
 
-  prior (kNN covariance or the shared prior)           lakde_knn_prior_cov
+  prior (kNN covariance; the shared prior is an input) lakde_knn_prior_cov
   posterior contributions S_n, nu_n                    lakde_posterior_contribs
   factorisation, L^{-1}, logdet, trace, tau update     lakde_posterior
   old column sums of block b                           lakde_rebalance_lazy_begin
   quad forms + LSE + renormalise + compaction to CSR   lakde_estep
   rebalance                                            lakde_rebalance_lazy
 
-with one host synchronisation (the new block's entry count and the
-stabilisation status).  Responsibilities live in an ``Arena``.
+with one host synchronisation (the new block's entry count, the
+stabilisation status, the lazy rebalance's unsafe-step flag).
+Responsibilities live in an ``Arena``.
 
 The rebalance (lakde/local_adaptive_kde.py:156-181) rescales every column
 of every block by per-column factors.  The default ``rebalance="lazy"``
@@ -22,12 +24,15 @@ handed out; ``rebalance="exact"`` rescales the whole arena every step
 (lakde_colsum + lakde_rebalance), as the reference does.
 """
 
+import ctypes
 import math
 
 import torch
 
 from . import _fused as F
-from ._lib import ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE, LakdeError, require_gpu
+from . import _lib
+from ._lib import (ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE, LAKDE_ECAPACITY, LAKDE_ESTABILIZE,
+                   LakdeError, check, require_gpu, stream_of)
 from .kdes import AbstractKDE
 from .utils import tensor_as
 
@@ -48,7 +53,6 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self.precision = precision
         self.rebalance = rebalance
         self._lazy = None
-        self._unsafe_step = False
         self.nu_0 = nu_0
         self.a_0 = a_0
         self.b_0 = b_0
@@ -59,8 +63,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self.rnm_log_consts = None
         self.logdet_W = None
         self.W_inv_jitter = None
-        self.rnm_active_contribs = None
-        self._estep_timer = None  # optional callable wrapping the E-step main launch
+        self._active = None         # rnm_active_contribs, kept on the host
+        self._active_device = None
+        self._step_cache = {}
+        self._estep_timer = None  # optional: .pair() -> two recorded torch events
 
     # ------------------------------------------------------------ helpers
     def _blocks(self, N):
@@ -113,6 +119,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
         else:
             self._colsum = torch.empty(N, device=dev)
             self._rebal_ws = torch.empty(2 * N, device=dev)
+        self._host_flags = torch.zeros(self.arena.num_blocks + 2, dtype=torch.int32,
+                                       pin_memory=True)
+        self._flags = self._host_flags.numpy()
+        self._step_cache = {}
         self._ews = F.EstepWorkspace(N, bs, dev)
         if self.precision == "bf16":
             if not 8 <= D <= 63:
@@ -153,11 +163,30 @@ class LocallyAdaptiveKDE(AbstractKDE):
             s, e, nb = self._span(b, N)
             ind2ptr(arena.row[b, :arena.nnz[b]], nb, out=arena.rowptr[b, :nb + 1])
         self.arena = arena
-        self.rnm_active_contribs = X.new_tensor(float(sum(arena.nnz)))
+        self._step_cache = {}
+        self._active_device = X.device
+        self._active = float(sum(arena.nnz))
+
+    @property
+    def rnm_active_contribs(self):
+        """Number of kept responsibilities (lakde/local_adaptive_kde.py:146)."""
+        if self._active is None:
+            return None
+        return torch.tensor(self._active, dtype=torch.float32, device=self._active_device)
+
+    @rnm_active_contribs.setter
+    def rnm_active_contribs(self, v):
+        self._active = None if v is None else float(v)
+        if torch.is_tensor(v):
+            self._active_device = v.device
 
     # ---------------------------------------------------- posterior of a block
-    def _prior(self, X, b, nb):
-        """Returns (prior_mode, prior, prior_jitter, prior_tril, logdet_prior)."""
+    def _prior_spec(self, X, b, nb):
+        """(prior_mode, prior, prior_jitter, prior_tril, knn_graph rows or None)."""
+        raise NotImplementedError
+
+    def _step_outputs(self, b, nb):
+        """Posterior outputs of a partial step: (nu, logdet_W, trace, logdet_prior)."""
         raise NotImplementedError
 
     def _block_posterior(self, X, b, nu_out, logdet_out, trace_out, logdet_prior_out,
@@ -170,7 +199,9 @@ class LocallyAdaptiveKDE(AbstractKDE):
         S = self._S[:nb]
         F.posterior_contribs(arena.rowptr[b, :nb + 1], arena.col[b], arena.val[b], X[s:e], X,
                              self._nu0, S, nu_out, colscale=self._colscale())
-        mode, prior, prior_jitter, prior_tril = self._prior(X, b, nb)
+        mode, prior, prior_jitter, prior_tril, graph = self._prior_spec(X, b, nb)
+        if graph is not None:
+            F.knn_prior_cov(X, s, graph, prior)
         F.posterior(mode, prior, prior_jitter, S, nu_out, self.a[s:e], self.b[s:e], self._nu0,
                     self.W_inv_jitter[s:e], X[s:e], self._Linv[:nb], prior_tril, logdet_out,
                     logdet_prior_out, trace_out, self.rec[s:e], arena.status_ptr,
@@ -178,26 +209,23 @@ class LocallyAdaptiveKDE(AbstractKDE):
 
     def _sync(self, b):
         flags = self.arena.nnz_dev.cpu()
-        nblk = self.arena.num_blocks
-        if int(flags[nblk]) != 0:
+        if int(flags[self.arena.num_blocks]) != 0:
             raise RuntimeError("Stabilization did not converge")
-        self._unsafe_step = int(flags[nblk + 1]) != 0
         return int(flags[b]) if b is not None else None
 
-    def _estep_into_arena(self, X, b, mode, threshold, c_old=None, old_colsum=None):
+    def _estep_into_arena(self, X, b, mode, threshold):
+        """A FIXED-mode E-step of block b into the arena (initialisation)."""
         N = X.size(0)
         s, e, nb = self._span(b, N)
-        c = self._c_new if mode == ESTEP_UPDATE else self.rnm_log_consts
-        F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, c_old=c_old,
-                old_colsum=old_colsum, arena=self.arena, b=b, timer=self._estep_timer,
+        c = self.rnm_log_consts
+        F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, arena=self.arena, b=b,
                 bf16=self._bf16_operands(b, N))
-        if mode == ESTEP_UPDATE and self._lazy is not None:
-            self._lazy.check(self.arena, c_old, c)
         nnz = self._sync(b)
         if nnz > self.arena.cap:
             self.arena.grow(nnz)
-            F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, c_old=c_old,
-                    old_colsum=old_colsum, arena=self.arena, b=b, compact_only=True)
+            self._step_cache = {}
+            F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, arena=self.arena, b=b,
+                    compact_only=True)
         old = self.arena.nnz[b]
         self.arena.nnz[b] = nnz
         return old, nnz
@@ -221,34 +249,96 @@ class LocallyAdaptiveKDE(AbstractKDE):
     def _init_fixed_pass(self, X, b, sparsity_threshold):
         """partial_expectation_step(update_normalization=False)."""
         old, new = self._estep_into_arena(X, b, ESTEP_FIXED, sparsity_threshold)
-        self.rnm_active_contribs += new
+        self._active += new
 
-    def _update_step(self, X, b, sparsity_threshold):
-        """partial_expectation_step + rebalance_responsibilities_
-        (lakde/local_adaptive_kde.py:98-181)."""
-        N = X.size(0)
+    # --------------------------------------------------------- partial step
+    def _step_args(self, X, b):
+        """The step driver's arguments for block b, built once per arena
+        layout; the per-step fields are set by _partial_step."""
+        a = self._step_cache.get(b)
+        if a is not None:
+            return a
+        N, D = X.shape
+        s, e, nb = self._span(b, N)
         arena = self.arena
+        mode, prior, prior_jitter, prior_tril, graph = self._prior_spec(X, b, nb)
+        nu_out, logdet_out, trace_out, logdet_prior_out = self._step_outputs(b, nb)
         lazy = self._lazy
+        a = _lib.StepArgs()
+        a.post = F.posterior_args(mode, prior, prior_jitter, self._S[:nb], nu_out, self.a[s:e],
+                                  self.b[s:e], self._nu0, self.W_inv_jitter[s:e], X[s:e],
+                                  self._Linv[:nb], prior_tril, logdet_out, logdet_prior_out,
+                                  trace_out, self.rec[s:e], arena.status_ptr, b0=self._b0,
+                                  b_out=self.b[s:e])
+        a.estep = F.estep_args(X, self.rec[s:e], s, ESTEP_UPDATE, 0.0, self._c_new, self._ews,
+                               c_old=self.rnm_log_consts,
+                               old_colsum=lazy.old_colsum if lazy is not None else self._colsum,
+                               arena=arena, b=b)
+        if graph is not None:
+            a.knn_graph = graph.data_ptr()
+            a.knn_graph_dtype = _lib.idx_code(graph.dtype)
+            a.knn_k = graph.size(1)
+        a.nu0 = self._nu0
+        a.block, a.num_blocks, a.cap = b, arena.num_blocks, arena.cap
+        a.nnz_all = arena.nnz_dev.data_ptr()
+        a.cols_all = arena.col.data_ptr()
+        a.vals_all = arena.val.data_ptr()
         if lazy is not None:
-            if b == 0 and lazy.dirty:
-                lazy.fold(arena)  # once per sweep: bounds the drift of G and U
-            lazy.begin(arena, b)
-            old_colsum = lazy.old_colsum
+            a.G, a.U, a.U_old = lazy.G.data_ptr(), lazy.U.data_ptr(), lazy.U_old.data_ptr()
+            a.lazy_ws = lazy.ws.data_ptr()
+            a.rebal_ws = lazy.exact_ws.data_ptr()
         else:
-            F.colsum(arena.col[b], arena.val[b], arena.nnz_dev[b:b + 1], N, self._colsum)
-            old_colsum = self._colsum
-        old, new = self._estep_into_arena(X, b, ESTEP_UPDATE, sparsity_threshold,
-                                          c_old=self.rnm_log_consts, old_colsum=old_colsum)
-        self.rnm_active_contribs += new - old
+            a.rebal_ws = self._rebal_ws.data_ptr()
+        a.host_flags = self._host_flags.data_ptr()
+        self._step_cache[b] = a
+        return a
+
+    def _partial_step(self, X, b, sparsity_threshold):
+        """Posterior of block b, partial_expectation_step and
+        rebalance_responsibilities_ (lakde/local_adaptive_kde.py:66-181) in
+        one driver call."""
+        a = self._step_args(X, b)
+        lazy = self._lazy
+        a.estep.c_old = self.rnm_log_consts.data_ptr()
+        a.estep.c = self._c_new.data_ptr()
+        a.estep.threshold = float(sparsity_threshold or 0.0)
+        a.post.prior = self._prior_ptr(b, a.post.prior)
+        a.fold_first = int(lazy is not None and b == 0 and lazy.dirty)
+        a.bf16_frag = self._frag_blk.data_ptr() if self.precision == "bf16" else None
+        if self._estep_timer is not None:
+            e0, e1 = self._estep_timer.pair()
+            a.ev_main[0], a.ev_main[1] = e0.cuda_event, e1.cuda_event
+        else:
+            a.ev_main[0] = a.ev_main[1] = None
+        lib = _lib.load()
+        st = stream_of(X)
+        rc = lib.lakde_partial_step(ctypes.byref(a), st)
+        if rc == LAKDE_ECAPACITY:
+            self.arena.grow(int(self._flags[b]))
+            self._step_cache = {}
+            a2 = self._step_args(X, b)
+            for f in ("fold_first", "bf16_frag"):
+                setattr(a2, f, getattr(a, f))
+            a2.estep.c_old, a2.estep.c = a.estep.c_old, a.estep.c
+            a2.estep.threshold = a.estep.threshold
+            a2.post.prior = a.post.prior
+            rc = lib.lakde_partial_step_resume(ctypes.byref(a2), st)
+        if rc == LAKDE_ESTABILIZE:
+            raise RuntimeError("Stabilization did not converge")
+        check(rc, "partial_step")
+        nnz = int(self._flags[b])
+        self._active += nnz - self.arena.nnz[b]
+        self.arena.nnz[b] = nnz
         if lazy is not None:
-            if self._unsafe_step:
-                lazy.exact_step(arena, b, self.rnm_log_consts, self._c_new)
-                lazy.exact_steps += 1
-            else:
-                lazy.end(arena, b, self.rnm_log_consts, self._c_new)
-        else:
-            F.rebalance(arena, b, self.rnm_log_consts, self._c_new, self._rebal_ws)
+            unsafe = int(self._flags[self.arena.num_blocks + 1]) != 0
+            lazy.dirty = not unsafe  # an unsafe step ends folded (G = 1)
+            lazy.exact_steps += int(unsafe)
         self.rnm_log_consts, self._c_new = self._c_new, self.rnm_log_consts
+
+    def _prior_ptr(self, b, current):
+        """Device address of the prior for block b's step (the shared prior
+        of the hierarchical model is a new tensor every step)."""
+        return current
 
     def _init_rebalance(self):
         if self._lazy is not None:

@@ -83,8 +83,16 @@ class LocalHierarchicalKDE(LocallyAdaptiveKDE):
         return (self.expect_lambda_contribs.sum(dim=0).mul_(self.nu_0)
                 .add_(self.lambda_0, alpha=float(self.nu_shared_0)))
 
-    def _prior(self, X, b, nb):
-        return PRIOR_SHARED, self._sigma_bar32, None, None
+    def _prior_spec(self, X, b, nb):
+        return PRIOR_SHARED, self._sigma_bar32, None, None, None
+
+    def _step_outputs(self, b, nb):
+        s = b * self.block_size
+        e = s + nb
+        return self.nu[s:e], self.logdet_W[s:e], self._trace_s[:nb], None
+
+    def _prior_ptr(self, b, current):
+        return self._sigma_bar32.data_ptr()
 
     def _set_sigma_bar(self):
         sigma_bar, _ = self.calc_prior_cov()
@@ -109,7 +117,7 @@ class LocalHierarchicalKDE(LocallyAdaptiveKDE):
             self.expect_lambda_contribs[b] = self._lambda_contrib(nb, self.nu[s:e], self.b[s:e],
                                                                   s, e)
             self._init_lse_pass(X, b)
-        self.rnm_active_contribs.zero_()
+        self._active = 0.0
         for b in self._blocks(N):
             s, e, nb = self._span(b, N)
             self._block_posterior(X, b, self._nu_s[:nb], self._logdet_s[:nb],
@@ -124,11 +132,10 @@ class LocalHierarchicalKDE(LocallyAdaptiveKDE):
         self._set_sigma_bar()
         # b[s:e] <- b0 + nu0 * trace / 2 is written by the posterior kernel
         # after E[tau] has been read; nothing in the E-step reads b
-        self._block_posterior(X, block_idx, self.nu[s:e], self.logdet_W[s:e],
-                              self._trace_s[:nb], None, update_b=True)
-        lam = self._lambda_contrib(nb, self.nu[s:e], self.b[s:e], s, e)
-        self._update_step(X, block_idx, sparsity_threshold)
-        self.expect_lambda_contribs[block_idx] = lam
+        self._partial_step(X, block_idx, sparsity_threshold)
+        # nu, b and the factors are not touched by the E-step
+        self.expect_lambda_contribs[block_idx] = self._lambda_contrib(nb, self.nu[s:e],
+                                                                      self.b[s:e], s, e)
         W_shared_inv = self.calc_prior_prec()
         W_shared_inv_tril, self.W_shared_inv_jitter = stabilized_cholesky(
             W_shared_inv, self.W_shared_inv_jitter)

@@ -58,11 +58,17 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
 
     # lakde/local_knn_kde.py:195-207 (prior + its jittered Cholesky happen in
     # the posterior kernel)
-    def _prior(self, X, b, nb):
+    def _prior_spec(self, X, b, nb):
         N = X.size(0)
         s, e, _ = self._span(b, N)
-        F.knn_prior_cov(X, s, self.knn_graph[s:e], self._prior_buf[:nb])
-        return PRIOR_KNN, self._prior_buf[:nb], self.sigma_0_jitter[s:e], self._prior_tril[:nb]
+        return (PRIOR_KNN, self._prior_buf[:nb], self.sigma_0_jitter[s:e], self._prior_tril[:nb],
+                self.knn_graph[s:e])
+
+    def _step_outputs(self, b, nb):
+        s = b * self.block_size
+        e = s + nb
+        return (self.nu[s:e], self.logdet_W[s:e], self.sigma_0_lambda_mm_trace[s:e],
+                self.logdet_sigma_0[s:e])
 
     # lakde/local_knn_kde.py:140-193
     def init_summaries_(self, X, sparsity_threshold=None):
@@ -73,7 +79,7 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
                                   self.sigma_0_lambda_mm_trace[s:e], self.logdet_sigma_0[s:e],
                                   update_b=False)
             self._init_lse_pass(X, b)
-        self.rnm_active_contribs.zero_()
+        self._active = 0.0
         for b in self._blocks(N):
             s, e, nb = self._span(b, N)
             self._block_posterior(X, b, self._nu_s[:nb], self._logdet_s[:nb],
@@ -83,14 +89,9 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
 
     # lakde/local_knn_kde.py:209-239
     def partial_step_(self, X, block_idx, sparsity_threshold=None):
-        N = X.size(0)
-        s, e, nb = self._span(block_idx, N)
         # nu, logdet_W, logdet_sigma_0, the trace term and b are written in
-        # place; none of them is read by the E-step of this block
-        self._block_posterior(X, block_idx, self.nu[s:e], self.logdet_W[s:e],
-                              self.sigma_0_lambda_mm_trace[s:e], self.logdet_sigma_0[s:e],
-                              update_b=True)
-        self._update_step(X, block_idx, sparsity_threshold)
+        # place by the posterior; none of them is read by the E-step
+        self._partial_step(X, block_idx, sparsity_threshold)
 
     def state_dict(self):
         d = super().state_dict()
