@@ -271,8 +271,10 @@ int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32
  *   status set: LAKDE_ESTABILIZE; host_flags[block] > cap: LAKDE_ECAPACITY
  *     (the caller grows the arena, updates the pointers and calls
  *     lakde_partial_step_resume, which compacts again and rebalances);
- *   otherwise the rebalance: lazy (lakde_rebalance_lazy, or for an unsafe
- *   step fold + lakde_rebalance + fold) or lakde_rebalance(exclude block).
+ *   otherwise the rebalance: lazy (lakde_rebalance_lazy, enqueued before the
+ *   copy with a device-side guard that skips it for an unsafe or
+ *   overflowing step; for an unsafe step fold + lakde_rebalance + fold
+ *   after the synchronisation) or lakde_rebalance(exclude block).
  *
  * estep must be an ESTEP_UPDATE step whose count, status and entry arrays are
  * the block's slots of the arena: estep.nnz_dev == nnz_all + block,
@@ -302,6 +304,18 @@ typedef struct {
 
 int lakde_partial_step(const lakde_step_args *args, lakde_stream_t stream);
 int lakde_partial_step_resume(const lakde_step_args *args, lakde_stream_t stream);
+
+/* Partial steps of blocks first .. first + count - 1 in order (one VI
+ * iteration when first = 0 and count = num_blocks; lakde/kdes.py:64-67):
+ * steps[b] are the arguments of block b.  Step b reads the normalisers
+ * c[*cur] and writes c[1 - *cur] (estep.c_old / estep.c are set here), after
+ * which *cur flips.  Lazy rebalance: fold_first is set for block 0 when
+ * *lazy_dirty, and *lazy_dirty is updated after every step.  Stops at the
+ * first step that does not return LAKDE_OK, with *done = steps completed
+ * (LAKDE_ECAPACITY: grow the arena, rebuild the arguments, finish that step
+ * with lakde_partial_step_resume, flip *cur and continue after it). */
+int lakde_partial_sweep(lakde_step_args *steps, int64_t first, int64_t count, float *const *c,
+                        int *cur, int *lazy_dirty, int64_t *done, lakde_stream_t stream);
 
 /* Streaming predictive density (lakde/local_adaptive_kde.py:200-223):
  * out[m] = -log N + LSE_n [coeff_n - (nu_n+1)/2 log1p(Q_nm)] over all N

@@ -167,7 +167,6 @@ class LazyRebalance:
         self.ws = torch.zeros(2 * N, dtype=torch.float64, device=device)
         self.exact_ws = torch.empty(2 * N, device=device)
         self.dirty = False  # G != 1 somewhere
-        self.exact_steps = 0
 
     def end(self, arena, b, c_prev, c):
         if b is None:

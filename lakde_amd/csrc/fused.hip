@@ -198,6 +198,13 @@ __global__ void __launch_bounds__(256) lazy_check_kernel(int64_t N, const float 
   if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(unsafe, 1);
 }
 
+// A guard lets the step driver enqueue the lazy rebalance before it knows
+// the step's outcome: the kernels do nothing when the unsafe flag is set or
+// the block's entries did not fit the arena (the host then finishes the step).
+__device__ __forceinline__ bool lazy_skip(const LazyGuard &g) {
+  return g.unsafe != nullptr && (*g.unsafe != 0 || (int64_t)*g.nnz > g.cap);
+}
+
 // after the E-step: the block's true values r -> stored u = r / (G coeff),
 // with column sums of both
 __global__ void __launch_bounds__(256) lazy_block_kernel(const int32_t *__restrict__ cols,
@@ -207,7 +214,9 @@ __global__ void __launch_bounds__(256) lazy_block_kernel(const int32_t *__restri
                                                         const float *__restrict__ c,
                                                         const float *__restrict__ G,
                                                         double *__restrict__ Rnew,
-                                                        double *__restrict__ Unew) {
+                                                        double *__restrict__ Unew,
+                                                        LazyGuard guard) {
+  if (lazy_skip(guard)) return;
   const int32_t n = *nnz_dev;
   for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const int32_t col = cols[e];
@@ -229,9 +238,10 @@ __global__ void __launch_bounds__(256) lazy_norm_kernel(int64_t N, const float *
                                                        double *__restrict__ Uold,
                                                        float *__restrict__ old_colsum,
                                                        double *__restrict__ Rnew,
-                                                       double *__restrict__ Unew) {
+                                                       double *__restrict__ Unew,
+                                                       LazyGuard guard) {
   const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (m >= N) return;
+  if (m >= N || lazy_skip(guard)) return;
   const float cp = c_prev[m], cn = c[m];
   const float coeff = clamp_nan(expf(cp - cn), FLT_MIN, FLT_MAX);
   const float g = G[m];
@@ -365,6 +375,24 @@ int pred_combine(const PParams &p, lakde_stream_t stream) {
   return LAKDE_OK;
 }
 
+int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
+                           const float *c_prev, float *c, float *G, double *U, double *U_old,
+                           float *old_colsum, double *ws, LazyGuard guard, hipStream_t s) {
+  LAKDE_CHECK_ARG(N >= 1 && c_prev && c && G && U && U_old && old_colsum && ws, "bad args");
+  LAKDE_CHECK_ARG((col && val && nnz_dev) || (!col && !val && !nnz_dev),
+                  "col, val and nnz_dev must all be given or all be null");
+  double *Rnew = ws, *Unew = ws + N;
+  if (col) {
+    hipLaunchKernelGGL(lazy_block_kernel, dim3(256), dim3(256), 0, s, col, val, nnz_dev, c_prev,
+                       c, G, Rnew, Unew, guard);
+    LAKDE_LAUNCHED();
+  }
+  hipLaunchKernelGGL(lazy_norm_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, N,
+                     c_prev, c, G, U, U_old, old_colsum, Rnew, Unew, guard);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
 }  // namespace lakde
 
 using namespace lakde;
@@ -467,20 +495,8 @@ int lakde_rebalance_lazy_check(int64_t N, const float *c_prev, const float *c, c
 int lakde_rebalance_lazy(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
                          const float *c_prev, float *c, float *G, double *U, double *U_old,
                          float *old_colsum, double *ws, lakde_stream_t stream) {
-  LAKDE_CHECK_ARG(N >= 1 && c_prev && c && G && U && U_old && old_colsum && ws, "bad args");
-  LAKDE_CHECK_ARG((col && val && nnz_dev) || (!col && !val && !nnz_dev),
-                  "col, val and nnz_dev must all be given or all be null");
-  hipStream_t s = as_stream(stream);
-  double *Rnew = ws, *Unew = ws + N;
-  if (col) {
-    hipLaunchKernelGGL(lazy_block_kernel, dim3(256), dim3(256), 0, s, col, val, nnz_dev, c_prev,
-                       c, G, Rnew, Unew);
-    LAKDE_LAUNCHED();
-  }
-  hipLaunchKernelGGL(lazy_norm_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, N,
-                     c_prev, c, G, U, U_old, old_colsum, Rnew, Unew);
-  LAKDE_LAUNCHED();
-  return LAKDE_OK;
+  return rebalance_lazy_guarded(N, col, val, nnz_dev, c_prev, c, G, U, U_old, old_colsum, ws,
+                                LazyGuard{nullptr, 0, nullptr}, as_stream(stream));
 }
 
 int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32_t *nnz,

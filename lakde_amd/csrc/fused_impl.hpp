@@ -368,7 +368,18 @@ int launch_bwd_y(const PParams &p, const float *lse, const float *g, float *part
   return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
 }
 
+// guard of the lazy rebalance kernels enqueued by the step driver before
+// the host has read the step's flags (unsafe == nullptr: no guard)
+struct LazyGuard {
+  const int32_t *nnz;     // the block's entry count
+  int64_t cap;
+  const int32_t *unsafe;  // the unsafe-step flag
+};
+
 // shared host helpers (fused.hip)
+int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
+                           const float *c_prev, float *c, float *G, double *U, double *U_old,
+                           float *old_colsum, double *ws, LazyGuard guard, hipStream_t s);
 void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit);
 int estep_fill_params(const lakde_estep_args &a, EParams *p);
 int pred_fill_params(const float *rec, int64_t N, int64_t M, float *out, void *workspace,

@@ -67,6 +67,7 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
   else { beg = (int64_t)rowptr[n]; end = (int64_t)rowptr[n + 1]; }
 
   for (int64_t d = tid; d < D; d += kOuterThreads) xn_s[d] = Xn[n * D + d];
+  T nu_acc = nu0;
 
   // pair batches (a single batch for D <= 70)
   for (int64_t pbase = 0; pbase < P; pbase += (int64_t)kOuterThreads * kMaxPairs) {
@@ -109,6 +110,10 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
         wts[e] = wv;
       }
       __syncthreads();
+      // nu = nu0 + sum of the row's weights in entry order, from the staged
+      // weights (first pair batch only)
+      if (!GRAPH && nu_out != nullptr && pbase == 0 && tid == 0)
+        for (int64_t e = 0; e < ne; ++e) nu_acc += wts[e];
       if (active) {
         for (int64_t e = g; e < ne; e += G) {
           const T w = wts[e];
@@ -139,13 +144,7 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
       }
     }
   }
-  // nu = nu0 + sum of the row's weights, accumulated in entry order
-  if (!GRAPH && nu_out != nullptr && tid == 0) {
-    T s = nu0;
-    for (int64_t e = beg; e < end; ++e)
-      s += colscale != nullptr ? value[e] * colscale[(int64_t)col[e]] : value[e];
-    nu_out[n] = s;
-  }
+  if (!GRAPH && nu_out != nullptr && tid == 0) nu_out[n] = nu_acc;
 }
 
 template <typename T, typename RP, typename CI, bool GRAPH>

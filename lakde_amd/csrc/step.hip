@@ -14,11 +14,21 @@
 //   -- copy of the arena counts and flags, stream synchronise --
 //   rebalance                                   lakde_rebalance_lazy / exact fallback /
 //                                               lakde_rebalance
-#include "common.hpp"
+#include "fused_impl.hpp"
 
 using namespace lakde;
 
 namespace {
+
+// Wait for the stream by polling it: the step's wait is on the critical
+// path of every partial step, and a blocking synchronisation adds the
+// host's wake-up latency to it.
+hipError_t wait_stream(hipStream_t s) {
+  hipError_t q;
+  while ((q = hipStreamQuery(s)) == hipErrorNotReady) {
+  }
+  return q;
+}
 
 int step_rebalance(const lakde_step_args *a, hipStream_t s) {
   const lakde_estep_args &e = a->estep;
@@ -112,10 +122,16 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
   if (a->G != nullptr) {
     rc = lakde_rebalance_lazy_check(N, e.c_old, e.c, a->G, unsafe, s);
     if (rc) return rc;
+    // the usual outcome, enqueued before the host knows it: the kernels
+    // skip themselves when the step is unsafe or its entries did not fit
+    rc = rebalance_lazy_guarded(N, e.out_col, e.out_val, e.nnz_dev, e.c_old, e.c, a->G, a->U,
+                                a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws,
+                                LazyGuard{e.nnz_dev, a->cap, unsafe}, s);
+    if (rc) return rc;
   }
   LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all, (size_t)(a->num_blocks + 2) * sizeof(int32_t),
                            hipMemcpyDeviceToHost, s));
-  LAKDE_HIP(hipStreamSynchronize(s));
+  LAKDE_HIP(wait_stream(s));
   if (a->host_flags[a->num_blocks] != 0) {
     set_error("lakde_partial_step: Stabilization did not converge");
     return LAKDE_ESTABILIZE;
@@ -124,7 +140,29 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
     set_error("lakde_partial_step: the block's entries exceed the arena capacity");
     return LAKDE_ECAPACITY;
   }
+  if (a->G != nullptr && a->host_flags[a->num_blocks + 1] == 0) return LAKDE_OK;  // done
   return step_rebalance(a, s);
+}
+
+int lakde_partial_sweep(lakde_step_args *steps, int64_t first, int64_t count, float *const *c,
+                        int *cur, int *lazy_dirty, int64_t *done, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(steps && c && c[0] && c[1] && cur && lazy_dirty && done, "null pointer");
+  LAKDE_CHECK_ARG(*cur == 0 || *cur == 1, "cur must be 0 or 1");
+  LAKDE_CHECK_ARG(first >= 0 && count >= 0, "bad range");
+  *done = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    lakde_step_args *a = steps + first + i;
+    LAKDE_CHECK_ARG(a->block == first + i, "steps[b] must be the arguments of block b");
+    a->estep.c_old = c[*cur];
+    a->estep.c = c[1 - *cur];
+    a->fold_first = (a->G != nullptr && a->block == 0 && *lazy_dirty) ? 1 : 0;
+    const int rc = lakde_partial_step(a, stream);
+    if (rc) return rc;
+    *cur = 1 - *cur;
+    if (a->G != nullptr) *lazy_dirty = a->host_flags[a->num_blocks + 1] == 0 ? 1 : 0;
+    *done = i + 1;
+  }
+  return LAKDE_OK;
 }
 
 int lakde_partial_step_resume(const lakde_step_args *a, lakde_stream_t stream) {

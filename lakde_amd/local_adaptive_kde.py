@@ -32,7 +32,7 @@ import torch
 from . import _fused as F
 from . import _lib
 from ._lib import (ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE, LAKDE_ECAPACITY, LAKDE_ESTABILIZE,
-                   LakdeError, check, require_gpu, stream_of)
+                   LAKDE_OK, LakdeError, check, require_gpu, stream_of)
 from .kdes import AbstractKDE
 from .utils import tensor_as
 
@@ -253,18 +253,22 @@ class LocallyAdaptiveKDE(AbstractKDE):
 
     # --------------------------------------------------------- partial step
     def _step_args(self, X, b):
-        """The step driver's arguments for block b, built once per arena
-        layout; the per-step fields are set by _partial_step."""
-        a = self._step_cache.get(b)
-        if a is not None:
-            return a
+        """The step driver's arguments of block b, an element of one array
+        over all blocks built once per arena layout; the per-step fields are
+        set by _run_steps and the driver."""
+        arena = self.arena
+        arr = self._step_cache.get("arr")
+        if arr is None:
+            arr = (_lib.StepArgs * arena.num_blocks)()
+            self._step_cache = {"arr": arr, "built": [False] * arena.num_blocks}
+        if self._step_cache["built"][b]:
+            return arr[b]
         N, D = X.shape
         s, e, nb = self._span(b, N)
-        arena = self.arena
         mode, prior, prior_jitter, prior_tril, graph = self._prior_spec(X, b, nb)
         nu_out, logdet_out, trace_out, logdet_prior_out = self._step_outputs(b, nb)
         lazy = self._lazy
-        a = _lib.StepArgs()
+        a = arr[b]
         a.post = F.posterior_args(mode, prior, prior_jitter, self._S[:nb], nu_out, self.a[s:e],
                                   self.b[s:e], self._nu0, self.W_inv_jitter[s:e], X[s:e],
                                   self._Linv[:nb], prior_tril, logdet_out, logdet_prior_out,
@@ -290,55 +294,93 @@ class LocallyAdaptiveKDE(AbstractKDE):
         else:
             a.rebal_ws = self._rebal_ws.data_ptr()
         a.host_flags = self._host_flags.data_ptr()
-        self._step_cache[b] = a
+        self._step_cache["built"][b] = True
         return a
 
-    def _partial_step(self, X, b, sparsity_threshold):
-        """Posterior of block b, partial_expectation_step and
-        rebalance_responsibilities_ (lakde/local_adaptive_kde.py:66-181) in
-        one driver call."""
-        a = self._step_args(X, b)
+    def _run_steps(self, X, first, count, sparsity_threshold):
+        """Partial steps of blocks first .. first+count-1: posterior,
+        partial_expectation_step and rebalance_responsibilities_
+        (lakde/local_adaptive_kde.py:66-181) through the native step driver,
+        with one host synchronisation per step."""
         lazy = self._lazy
-        a.estep.c_old = self.rnm_log_consts.data_ptr()
-        a.estep.c = self._c_new.data_ptr()
-        a.estep.threshold = float(sparsity_threshold or 0.0)
-        a.post.prior = self._prior_ptr(b, a.post.prior)
-        a.fold_first = int(lazy is not None and b == 0 and lazy.dirty)
-        a.bf16_frag = self._frag_blk.data_ptr() if self.precision == "bf16" else None
-        if self._estep_timer is not None:
-            e0, e1 = self._estep_timer.pair()
-            a.ev_main[0], a.ev_main[1] = e0.cuda_event, e1.cuda_event
-        else:
-            a.ev_main[0] = a.ev_main[1] = None
+        t = float(sparsity_threshold or 0.0)
+        frag = self._frag_blk.data_ptr() if self.precision == "bf16" else None
+        for b in range(first, first + count):
+            a = self._step_args(X, b)
+            a.estep.threshold = t
+            a.post.prior = self._prior_ptr(b, a.post.prior)
+            a.bf16_frag = frag
+            if self._estep_timer is not None:
+                e0, e1 = self._estep_timer.pair()
+                a.ev_main[0], a.ev_main[1] = e0.cuda_event, e1.cuda_event
+            else:
+                a.ev_main[0] = a.ev_main[1] = None
         lib = _lib.load()
         st = stream_of(X)
-        rc = lib.lakde_partial_step(ctypes.byref(a), st)
-        if rc == LAKDE_ECAPACITY:
-            self.arena.grow(int(self._flags[b]))
+        arena = self.arena
+        b, end = first, first + count
+        while b < end:
+            cbuf = (ctypes.c_void_p * 2)(self.rnm_log_consts.data_ptr(), self._c_new.data_ptr())
+            cur = ctypes.c_int(0)
+            dirty = ctypes.c_int(int(lazy is not None and lazy.dirty))
+            done = ctypes.c_int64(0)
+            rc = lib.lakde_partial_sweep(self._step_cache["arr"], b, end - b, cbuf,
+                                         ctypes.byref(cur), ctypes.byref(dirty),
+                                         ctypes.byref(done), st)
+            self._account(b, b + done.value, cur.value, dirty.value)
+            b += done.value
+            if rc == LAKDE_OK:
+                break
+            if rc == LAKDE_ESTABILIZE:
+                raise RuntimeError("Stabilization did not converge")
+            if rc != LAKDE_ECAPACITY:
+                check(rc, "partial_sweep")
+            # the block's entries exceed the arena: grow, rebuild the
+            # arguments (same per-step fields) and finish the step
+            old = self._step_cache["arr"][b]
+            keep = (old.estep.threshold, old.post.prior, old.bf16_frag, old.ev_main[0],
+                    old.ev_main[1], old.fold_first)
+            arena.grow(int(self._flags[b]))
             self._step_cache = {}
-            a2 = self._step_args(X, b)
-            for f in ("fold_first", "bf16_frag"):
-                setattr(a2, f, getattr(a, f))
-            a2.estep.c_old, a2.estep.c = a.estep.c_old, a.estep.c
-            a2.estep.threshold = a.estep.threshold
-            a2.post.prior = a.post.prior
-            rc = lib.lakde_partial_step_resume(ctypes.byref(a2), st)
-        if rc == LAKDE_ESTABILIZE:
-            raise RuntimeError("Stabilization did not converge")
-        check(rc, "partial_step")
-        nnz = int(self._flags[b])
-        self._active += nnz - self.arena.nnz[b]
-        self.arena.nnz[b] = nnz
-        if lazy is not None:
-            unsafe = int(self._flags[self.arena.num_blocks + 1]) != 0
-            lazy.dirty = not unsafe  # an unsafe step ends folded (G = 1)
-            lazy.exact_steps += int(unsafe)
-        self.rnm_log_consts, self._c_new = self._c_new, self.rnm_log_consts
+            for j in range(b, end):
+                a = self._step_args(X, j)
+                a.estep.threshold, a.post.prior, a.bf16_frag = keep[0], keep[1], keep[2]
+                if j == b:
+                    a.ev_main[0], a.ev_main[1], a.fold_first = keep[3], keep[4], keep[5]
+            a = self._step_cache["arr"][b]
+            a.estep.c_old = self.rnm_log_consts.data_ptr()
+            a.estep.c = self._c_new.data_ptr()
+            check(lib.lakde_partial_step_resume(ctypes.byref(a), st), "partial_step_resume")
+            unsafe = int(self._flags[arena.num_blocks + 1]) != 0
+            self._account(b, b + 1, 1, int(lazy is not None and not unsafe))
+            b += 1
+
+    def _account(self, first, stop, cur, dirty):
+        """Host bookkeeping after driver steps first .. stop-1: entry counts
+        (from the copied arena counts), the normaliser buffers' roles, the
+        lazy state."""
+        arena = self.arena
+        for j in range(first, stop):
+            nnz = int(self._flags[j])
+            self._active += nnz - arena.nnz[j]
+            arena.nnz[j] = nnz
+        if cur == 1:
+            self.rnm_log_consts, self._c_new = self._c_new, self.rnm_log_consts
+        if self._lazy is not None and stop > first:
+            self._lazy.dirty = bool(dirty)
+
+    def _partial_step(self, X, b, sparsity_threshold):
+        self._run_steps(X, b, 1, sparsity_threshold)
 
     def _prior_ptr(self, b, current):
         """Device address of the prior for block b's step (the shared prior
         of the hierarchical model is a new tensor every step)."""
         return current
+
+    def _sweep_fusable(self):
+        """Whether a whole sweep can run in the native loop (no host work
+        between partial steps)."""
+        return False
 
     def _init_rebalance(self):
         if self._lazy is not None:
@@ -359,7 +401,26 @@ class LocallyAdaptiveKDE(AbstractKDE):
             self._lazy.fold(self.arena)
 
     def fit(self, X, iterations=200, callbacks=None, sparsity_threshold=None, dataset_label=None):
-        super().fit(X, iterations, callbacks, sparsity_threshold, dataset_label)
+        """lakde/kdes.py:34-87.  Without callbacks, and where no host work
+        separates the partial steps (LocalKNearestKDE), each VI iteration is
+        one call of the native sweep loop."""
+        if callbacks is not None or not self._sweep_fusable():
+            super().fit(X, iterations, callbacks, sparsity_threshold, dataset_label)
+            self.materialize()
+            return self
+        assert X.is_contiguous(), "Training data must be contiguous, call .contiguous() before fitting"
+        if sparsity_threshold is None:
+            sparsity_threshold = 0
+        if self.iter_steps < 1:
+            self.init_parameters_(X, sparsity_threshold)
+            self.init_summaries_(X, sparsity_threshold)
+        num_blocks = (X.size(0) - 1) // self.block_size + 1
+        try:
+            for _ in range(iterations):
+                self._run_steps(X, 0, num_blocks, sparsity_threshold)
+                self.iter_steps += num_blocks
+        except (KeyboardInterrupt, StopIteration):
+            pass
         self.materialize()
         return self
 

@@ -93,6 +93,9 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
         # place by the posterior; none of them is read by the E-step
         self._partial_step(X, block_idx, sparsity_threshold)
 
+    def _sweep_fusable(self):
+        return True
+
     def state_dict(self):
         d = super().state_dict()
         d.update({"knn_graph": self.knn_graph, "k": self.k,

@@ -205,3 +205,41 @@ def test_lazy_rebalance_matches_exact_mid_sweep():
     a = ms["lazy"].log_pred_density(Xd, _t(Y)).cpu().numpy()
     b = ms["exact"].log_pred_density(Xd, _t(Y)).cpu().numpy()
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5)
+
+
+def test_arena_growth_inside_sweep_matches():
+    """A step whose entries outgrow the arena inside the native sweep loop
+    (the driver returns, the arena grows, the step is finished by the
+    resume entry point) gives the same fit as an arena that never grows:
+    normalisers within 1e-5 relative (float atomics in the column sums make
+    the last bit order-dependent), entry counts within 1e-3."""
+    from lakde_amd import LocalKNearestKDE
+    from lakde_amd import _fused as F
+    from lakde_amd import synthetic
+    X, Y = synthetic.make("mix32_d6", 3000, 100)
+    graph = _t(O.compute_knn_graph(X, 32))
+    Xd = _t(X)
+    t = 1 / 32
+    ref = LocalKNearestKDE(k_or_knn_graph=graph, block_size=256)
+    ref.fit(Xd, iterations=2, sparsity_threshold=t)
+    m = LocalKNearestKDE(k_or_knn_graph=graph, block_size=256)
+    m.init_parameters_(Xd, t)
+    m.init_summaries_(Xd, t)
+    # shrink the arena to the current largest block: later steps overflow
+    old = m.arena
+    cap = max(old.nnz)
+    new = F.Arena(old.num_blocks, old.block_size, cap, Xd.device)
+    for name in ("row", "col", "val"):
+        getattr(new, name).copy_(getattr(old, name)[:, :cap])
+    new.rowptr.copy_(old.rowptr)
+    new.nnz_dev.copy_(old.nnz_dev)
+    new.nnz = list(old.nnz)
+    m.arena = new
+    m._step_cache = {}
+    m.iter_steps = 1  # initialised: fit() continues instead of initialising again
+    m.fit(Xd, iterations=2, sparsity_threshold=t)
+    print(f"cap {cap} -> {m.arena.cap}")
+    assert m.arena.cap > cap
+    np.testing.assert_allclose(m.rnm_log_consts.cpu().numpy(), ref.rnm_log_consts.cpu().numpy(),
+                               rtol=1e-5, atol=1e-5)
+    assert abs(m.arena.total_nnz() - ref.arena.total_nnz()) <= 1e-3 * ref.arena.total_nnz()
