@@ -158,7 +158,20 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
   const int cofs = (int)rec_c(Dreal);
   float cm = 0.f;
 
+  // UPDATE: pass 2 tests only the pairs queued in pass 1 (the bound of the
+  // FP32 kernel: the normaliser is at least max(lb, running max))
+  __shared__ int32_t q_n[MODE == LAKDE_ESTEP_UPDATE ? kEQueue : 1][256];
+  __shared__ float q_x[MODE == LAKDE_ESTEP_UPDATE ? kEQueue : 1][256];
+  __shared__ uint64_t sm_win[4][kWave];
+  const int tid = threadIdx.x;
+  int cnt = 0;
+
   if (MODE != LAKDE_ESTEP_FIXED) {
+    float lb = neg_inf();
+    if (MODE == LAKDE_ESTEP_UPDATE && valid) {
+      const float co = p.c_old[m];
+      lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
+    }
     Lse l;
     l.init();
     stage_frag<T::FB>(frag, lds[0]);
@@ -170,23 +183,47 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
       float x = __fadd_rn(__fmul_rn(q, rc[1]), rc[0]);
       if (n == diag_n) x = neg_inf();
       l.add1(x);
+      if (MODE == LAKDE_ESTEP_UPDATE && valid && __fsub_rn(x, fmaxf(lb, l.m)) > p.log_t_lo) {
+        if (cnt < kEQueue) {
+          q_n[cnt][tid] = (int32_t)n;
+          q_x[cnt][tid] = x;
+        }
+        ++cnt;
+      }
       __syncthreads();
     }
     const float lse = l.value();
     if (valid) {
-      if (MODE == LAKDE_ESTEP_ACCUM) {
-        cm = logaddexp_f(p.c[m], lse);
-      } else {
-        const float co = p.c_old[m];
-        const float oc = logf(p.old_colsum[m]) + co;
-        cm = logaddexp_f(logsubexp_f(co, oc), lse);
-      }
+      if (MODE == LAKDE_ESTEP_ACCUM) cm = logaddexp_f(p.c[m], lse);
+      else cm = logaddexp_f(lb, lse);
       p.c[m] = cm;
     }
   } else {
     cm = valid ? p.c[m] : 0.f;
   }
   if (MODE == LAKDE_ESTEP_ACCUM) return;
+
+  // block-uniform decision (the recompute path below synchronises the block)
+  const int ovf = MODE == LAKDE_ESTEP_UPDATE ? __syncthreads_or(cnt > kEQueue) : 1;
+  if (!ovf) {
+    int ptr = 0;
+    for (int64_t wb = 0; wb < nb; wb += kWave) {
+      sm_win[w][lane] = 0;
+      __builtin_amdgcn_wave_barrier();
+      while (ptr < cnt && q_n[ptr][tid] < wb + kWave) {
+        const float d = __fsub_rn(q_x[ptr][tid], cm);
+        float r;
+        if (keep_resp(d, p.threshold, p.log_t_lo, &r))
+          atomicOr((unsigned long long *)&sm_win[w][q_n[ptr][tid] - wb], 1ull << lane);
+        ++ptr;
+      }
+      __builtin_amdgcn_wave_barrier();
+      const int64_t n = wb + lane;
+      if (n < nb && chunk < p.nchunks) p.masks[n * p.nchunks + chunk] = sm_win[w][lane];
+      __builtin_amdgcn_wave_barrier();
+    }
+    return;
+  }
 
   stage_frag<T::FB>(frag, lds[0]);
   __syncthreads();
