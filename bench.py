@@ -33,7 +33,24 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision-form", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="finish with a plain device copy of this many MiB (a known-size "
+                         "reference for hardware byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain elementwise copy kernel over ``mib`` MiB (b = -a: reads and
+    writes exactly that many bytes), run twice."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+    print(f"# copy probe: {mib} MiB read + {mib} MiB written (neg kernel)", file=sys.stderr,
+          flush=True)
 
 
 class KernelTimer:
@@ -310,6 +327,8 @@ def main():
         "precision_form": prec,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
 
 
 if __name__ == "__main__":

@@ -219,7 +219,7 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
       }
       __builtin_amdgcn_wave_barrier();
       const int64_t n = wb + lane;
-      if (n < nb && chunk < p.nchunks) p.masks[n * p.nchunks + chunk] = sm_win[w][lane];
+      if (n < nb && chunk < p.nchunks) p.masks[chunk * p.nb + n] = sm_win[w][lane];
       __builtin_amdgcn_wave_barrier();
     }
     return;
@@ -237,7 +237,7 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
     float r;
     const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
     const uint64_t mask = __ballot(keep);
-    if (lane == 0 && chunk < p.nchunks) p.masks[n * p.nchunks + chunk] = mask;
+    if (lane == 0 && chunk < p.nchunks) p.masks[chunk * p.nb + n] = mask;
     __syncthreads();
   }
 }

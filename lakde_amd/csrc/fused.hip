@@ -40,34 +40,28 @@ static int dispatch_pred_main(int64_t D, const PParams &p, hipStream_t s) {
 }
 
 // ----------------------------------------------------------- row scans
-// Exclusive scan of per-(kernel, chunk) popcounts along each kernel row.
-__global__ void __launch_bounds__(256) estep_scan_rows_kernel(EParams p) {
-  __shared__ int32_t sh[256];
-  const int64_t n = blockIdx.x;
-  const int tid = threadIdx.x;
-  const uint64_t *mrow = p.masks + n * p.nchunks;
-  const int64_t per = ceil_div(p.nchunks, 256);
-  const int64_t c0 = tid * per;
-  const int64_t c1 = (c0 + per) < p.nchunks ? (c0 + per) : p.nchunks;
-  int32_t s = 0;
-  for (int64_t c = c0; c < c1; ++c) s += __popcll(mrow[c]);
-  sh[tid] = s;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    int32_t v = (tid >= off) ? sh[tid - off] : 0;
-    __syncthreads();
-    sh[tid] += v;
-    __syncthreads();
+// Kept entries per (kernel n, chunk segment): a wave per (64 kernels,
+// segment), lane = kernel; coalesced reads of the chunk-major masks.
+__global__ void __launch_bounds__(256) estep_seg_count_kernel(EParams p) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t n = (int64_t)blockIdx.x * kWave + lane;
+  const int64_t seg = (int64_t)blockIdx.y * 4 + threadIdx.x / kWave;
+  if (n >= p.nb || seg >= kESeg) return;
+  const int64_t c0 = seg * p.seg_len;
+  const int64_t c1 = (c0 + p.seg_len) < p.nchunks ? (c0 + p.seg_len) : p.nchunks;
+  int32_t t = 0;
+  for (int64_t cb = c0; cb < c1; cb += 8) {
+    uint64_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += __popcll(v[u]);
   }
-  int32_t run = sh[tid] - s;  // exclusive
-  for (int64_t c = c0; c < c1; ++c) {
-    p.offs[n * p.nchunks + c] = run;
-    run += __popcll(mrow[c]);
-  }
-  if (tid == 255) p.totals[n] = sh[255];
+  p.seg_tot[seg * p.nb + n] = t;
 }
 
-// rowptr = exclusive scan of the row totals; nnz = rowptr[nb].
+// rowptr = exclusive scan of the row totals (sums of seg_tot over the
+// segments); nnz = rowptr[nb].  One block.
 __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   __shared__ int32_t sh[1024];
   const int tid = threadIdx.x;
@@ -75,7 +69,7 @@ __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   const int64_t r0 = tid * per;
   const int64_t r1 = (r0 + per) < p.nb ? (r0 + per) : p.nb;
   int32_t s = 0;
-  for (int64_t r = r0; r < r1; ++r) s += p.totals[r];
+  for (int64_t r = r0; r < r1; ++r) s += strided_sum(p.seg_tot + r, p.nb, 0, kESeg);
   sh[tid] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -87,7 +81,7 @@ __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   int32_t run = sh[tid] - s;
   for (int64_t r = r0; r < r1; ++r) {
     p.rowptr[r] = run;
-    run += p.totals[r];
+    if (r + 1 < r1) run += strided_sum(p.seg_tot + r, p.nb, 0, kESeg);
   }
   if (tid == 1023) {
     p.rowptr[p.nb] = sh[1023];
@@ -340,8 +334,8 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   char *ws = (char *)a.workspace;
   const int64_t cells = a.nb * p->nchunks;
   p->masks = (uint64_t *)ws;
-  p->offs = (int32_t *)(ws + align256((size_t)cells * 8));
-  p->totals = (int32_t *)(ws + align256((size_t)cells * 8) + align256((size_t)cells * 4));
+  p->seg_tot = (int32_t *)(ws + align256((size_t)cells * 8));
+  p->seg_len = ceil_div(p->nchunks, kESeg);
   p->rowptr = a.rowptr;
   p->out_row = a.out_row;
   p->out_col = a.out_col;
@@ -401,8 +395,7 @@ extern "C" {
 
 size_t lakde_estep_workspace(int64_t N, int64_t nb) {
   const int64_t nchunks = ceil_div(N, kWave);
-  return align256((size_t)nb * nchunks * 8) + align256((size_t)nb * nchunks * 4) +
-         align256((size_t)nb * 4);
+  return align256((size_t)nb * nchunks * 8) + align256((size_t)kESeg * nb * 4);
 }
 
 int lakde_estep_main(const lakde_estep_args *args, lakde_stream_t stream) {
@@ -422,7 +415,9 @@ int lakde_estep_finish(const lakde_estep_args *args, lakde_stream_t stream) {
   int rc = estep_fill_params(*args, &p);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(estep_scan_rows_kernel, dim3((unsigned)p.nb), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(estep_seg_count_kernel,
+                     dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
+                     s, p);
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(estep_rowptr_kernel, dim3(1), dim3(1024), 0, s, p);
   LAKDE_LAUNCHED();

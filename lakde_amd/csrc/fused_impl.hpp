@@ -16,9 +16,9 @@ struct EParams {
   const float *c_old;
   const float *old_colsum;
   float *c;
-  uint64_t *masks;   // nb x nchunks
-  int32_t *offs;     // nb x nchunks
-  int32_t *totals;   // nb
+  uint64_t *masks;   // nchunks x nb (chunk-major: a wave's 64 kernels are contiguous)
+  int32_t *seg_tot;  // kESeg x nb: kept entries of kernel n in chunk segment s
+  int64_t seg_len;   // chunks per segment
   int32_t *rowptr;   // nb + 1
   int32_t *out_row, *out_col;
   float *out_val;
@@ -35,6 +35,23 @@ struct PParams {
 };
 
 constexpr int kEThreads = 256;  // 4 waves; lane = column, wave = kernel quarter
+constexpr int kESeg = 128;      // chunk segments of the row scan and the compaction
+
+// sum of p[k * stride] for k in [k0, k1), 8 loads in flight
+__device__ __forceinline__ int32_t strided_sum(const int32_t *p, int64_t stride, int64_t k0,
+                                               int64_t k1) {
+  int32_t t = 0;
+  int64_t k = k0;
+  for (; k + 8 <= k1; k += 8) {
+    int32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(k + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += v[u];
+  }
+  for (; k < k1; ++k) t += p[k * stride];
+  return t;
+}
 
 // ---------------------------------------------------------------- E-step
 // One workgroup per chunk of 64 columns (points m) and all nb kernels of the
@@ -162,7 +179,7 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
       }
       __builtin_amdgcn_wave_barrier();
       const int64_t n = nbeg + wb + lane;
-      if (n < nend) p.masks[n * p.nchunks + chunk] = sm_win[w][lane];
+      if (n < nend) p.masks[chunk * p.nb + n] = sm_win[w][lane];
       __builtin_amdgcn_wave_barrier();
     }
     return;
@@ -175,41 +192,53 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
     float r;
     const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
     const uint64_t mask = __ballot(keep);
-    if (lane == 0) p.masks[n * p.nchunks + chunk] = mask;
+    if (lane == 0) p.masks[chunk * p.nb + n] = mask;
   }
 }
 
-// Compaction: one thread per (kernel, chunk) mask; the kept entries are
-// recomputed with the same device functions (identical bits) and written at
-// rowptr[n] + offs[n, chunk] in column order -> canonical CSR.
+// Compaction: a wave per (64 kernels, chunk segment), lane = kernel n; the
+// lane walks the segment's chunks in column order and writes each kept
+// entry at rowptr[n] + (kept in earlier segments) + (kept so far),
+// recomputing it with the same device functions (identical bits) ->
+// canonical CSR.
 template <int D>
 __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
   const int64_t nnz = *p.nnz_dev;
   if (nnz > p.cap) return;
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= p.nb * p.nchunks) return;
-  uint64_t mask = p.masks[idx];
-  if (mask == 0) return;
-  const int64_t n = idx / p.nchunks;
-  const int64_t chunk = idx - n * p.nchunks;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t n = (int64_t)blockIdx.x * kWave + lane;
+  const int64_t seg = (int64_t)blockIdx.y * 4 + threadIdx.x / kWave;
+  if (n >= p.nb || seg >= kESeg) return;
+  const int64_t c0 = seg * p.seg_len;
+  const int64_t c1 = (c0 + p.seg_len) < p.nchunks ? (c0 + p.seg_len) : p.nchunks;
+  if (c0 >= c1 || p.seg_tot[seg * p.nb + n] == 0) return;
   constexpr int64_t R = rec_floats(D);
   const float *rec = p.rec + n * R;
-  int64_t pos = (int64_t)p.rowptr[n] + p.offs[idx];
-  while (mask) {
-    const int b = __builtin_ctzll(mask);
-    mask &= mask - 1;
-    const int64_t m = chunk * kWave + b;
-    float xm[D];
+  int64_t pos = (int64_t)p.rowptr[n] + strided_sum(p.seg_tot + n, p.nb, 0, seg);
+  for (int64_t cb = c0; cb < c1; cb += 8) {
+    uint64_t v[8];
 #pragma unroll
-    for (int j = 0; j < D; ++j) xm[j] = p.X[m * D + j];
-    const float x = log_rho<D>(rec, xm);
-    const float d = __fsub_rn(x, p.c[m]);
-    float r = 0.f;
-    keep_resp(d, p.threshold, p.log_t_lo, &r);
-    p.out_row[pos] = (int32_t)n;
-    p.out_col[pos] = (int32_t)m;
-    p.out_val[pos] = r;
-    ++pos;
+    for (int u = 0; u < 8; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      uint64_t mask = v[u];
+      while (mask) {
+        const int b = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int64_t m = (cb + u) * kWave + b;
+        float xm[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) xm[j] = p.X[m * D + j];
+        const float x = log_rho<D>(rec, xm);
+        const float d = __fsub_rn(x, p.c[m]);
+        float r = 0.f;
+        keep_resp(d, p.threshold, p.log_t_lo, &r);
+        p.out_row[pos] = (int32_t)n;
+        p.out_col[pos] = (int32_t)m;
+        p.out_val[pos] = r;
+        ++pos;
+      }
+    }
   }
 }
 
@@ -346,9 +375,9 @@ int launch_estep_main(const EParams &p, int mode, hipStream_t s) {
 
 template <int D>
 int launch_estep_compact(const EParams &p, hipStream_t s) {
-  const int64_t total = p.nb * p.nchunks;
-  if (total == 0) return LAKDE_OK;
-  hipLaunchKernelGGL((estep_compact_kernel<D>), dim3((unsigned)ceil_div(total, 256)), dim3(256), 0,
+  if (p.nb == 0) return LAKDE_OK;
+  hipLaunchKernelGGL((estep_compact_kernel<D>),
+                     dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
                      s, p);
   return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
 }
