@@ -265,9 +265,11 @@ int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32
  *   bf16_frag != NULL: lakde_pack_frag_bf16 of estep.rec and
  *     lakde_estep_main_bf16, else lakde_estep_main (ev_main[0], ev_main[1]:
  *     optional hipEvent_t recorded around it); lakde_estep_finish;
- *   lazy: lakde_rebalance_lazy_check;
+ *   the row counts of the E-step; lazy: lakde_rebalance_lazy_check;
  *   host_flags[0 .. num_blocks+1] <- nnz_all (per-block counts, status,
- *     unsafe flag); synchronise;
+ *     unsafe flag) [record ev_flags]; the compaction and the guarded lazy
+ *     rebalance; wait for ev_flags (or the whole stream without it), so the
+ *     compaction and rebalance overlap the host's next launches;
  *   status set: LAKDE_ESTABILIZE; host_flags[block] > cap: LAKDE_ECAPACITY
  *     (the caller grows the arena, updates the pointers and calls
  *     lakde_partial_step_resume, which compacts again and rebalances);
@@ -300,6 +302,7 @@ typedef struct {
   void *bf16_frag;         /* nb * lakde_bf16_frag_bytes(D) bytes, or NULL */
   int32_t *host_flags;     /* pinned host, num_blocks + 2 */
   void *ev_main[2];
+  void *ev_flags;          /* optional hipEvent_t: the wait is for the flags copy only */
 } lakde_step_args;
 
 int lakde_partial_step(const lakde_step_args *args, lakde_stream_t stream);

@@ -387,6 +387,21 @@ int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int3
   return LAKDE_OK;
 }
 
+// the row counts and row pointer of an E-step (the first half of
+// lakde_estep_finish; *nnz_dev is final afterwards)
+int estep_count(const lakde_estep_args &a, hipStream_t s) {
+  EParams p;
+  int rc = estep_fill_params(a, &p);
+  if (rc) return rc;
+  hipLaunchKernelGGL(estep_seg_count_kernel,
+                     dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
+                     s, p);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(estep_rowptr_kernel, dim3(1), dim3(1024), 0, s, p);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
 }  // namespace lakde
 
 using namespace lakde;
@@ -411,19 +426,9 @@ int lakde_estep_main(const lakde_estep_args *args, lakde_stream_t stream) {
 int lakde_estep_finish(const lakde_estep_args *args, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(args != nullptr, "null args");
   if (args->mode == LAKDE_ESTEP_ACCUM) return LAKDE_OK;
-  EParams p;
-  int rc = estep_fill_params(*args, &p);
+  const int rc = estep_count(*args, as_stream(stream));
   if (rc) return rc;
-  hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(estep_seg_count_kernel,
-                     dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
-                     s, p);
-  LAKDE_LAUNCHED();
-  hipLaunchKernelGGL(estep_rowptr_kernel, dim3(1), dim3(1024), 0, s, p);
-  LAKDE_LAUNCHED();
-  rc = dispatch_estep_compact(args->D, p, s);
-  if (rc) set_error("lakde_estep_finish: compaction launch failed");
-  return rc;
+  return lakde_estep_compact(args, stream);
 }
 
 int lakde_estep(const lakde_estep_args *args, lakde_stream_t stream) {

@@ -406,6 +406,7 @@ struct LazyGuard {
 };
 
 // shared host helpers (fused.hip)
+int estep_count(const lakde_estep_args &a, hipStream_t s);
 int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
                            const float *c_prev, float *c, float *G, double *U, double *U_old,
                            float *old_colsum, double *ws, LazyGuard guard, hipStream_t s);

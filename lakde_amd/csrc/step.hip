@@ -30,6 +30,13 @@ hipError_t wait_stream(hipStream_t s) {
   return q;
 }
 
+hipError_t wait_event(hipEvent_t ev) {
+  hipError_t q;
+  while ((q = hipEventQuery(ev)) == hipErrorNotReady) {
+  }
+  return q;
+}
+
 int step_rebalance(const lakde_step_args *a, hipStream_t s) {
   const lakde_estep_args &e = a->estep;
   const int64_t N = e.N;
@@ -117,21 +124,29 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
                                : lakde_estep_main(&e, s);
   if (rc) return rc;
   if (a->ev_main[1]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[1], s));
-  rc = lakde_estep_finish(&e, s);
+  // counts first: the flags the host needs are final before the compaction
+  rc = estep_count(e, s);
   if (rc) return rc;
   if (a->G != nullptr) {
     rc = lakde_rebalance_lazy_check(N, e.c_old, e.c, a->G, unsafe, s);
     if (rc) return rc;
-    // the usual outcome, enqueued before the host knows it: the kernels
-    // skip themselves when the step is unsafe or its entries did not fit
+  }
+  LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all, (size_t)(a->num_blocks + 2) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, s));
+  if (a->ev_flags) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_flags, s));
+  // the compaction (skips itself when the entries do not fit) and the usual
+  // rebalance outcome run while the host reads the flags and prepares the
+  // next step; the lazy kernels skip themselves for an unsafe or
+  // overflowing step
+  rc = lakde_estep_compact(&e, s);
+  if (rc) return rc;
+  if (a->G != nullptr) {
     rc = rebalance_lazy_guarded(N, e.out_col, e.out_val, e.nnz_dev, e.c_old, e.c, a->G, a->U,
                                 a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws,
                                 LazyGuard{e.nnz_dev, a->cap, unsafe}, s);
     if (rc) return rc;
   }
-  LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all, (size_t)(a->num_blocks + 2) * sizeof(int32_t),
-                           hipMemcpyDeviceToHost, s));
-  LAKDE_HIP(wait_stream(s));
+  LAKDE_HIP(a->ev_flags ? wait_event((hipEvent_t)a->ev_flags) : wait_stream(s));
   if (a->host_flags[a->num_blocks] != 0) {
     set_error("lakde_partial_step: Stabilization did not converge");
     return LAKDE_ESTABILIZE;

@@ -122,6 +122,8 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._host_flags = torch.zeros(self.arena.num_blocks + 2, dtype=torch.int32,
                                        pin_memory=True)
         self._flags = self._host_flags.numpy()
+        self._flag_event = torch.cuda.Event()  # the driver waits on it for the flags copy
+        self._flag_event.record()
         self._step_cache = {}
         self._ews = F.EstepWorkspace(N, bs, dev)
         if self.precision == "bf16":
@@ -294,6 +296,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         else:
             a.rebal_ws = self._rebal_ws.data_ptr()
         a.host_flags = self._host_flags.data_ptr()
+        a.ev_flags = self._flag_event.cuda_event
         self._step_cache["built"][b] = True
         return a
 
