@@ -186,6 +186,12 @@ typedef struct {
   int32_t *nnz_dev;        /* device int out */
   void *workspace;
   size_t workspace_bytes;
+  /* optional (UPDATE): the unsafe-step test of lakde_rebalance_lazy_check
+   * fused into the main kernel: *lazy_unsafe |= 1 for a column whose
+   * lazy_G[m] clamp(exp(c_old - c)) leaves [2^-60, 2^60] (the caller clears
+   * *lazy_unsafe beforehand); NULL: no test */
+  const float *lazy_G;
+  int32_t *lazy_unsafe;
 } lakde_estep_args;
 
 size_t lakde_estep_workspace(int64_t N, int64_t nb);

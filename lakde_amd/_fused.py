@@ -121,7 +121,7 @@ def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=
         out = (ptr(None), ptr(None), ptr(None), ptr(None), 0, ptr(None))
     return EstepArgs(N, D, nb, row0, ptr(X), ptr(rec_blk), mode, float(threshold or 0.0),
                      ptr(c_old), ptr(old_colsum), ptr(c), out[0], out[1], out[2], out[3], out[4],
-                     out[5], ptr(ws.buf), ws.buf.numel())
+                     out[5], ptr(ws.buf), ws.buf.numel(), None, None)
 
 
 def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,

@@ -198,6 +198,8 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
       else cm = logaddexp_f(lb, lse);
       p.c[m] = cm;
     }
+    if (MODE == LAKDE_ESTEP_UPDATE && p.lazy_unsafe != nullptr)
+      lazy_unsafe_test(p, valid, m, valid ? p.c_old[m] : 0.f, cm);
   } else {
     cm = valid ? p.c[m] : 0.f;
   }

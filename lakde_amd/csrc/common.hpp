@@ -76,6 +76,19 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+// column sums of a block's old entries accumulated by the contributions
+// kernel for the lazy rebalance (all null: not accumulated)
+struct OldSums {
+  float *colsum;         // true values (stored value x column factor)
+  double *stored;        // stored values
+  int32_t *unsafe_reset; // cleared by the launch
+};
+
+int posterior_contribs_lazy(const int32_t *rowptr, const int32_t *col, const float *value,
+                            const float *colscale, const float *Xn, const float *Xm, int64_t nb,
+                            int64_t D, float nu0, float *S, float *nu, float *old_colsum,
+                            double *U_old, int32_t *unsafe, hipStream_t s);
+
 // Packed record layout (floats): [ Linv packed lower row-major: T = D(D+1)/2 ]
 // [ x_n: D ] [ estep_const, estep_expo, pred_coeff, pred_expo ], padded to 4.
 __host__ __device__ constexpr int64_t rec_tri(int64_t D) { return D * (D + 1) / 2; }

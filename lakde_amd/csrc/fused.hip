@@ -90,9 +90,6 @@ __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
 }
 
 // ------------------------------------------------------------ rebalance
-__device__ __forceinline__ float clamp_nan(float x, float lo, float hi) {
-  return (x != x) ? x : fminf(fmaxf(x, lo), hi);
-}
 
 // coeff[m] = clamp(exp(c_prev[m] - c[m]), tiny, max); sums[m] = 0
 __global__ void __launch_bounds__(256) rebal_coeff_kernel(int64_t N, const float *c_prev,
@@ -342,6 +339,13 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->out_val = a.out_val;
   p->cap = a.cap;
   p->nnz_dev = a.nnz_dev;
+  p->lazy_G = nullptr;
+  p->lazy_unsafe = nullptr;
+  if (a.mode == LAKDE_ESTEP_UPDATE && a.lazy_unsafe != nullptr) {
+    LAKDE_CHECK_ARG(a.lazy_G != nullptr, "lazy_unsafe needs lazy_G");
+    p->lazy_G = a.lazy_G;
+    p->lazy_unsafe = a.lazy_unsafe;
+  }
   return LAKDE_OK;
 }
 

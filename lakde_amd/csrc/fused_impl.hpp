@@ -3,6 +3,8 @@
 // translation units (see lakde_amd/build.py) so they compile in parallel.
 #pragma once
 
+#include <cfloat>
+
 #include "qf.hpp"
 
 namespace lakde {
@@ -24,7 +26,26 @@ struct EParams {
   float *out_val;
   int64_t cap;
   int32_t *nnz_dev;
+  const float *lazy_G;   // optional unsafe-step test of the lazy rebalance
+  int32_t *lazy_unsafe;
 };
+
+// the lazy rebalance's unsafe-step test for column m of an UPDATE step
+// (lakde_rebalance_lazy_check): G[m] clamp(exp(c_old - c)) outside
+// [2^-60, 2^60] or NaN; one atomic per wave with an unsafe column
+__device__ __forceinline__ float clamp_nan(float x, float lo, float hi) {
+  return (x != x) ? x : fminf(fmaxf(x, lo), hi);
+}
+
+__device__ __forceinline__ void lazy_unsafe_test(const EParams &p, bool valid, int64_t m,
+                                                 float co, float cm) {
+  bool bad = false;
+  if (valid) {
+    const float gc = p.lazy_G[m] * clamp_nan(expf(co - cm), FLT_MIN, FLT_MAX);
+    bad = !(gc >= 0x1p-60f && gc <= 0x1p60f);
+  }
+  if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(p.lazy_unsafe, 1);
+}
 
 struct PParams {
   int64_t N, M, per_split, nsplit;
@@ -154,6 +175,8 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
         else cm = logaddexp_f(lb, lse);
         p.c[m] = cm;
       }
+      if (MODE == LAKDE_ESTEP_UPDATE && p.lazy_unsafe != nullptr)
+        lazy_unsafe_test(p, valid, m, valid ? p.c_old[m] : 0.f, cm);
       sm_c[lane] = cm;
     }
     __syncthreads();

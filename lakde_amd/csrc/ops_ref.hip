@@ -53,7 +53,8 @@ __global__ void __launch_bounds__(kOuterThreads)
 outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
                  const T *__restrict__ value, const T *__restrict__ Xn,
                  const T *__restrict__ Xm, int64_t D, int64_t K, T *__restrict__ out,
-                 T *__restrict__ nu_out, T nu0, const T *__restrict__ colscale) {
+                 T *__restrict__ nu_out, T nu0, const T *__restrict__ colscale,
+                 OldSums old) {
   __shared__ T stage[kStageElems];
   __shared__ T wts[64];
   __shared__ T xn_s[512];
@@ -67,6 +68,7 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
   else { beg = (int64_t)rowptr[n]; end = (int64_t)rowptr[n + 1]; }
 
   for (int64_t d = tid; d < D; d += kOuterThreads) xn_s[d] = Xn[n * D + d];
+  if (old.unsafe_reset != nullptr && n == 0 && tid == 0) *old.unsafe_reset = 0;
   T nu_acc = nu0;
 
   // pair batches (a single batch for D <= 70)
@@ -108,6 +110,11 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
         T wv = GRAPH ? T(1) : value[c0 + e];
         if (!GRAPH && colscale != nullptr) wv = wv * colscale[(int64_t)col[c0 + e]];
         wts[e] = wv;
+        if (!GRAPH && old.colsum != nullptr && pbase == 0) {
+          const int64_t ci = (int64_t)col[c0 + e];
+          atomicAdd(&old.colsum[ci], (float)wv);
+          atomicAdd(&old.stored[ci], (double)value[c0 + e]);
+        }
       }
       __syncthreads();
       // nu = nu0 + sum of the row's weights in entry order, from the staged
@@ -150,11 +157,12 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
 template <typename T, typename RP, typename CI, bool GRAPH>
 static int launch_outer(const void *rowptr, const void *col, const void *value, const void *Xn,
                         const void *Xm, int64_t N, int64_t D, int64_t K, void *out, void *nu,
-                        T nu0, hipStream_t s, const T *colscale = nullptr) {
+                        T nu0, hipStream_t s, const T *colscale = nullptr,
+                        OldSums old = OldSums{nullptr, nullptr, nullptr}) {
   if (N == 0) return LAKDE_OK;
   hipLaunchKernelGGL((outer_sum_kernel<T, RP, CI, GRAPH>), dim3((unsigned)N), dim3(kOuterThreads),
                      0, s, (const RP *)rowptr, (const CI *)col, (const T *)value, (const T *)Xn,
-                     (const T *)Xm, D, K, (T *)out, (T *)nu, nu0, colscale);
+                     (const T *)Xm, D, K, (T *)out, (T *)nu, nu0, colscale, old);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }
@@ -305,6 +313,21 @@ __global__ void __launch_bounds__(256) colsum_kernel(const int32_t *__restrict__
   const int64_t nnz = *nnz_dev;
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   for (; i < nnz; i += (int64_t)gridDim.x * blockDim.x) atomicAdd(&sums[col[i]], val[i]);
+}
+
+// posterior contributions of one block (int32 CSR) fused with the start of
+// a lazy-rebalance step (lakde_rebalance_lazy_begin): the true and stored
+// column sums of the block's entries, and the unsafe flag cleared
+int posterior_contribs_lazy(const int32_t *rowptr, const int32_t *col, const float *value,
+                            const float *colscale, const float *Xn, const float *Xm, int64_t nb,
+                            int64_t D, float nu0, float *S, float *nu, float *old_colsum,
+                            double *U_old, int32_t *unsafe, hipStream_t s) {
+  LAKDE_CHECK_ARG(rowptr && col && value && colscale && Xn && Xm && S && nu && old_colsum &&
+                      U_old && unsafe && nb >= 1 && D >= 1 && D <= 512,
+                  "bad args");
+  return launch_outer<float, int32_t, int32_t, false>(rowptr, col, value, Xn, Xm, nb, D, 0, S, nu,
+                                                      nu0, s, colscale,
+                                                      OldSums{old_colsum, U_old, unsafe});
 }
 
 }  // namespace lakde

@@ -5,15 +5,18 @@
 //
 // Device work, in stream order (lakde/local_knn_kde.py:209-239,
 // lakde/local_adaptive_kde.py:66-181):
+//   [lazy, first block of a sweep: fold]        lakde_rebalance_fold
 //   kNN prior covariances of the block          lakde_knn_prior_cov (optional)
-//   posterior contributions S_n, nu_n           lakde_posterior_contribs
+//   posterior contributions S_n, nu_n, with     lakde_posterior_contribs
+//     the lazy rebalance's old column sums        (+ lakde_rebalance_lazy_begin, fused)
 //   factorisation, records, tau update          lakde_posterior
-//   old column sums (+ stored sums, lazy)       lakde_rebalance_lazy_begin / lakde_colsum
-//   E-step main (+ optional events), finish     lakde_estep_main[_bf16], lakde_estep_finish
-//   unsafe-step test (lazy)                     lakde_rebalance_lazy_check
-//   -- copy of the arena counts and flags, stream synchronise --
-//   rebalance                                   lakde_rebalance_lazy / exact fallback /
-//                                               lakde_rebalance
+//   [exact rebalance: old column sums]          lakde_colsum
+//   E-step main (+ optional events), with the   lakde_estep_main[_bf16]
+//     lazy unsafe-step test                       (+ lakde_rebalance_lazy_check, fused)
+//   row counts                                  first half of lakde_estep_finish
+//   -- copy of the arena counts and flags --
+//   compaction, guarded lazy rebalance          lakde_estep_compact, lakde_rebalance_lazy
+//   -- wait for the copy; host-side outcome: exact fallback / lakde_rebalance --
 #include "fused_impl.hpp"
 
 using namespace lakde;
@@ -92,45 +95,51 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
   int32_t *unsafe = a->nnz_all + a->num_blocks + 1;
   int rc;
 
+  if (a->G != nullptr && a->fold_first) {
+    rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all,
+                              a->G, a->U, s);
+    if (rc) return rc;
+  }
   if (a->knn_graph != nullptr) {
     rc = lakde_knn_prior_cov(e.X, N, D, e.row0, nb, a->knn_graph, a->knn_graph_dtype, a->knn_k,
                              const_cast<float *>(p.prior), s);
     if (rc) return rc;
   }
-  rc = lakde_posterior_contribs(e.rowptr, LAKDE_I32, e.out_col, LAKDE_I32, e.out_val, a->G,
-                                e.X + e.row0 * D, e.X, nb, N, D, a->nu0,
-                                const_cast<float *>(p.S), const_cast<float *>(p.nu), s);
+  if (a->G != nullptr) {
+    // contributions + the lazy rebalance's old column sums in one pass
+    rc = posterior_contribs_lazy(e.rowptr, e.out_col, e.out_val, a->G, e.X + e.row0 * D, e.X, nb,
+                                 D, a->nu0, const_cast<float *>(p.S), const_cast<float *>(p.nu),
+                                 const_cast<float *>(e.old_colsum), a->U_old, unsafe, s);
+  } else {
+    rc = lakde_posterior_contribs(e.rowptr, LAKDE_I32, e.out_col, LAKDE_I32, e.out_val, nullptr,
+                                  e.X + e.row0 * D, e.X, nb, N, D, a->nu0,
+                                  const_cast<float *>(p.S), const_cast<float *>(p.nu), s);
+  }
   if (rc) return rc;
   rc = lakde_posterior(&p, s);
   if (rc) return rc;
-  if (a->G != nullptr) {
-    if (a->fold_first) {
-      rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all,
-                                a->G, a->U, s);
-      if (rc) return rc;
-    }
-    rc = lakde_rebalance_lazy_begin(N, e.out_col, e.out_val, e.nnz_dev, a->G,
-                                    const_cast<float *>(e.old_colsum), a->U_old, unsafe, s);
-  } else {
+  if (a->G == nullptr) {
     rc = lakde_colsum(e.out_col, e.out_val, e.nnz_dev, N, const_cast<float *>(e.old_colsum), s);
+    if (rc) return rc;
   }
-  if (rc) return rc;
   if (a->bf16_frag != nullptr) {
     rc = lakde_pack_frag_bf16(e.rec, nb, D, a->bf16_frag, s);
     if (rc) return rc;
   }
+  // the E-step main kernel also runs the lazy rebalance's unsafe-step test
+  lakde_estep_args em = e;
+  if (a->G != nullptr) {
+    em.lazy_G = a->G;
+    em.lazy_unsafe = unsafe;
+  }
   if (a->ev_main[0]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[0], s));
-  rc = a->bf16_frag != nullptr ? lakde_estep_main_bf16(&e, a->bf16_frag, s)
-                               : lakde_estep_main(&e, s);
+  rc = a->bf16_frag != nullptr ? lakde_estep_main_bf16(&em, a->bf16_frag, s)
+                               : lakde_estep_main(&em, s);
   if (rc) return rc;
   if (a->ev_main[1]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[1], s));
   // counts first: the flags the host needs are final before the compaction
   rc = estep_count(e, s);
   if (rc) return rc;
-  if (a->G != nullptr) {
-    rc = lakde_rebalance_lazy_check(N, e.c_old, e.c, a->G, unsafe, s);
-    if (rc) return rc;
-  }
   LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all, (size_t)(a->num_blocks + 2) * sizeof(int32_t),
                            hipMemcpyDeviceToHost, s));
   if (a->ev_flags) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_flags, s));
