@@ -21,6 +21,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of estep_main_kernel<6,UPDATE> on mix32_d6 with block
+# size 512: rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs, KiB per
+# dispatch) scaled by what they read for a plain 512 MiB elementwise copy
+# (factors 2.0000 and 1.0000); summary in profiles/estep_mix32_d6_hbm_traffic.md
+MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 6767002, "write": 13587763}}
+
 
 def _parse():
     ap = argparse.ArgumentParser()
@@ -283,9 +289,12 @@ def main():
     pairs_per_launch = (N * N) / nblocks
     flop = pairs_per_launch * D * (D + 1)
     achieved = flop / (k_ms * 1e-3) / 1e12
+    tr = MEASURED_TRAFFIC.get((args.workload, bs))
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": None, "kernel": f"estep_main_kernel<{D},UPDATE>",
+                "traffic": (tr["fetch"] + tr["write"]) if tr else None,
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)",
+                "kernel": f"estep_main_kernel<{D},UPDATE>",
                 "kernel_ms": round(k_ms, 5), "pairs_per_launch": int(pairs_per_launch)}
 
     prec = None
