@@ -24,8 +24,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of estep_main_kernel<6,UPDATE> on mix32_d6 with block
 # size 512: rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs, KiB per
 # dispatch) scaled by what they read for a plain 512 MiB elementwise copy
-# (factors 2.0000 and 1.0000); summary in profiles/estep_mix32_d6_hbm_traffic.md
-MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 6767002, "write": 13587763}}
+# (factors 1.9999 and 1.0000); summary in profiles/estep_mix32_d6_hbm_traffic.md
+MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 7367287, "write": 13587724}}
 
 
 def _parse():
