@@ -61,13 +61,17 @@ def copy_probe(mib):
 
 class KernelTimer:
     """HIP-event timing of the E-step main kernel on the stream it runs on:
-    the step driver records a pair of events around its launch.  Events are
-    created (and recorded once, which creates the HIP event) before the
-    timed region by ``reserve``."""
+    the step driver records a pair of events around its launch.  Every
+    ``every``-th launch is timed (an event pair on the stream adds a few
+    microseconds of launch gap, so timing every launch would slow the timed
+    region itself).  Events are created (and recorded once, which creates the
+    HIP event) before the timed region by ``reserve``."""
 
-    def __init__(self):
+    def __init__(self, every=1):
         import torch
         self.torch = torch
+        self.every = every
+        self.calls = 0
         self.pool = []
         self.pairs = []
 
@@ -77,11 +81,14 @@ class KernelTimer:
         return ev
 
     def reserve(self, n):
-        self.pool = [(self._new(), self._new()) for _ in range(n)]
+        self.pool = [(self._new(), self._new()) for _ in range((n - 1) // self.every + 1)]
         self.pool.reverse()
         self.torch.cuda.synchronize()
 
     def pair(self):
+        self.calls += 1
+        if (self.calls - 1) % self.every:
+            return None
         p = self.pool.pop() if self.pool else (self._new(), self._new())
         self.pairs.append(p)
         return p
@@ -258,7 +265,7 @@ def main():
 
     # timed: fit() of the reference API, K VI iterations (one native sweep
     # loop each), the responsibilities folded at the end
-    timer = KernelTimer()
+    timer = KernelTimer(every=8)
     timer.reserve(args.steps * nblocks)
     model._estep_timer = timer
     torch.cuda.synchronize()

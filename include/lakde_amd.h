@@ -156,6 +156,20 @@ typedef struct {
 
 int lakde_posterior(const lakde_posterior_args *args, lakde_stream_t stream);
 
+/* One block's posterior from one call: for a kNN prior (knn_graph = the
+ * block's nb x knn_k graph rows, int32/int64) lakde_knn_prior_cov into
+ * post->prior, then lakde_posterior_contribs of the block's CSR (rowptr nb+1,
+ * col, val; int32 indices; optional colscale) into post->S, post->nu with
+ * nu0 = post->nu0, then lakde_posterior(post).  A shared prior passes
+ * knn_graph = NULL.  For D <= 6 one fused kernel does all three (a wavefront
+ * per kernel point; post->prior and post->S are then not written).
+ * (lakde/local_knn_kde.py:195-207 calc_prior_cov,
+ * lakde/local_adaptive_kde.py:66-96 calc_posterior_contribs + calc_posterior) */
+int lakde_block_posterior(const lakde_posterior_args *post, const float *X, int64_t N,
+                          int64_t row0, const int32_t *rowptr, const int32_t *col,
+                          const float *val, const float *colscale, const void *knn_graph,
+                          int knn_graph_dtype, int64_t knn_k, lakde_stream_t stream);
+
 /* Column sums of one block's responsibilities: sums[m] = sum_{e: col[e]=m} val[e]
  * (sums is cleared first).  (lakde/local_adaptive_kde.py:120) */
 int lakde_colsum(const int32_t *col, const float *val, const int32_t *nnz_dev, int64_t N,

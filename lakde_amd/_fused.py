@@ -99,6 +99,18 @@ def posterior(*args, **kw):
     check(_lib.load().lakde_posterior(a, stream_of(args[9])), "posterior")
 
 
+def block_posterior(X, row0, rowptr, col, val, colscale, graph_blk, post_args):
+    """Prior (kNN graph rows, or None for a shared prior), contributions and
+    factorisation of one block from one call (lakde_block_posterior)."""
+    N = X.size(0)
+    gk = graph_blk.size(1) if graph_blk is not None else 0
+    gdt = idx_code(graph_blk.dtype) if graph_blk is not None else 0
+    check(_lib.load().lakde_block_posterior(post_args, ptr(X), N, row0, ptr(rowptr), ptr(col),
+                                            ptr(val), ptr(colscale), ptr(graph_blk), gdt, gk,
+                                            stream_of(X)),
+          "block_posterior")
+
+
 def colsum(col, val, nnz_dev, N, out):
     check(_lib.load().lakde_colsum(ptr(col), ptr(val), ptr(nnz_dev), N, ptr(out), stream_of(val)),
           "colsum")

@@ -71,6 +71,8 @@ _SIGNATURES = {
     "lakde_posterior_contribs": (_int, [_vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _i64, _i64,
                                         _i64, _f32, _vp, _vp, _vp]),
     "lakde_posterior": (_int, [ctypes.POINTER(PosteriorArgs), _vp]),
+    "lakde_block_posterior": (_int, [ctypes.POINTER(PosteriorArgs), _vp, _i64, _i64, _vp, _vp, _vp,
+                                     _vp, _vp, _int, _i64, _vp]),
     "lakde_colsum": (_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
     "lakde_estep_workspace": (_sz, [_i64, _i64]),
     "lakde_estep": (_int, [ctypes.POINTER(EstepArgs), _vp]),

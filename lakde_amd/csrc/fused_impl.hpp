@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cfloat>
+#include <type_traits>
 
 #include "qf.hpp"
 
@@ -123,8 +124,10 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
       const float co = p.c_old[m];
       lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
     }
+    // queue a pair whose exact log rho x can pass the keep test (B is a
+    // lower bound of the final normaliser)
     auto cand = [&](float x, int64_t n, float B) {
-      if (MODE == LAKDE_ESTEP_UPDATE && valid && __fsub_rn(x, B) > p.log_t_lo) {
+      if (__fsub_rn(x, B) > p.log_t_lo) {
         if (cnt < kEQueue) {
           q_n[cnt][tid] = (int32_t)(n - nbeg);
           q_x[cnt][tid] = x;
@@ -132,35 +135,56 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
         ++cnt;
       }
     };
-    Lse l;
+    LseE l;
     l.init();
-    int64_t n = nbeg;
-    if (D <= 24) {
-      for (; n + 4 <= nend; n += 4) {
-        float x0 = log_rho<D>(rec + (n + 0) * R, xm);
-        float x1 = log_rho<D>(rec + (n + 1) * R, xm);
-        float x2 = log_rho<D>(rec + (n + 2) * R, xm);
-        float x3 = log_rho<D>(rec + (n + 3) * R, xm);
-        if (n + 0 == diag_n) x0 = neg_inf();
-        if (n + 1 == diag_n) x1 = neg_inf();
-        if (n + 2 == diag_n) x2 = neg_inf();
-        if (n + 3 == diag_n) x3 = neg_inf();
-        l.add4(x0, x1, x2, x3);
-        if (MODE == LAKDE_ESTEP_UPDATE) {
-          const float B = fmaxf(lb, l.m);
-          cand(x0, n + 0, B);
-          cand(x1, n + 1, B);
-          cand(x2, n + 2, B);
-          cand(x3, n + 3, B);
+    float B = lb;  // max(lb, running maximum): a lower bound of the final normaliser
+    // only the waves whose columns overlap the block's own rows hold the
+    // excluded diagonal pair (n == m - row0)
+    const int64_t col0 = chunk * kWave - p.row0;
+    const bool has_diag = col0 < nend && col0 + kWave > nbeg;
+    auto scan = [&](auto diag_tag) {
+      constexpr bool DIAG = decltype(diag_tag)::value;
+      int64_t n = nbeg;
+      if (D <= 24) {
+        for (; n + 4 <= nend; n += 4) {
+          const float *r0 = rec + (n + 0) * R, *r1 = rec + (n + 1) * R;
+          const float *r2 = rec + (n + 2) * R, *r3 = rec + (n + 3) * R;
+          const float q0 = qf_rec<D>(r0, xm), q1 = qf_rec<D>(r1, xm);
+          const float q2 = qf_rec<D>(r2, xm), q3 = qf_rec<D>(r3, xm);
+          float x0 = log_rho_q<D>(r0, q0), x1 = log_rho_q<D>(r1, q1);
+          float x2 = log_rho_q<D>(r2, q2), x3 = log_rho_q<D>(r3, q3);
+          if (DIAG) {
+            if (n + 0 == diag_n) x0 = neg_inf();
+            if (n + 1 == diag_n) x1 = neg_inf();
+            if (n + 2 == diag_n) x2 = neg_inf();
+            if (n + 3 == diag_n) x3 = neg_inf();
+          }
+          const float g = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+          if (l.fold(g)) B = fmaxf(lb, g);
+          l.s += (l.term(x0) + l.term(x1)) + (l.term(x2) + l.term(x3));
+          if (MODE == LAKDE_ESTEP_UPDATE) {
+            // one pre-test on the group's largest value (fl(x - B) is
+            // monotone in x), then the group's pairs
+            if (valid && __fsub_rn(g, B) > p.log_t_lo) {
+              cand(x0, n + 0, B);
+              cand(x1, n + 1, B);
+              cand(x2, n + 2, B);
+              cand(x3, n + 3, B);
+            }
+          }
         }
       }
-    }
-    for (; n < nend; ++n) {
-      float x = log_rho<D>(rec + n * R, xm);
-      if (n == diag_n) x = neg_inf();
-      l.add1(x);
-      if (MODE == LAKDE_ESTEP_UPDATE) cand(x, n, fmaxf(lb, l.m));
-    }
+      for (; n < nend; ++n) {
+        const float *r = rec + n * R;
+        float x = log_rho<D>(r, xm);
+        if (DIAG && n == diag_n) x = neg_inf();
+        if (l.fold(x)) B = fmaxf(lb, x);
+        l.s += l.term(x);
+        if (MODE == LAKDE_ESTEP_UPDATE && valid) cand(x, n, B);
+      }
+    };
+    if (has_diag) scan(std::true_type{});
+    else scan(std::false_type{});
     sm_m[w][lane] = l.m;
     sm_s[w][lane] = l.s;
     __syncthreads();

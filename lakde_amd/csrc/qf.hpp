@@ -38,6 +38,12 @@ __device__ __forceinline__ float log_rho(const float *__restrict__ rec, const fl
   return __fadd_rn(__fmul_rn(q, rec[rec_c(D) + 1]), rec[rec_c(D) + 0]);
 }
 
+// log rho from a quadratic form already computed (the same two roundings)
+template <int D>
+__device__ __forceinline__ float log_rho_q(const float *__restrict__ rec, float q) {
+  return __fadd_rn(__fmul_rn(q, rec[rec_c(D) + 1]), rec[rec_c(D) + 0]);
+}
+
 // log p_n(y_m) = coeff - (nu+1)/2 * log1p(Q)
 template <int D>
 __device__ __forceinline__ float log_pred(const float *__restrict__ rec, const float (&ym)[D]) {
@@ -81,6 +87,28 @@ struct Lse {
     else s += s2 * __expf(m2 - m);
   }
   __device__ __forceinline__ float value() const { return m == neg_inf() ? m : m + logf(s); }
+};
+
+// Streaming log-sum-exp of the E-step's hot loop, with the arithmetic of
+// Lse::add4: a new maximum is rare, so its rescale is a real branch, and
+// the terms subtract a copy mz of the maximum that starts at 0 instead of
+// -inf, so that a -inf term (the excluded diagonal) gives exactly 0 without
+// a test.
+struct LseE {
+  float m, mz, s;
+  __device__ __forceinline__ void init() { m = neg_inf(); mz = 0.f; s = 0.f; }
+  // returns whether the maximum moved
+  __device__ __forceinline__ bool fold(float g) {
+    if (g > m) {
+      asm volatile("");  // keep the rescale a branch instead of speculating it
+      s = s * __expf(m - g);
+      m = g;
+      mz = g;
+      return true;
+    }
+    return false;
+  }
+  __device__ __forceinline__ float term(float x) const { return __expf(x - mz); }
 };
 
 }  // namespace lakde

@@ -6,10 +6,12 @@
 // Device work, in stream order (lakde/local_knn_kde.py:209-239,
 // lakde/local_adaptive_kde.py:66-181):
 //   [lazy, first block of a sweep: fold]        lakde_rebalance_fold
-//   kNN prior covariances of the block          lakde_knn_prior_cov (optional)
-//   posterior contributions S_n, nu_n, with     lakde_posterior_contribs
-//     the lazy rebalance's old column sums        (+ lakde_rebalance_lazy_begin, fused)
-//   factorisation, records, tau update          lakde_posterior
+//   kNN prior covariances of the block,         lakde_block_posterior (one fused
+//   posterior contributions S_n, nu_n, with       kernel for D <= 6; else
+//     the lazy rebalance's old column sums,       lakde_knn_prior_cov,
+//   factorisation, records, tau update            lakde_posterior_contribs with
+//                                                 lakde_rebalance_lazy_begin fused,
+//                                                 lakde_posterior)
 //   [exact rebalance: old column sums]          lakde_colsum
 //   E-step main (+ optional events), with the   lakde_estep_main[_bf16]
 //     lazy unsafe-step test                       (+ lakde_rebalance_lazy_check, fused)
@@ -100,23 +102,13 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
                               a->G, a->U, s);
     if (rc) return rc;
   }
-  if (a->knn_graph != nullptr) {
-    rc = lakde_knn_prior_cov(e.X, N, D, e.row0, nb, a->knn_graph, a->knn_graph_dtype, a->knn_k,
-                             const_cast<float *>(p.prior), s);
-    if (rc) return rc;
-  }
-  if (a->G != nullptr) {
-    // contributions + the lazy rebalance's old column sums in one pass
-    rc = posterior_contribs_lazy(e.rowptr, e.out_col, e.out_val, a->G, e.X + e.row0 * D, e.X, nb,
-                                 D, a->nu0, const_cast<float *>(p.S), const_cast<float *>(p.nu),
-                                 const_cast<float *>(e.old_colsum), a->U_old, unsafe, s);
-  } else {
-    rc = lakde_posterior_contribs(e.rowptr, LAKDE_I32, e.out_col, LAKDE_I32, e.out_val, nullptr,
-                                  e.X + e.row0 * D, e.X, nb, N, D, a->nu0,
-                                  const_cast<float *>(p.S), const_cast<float *>(p.nu), s);
-  }
-  if (rc) return rc;
-  rc = lakde_posterior(&p, s);
+  // prior, contributions (+ the lazy rebalance's old column sums) and
+  // factorisation
+  const bool lazy = a->G != nullptr;
+  rc = block_posterior(p, e.X, N, e.row0, e.rowptr, e.out_col, e.out_val, a->G, a->knn_graph,
+                       a->knn_graph_dtype, a->knn_k, a->nu0,
+                       lazy ? const_cast<float *>(e.old_colsum) : nullptr,
+                       lazy ? a->U_old : nullptr, lazy ? unsafe : nullptr, s);
   if (rc) return rc;
   if (a->G == nullptr) {
     rc = lakde_colsum(e.out_col, e.out_val, e.nnz_dev, N, const_cast<float *>(e.old_colsum), s);

@@ -66,7 +66,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._active = None         # rnm_active_contribs, kept on the host
         self._active_device = None
         self._step_cache = {}
-        self._estep_timer = None  # optional: .pair() -> two recorded torch events
+        self._estep_timer = None  # optional: .pair() -> two recorded torch events or None
 
     # ------------------------------------------------------------ helpers
     def _blocks(self, N):
@@ -198,16 +198,14 @@ class LocallyAdaptiveKDE(AbstractKDE):
         N, D = X.shape
         s, e, nb = self._span(b, N)
         arena = self.arena
-        S = self._S[:nb]
-        F.posterior_contribs(arena.rowptr[b, :nb + 1], arena.col[b], arena.val[b], X[s:e], X,
-                             self._nu0, S, nu_out, colscale=self._colscale())
         mode, prior, prior_jitter, prior_tril, graph = self._prior_spec(X, b, nb)
-        if graph is not None:
-            F.knn_prior_cov(X, s, graph, prior)
-        F.posterior(mode, prior, prior_jitter, S, nu_out, self.a[s:e], self.b[s:e], self._nu0,
-                    self.W_inv_jitter[s:e], X[s:e], self._Linv[:nb], prior_tril, logdet_out,
-                    logdet_prior_out, trace_out, self.rec[s:e], arena.status_ptr,
-                    b0=self._b0, b_out=self.b[s:e] if update_b else None)
+        post = F.posterior_args(mode, prior, prior_jitter, self._S[:nb], nu_out, self.a[s:e],
+                                self.b[s:e], self._nu0, self.W_inv_jitter[s:e], X[s:e],
+                                self._Linv[:nb], prior_tril, logdet_out, logdet_prior_out,
+                                trace_out, self.rec[s:e], arena.status_ptr, b0=self._b0,
+                                b_out=self.b[s:e] if update_b else None)
+        F.block_posterior(X, s, arena.rowptr[b, :nb + 1], arena.col[b], arena.val[b],
+                          self._colscale(), graph, post)
 
     def _sync(self, b):
         flags = self.arena.nnz_dev.cpu()
@@ -313,9 +311,9 @@ class LocallyAdaptiveKDE(AbstractKDE):
             a.estep.threshold = t
             a.post.prior = self._prior_ptr(b, a.post.prior)
             a.bf16_frag = frag
-            if self._estep_timer is not None:
-                e0, e1 = self._estep_timer.pair()
-                a.ev_main[0], a.ev_main[1] = e0.cuda_event, e1.cuda_event
+            ev = self._estep_timer.pair() if self._estep_timer is not None else None
+            if ev is not None:
+                a.ev_main[0], a.ev_main[1] = ev[0].cuda_event, ev[1].cuda_event
             else:
                 a.ev_main[0] = a.ev_main[1] = None
         lib = _lib.load()

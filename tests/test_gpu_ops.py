@@ -316,3 +316,135 @@ def test_posterior_jitter_schedule_exact():
     np.testing.assert_array_equal(out["jit_p"], 0)
     np.testing.assert_array_equal(out["jit_w"], np.array([eps, 7 * eps, (2 ** 14 - 1) * eps],
                                                          np.float32))
+
+
+# ----------------------------------------- one-call block posterior (fused D <= 6)
+def _block_inputs(rng, N, D, nb, K, quantise):
+    X = rng.standard_normal((N, D)).astype(np.float32)
+    if quantise:  # repeated coordinates: rank-deficient kNN covariances, jittered
+        X[:, 0] = np.round(X[:, 0] * 2) / 2
+        X[:, -1] = 0.25
+    graph = rng.integers(0, N, size=(nb, K)).astype(np.int32)
+    if quantise:
+        graph[:5] = graph[:5, :1]  # every neighbour the same point: zero covariance
+    # ragged CSR rows: empty, short, and longer than one wavefront
+    lens = rng.integers(0, 90, size=nb)
+    lens[0], lens[1], lens[2] = 0, 1, 200
+    rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(N, size=l, replace=False)) for l in lens])
+    val = rng.random(col.size).astype(np.float32) * 0.5
+    colscale = (0.5 + rng.random(N)).astype(np.float32)
+    return X, graph, rowptr, col.astype(np.int32), val, colscale
+
+
+def _block_posterior_both(X, graph, rowptr, col, val, colscale, row0, shared_prior=None):
+    """lakde_block_posterior and the three separate entry points on the same
+    inputs; returns both output dicts."""
+    from lakde_amd import _fused as F
+    from lakde_amd._lib import PRIOR_KNN, PRIOR_SHARED
+    N, D = X.shape
+    nb = rowptr.size - 1
+    rng = np.random.default_rng(5)
+    a = np.full(nb, 2.0, np.float32)
+    b = (1 + rng.random(nb)).astype(np.float32)
+    nu0 = np.float32(D)
+    # the argument structs hold raw pointers: keep every input tensor alive
+    Xd, a_t, b_t = _t(X), _t(a), _t(b)
+    rp_t, col_t, val_t = _t(rowptr), _t(col), _t(val)
+    cs = _t(colscale) if colscale is not None else None
+    g_t = _t(graph) if graph is not None else None
+    sp_t = _t(shared_prior) if shared_prior is not None else None
+    outs = []
+    for fused in (True, False):
+        o = dict(prior=torch.empty(nb, D, D, device=DEV), S=torch.empty(nb, D, D, device=DEV),
+                 nu=torch.empty(nb, device=DEV), b_out=torch.empty(nb, device=DEV),
+                 jw=torch.zeros(nb, device=DEV), jp=torch.zeros(nb, device=DEV),
+                 Linv=torch.empty(nb, D, D, device=DEV), ptril=torch.empty(nb, D, D, device=DEV),
+                 logdet=torch.empty(nb, device=DEV), ldp=torch.empty(nb, device=DEV),
+                 trace=torch.empty(nb, device=DEV),
+                 rec=torch.zeros(nb, F.record_floats(D), device=DEV),
+                 status=torch.zeros(1, dtype=torch.int32, device=DEV))
+        if shared_prior is None:
+            mode, prior, jp, ptril, ldp, g = (PRIOR_KNN, o["prior"], o["jp"], o["ptril"],
+                                             o["ldp"], g_t)
+        else:
+            mode, prior, jp, ptril, ldp, g = PRIOR_SHARED, sp_t, None, None, None, None
+        post = F.posterior_args(mode, prior, jp, o["S"], o["nu"], a_t, b_t, nu0, o["jw"],
+                                Xd[row0:row0 + nb], o["Linv"], ptril, o["logdet"], ldp,
+                                o["trace"], o["rec"], o["status"], b0=1e-6, b_out=o["b_out"])
+        if fused:
+            F.block_posterior(Xd, row0, rp_t, col_t, val_t, cs, g, post)
+        else:
+            if g is not None:
+                F.knn_prior_cov(Xd, row0, g, prior)
+            F.posterior_contribs(rp_t, col_t, val_t, Xd[row0:row0 + nb], Xd, nu0,
+                                 o["S"], o["nu"], colscale=cs)
+            from lakde_amd import _lib
+            _lib.check(_lib.load().lakde_posterior(post, _lib.stream_of(Xd)), "posterior")
+        torch.cuda.synchronize()
+        outs.append({k: _np(v) for k, v in o.items()})
+    return outs
+
+
+@pytest.mark.parametrize("D,quantise,scaled", [(1, False, False), (2, False, True),
+                                               (5, True, True), (6, False, True),
+                                               (6, True, False), (7, False, True)])
+def test_block_posterior_matches_separate_kernels(D, quantise, scaled):
+    """lakde_block_posterior (one fused kernel for D <= 6; the separate
+    kernels for D = 7) against lakde_knn_prior_cov + lakde_posterior_contribs
+    + lakde_posterior on ragged CSR rows (empty, 1 entry, 200 entries),
+    jittered priors and per-column factors: nu bit-exact (entry order),
+    factors and constants to 2e-5 relative (the sums are reduced in a
+    different order).  Jitter decisions are equal except where a rank-deficient
+    prior sits on the FP32 decision boundary (rounding of the differently
+    ordered sums decides); such rows must be few and are checked to carry a
+    jitter of the reference schedule."""
+    rng = np.random.default_rng(100 + D)
+    X, graph, rowptr, col, val, colscale = _block_inputs(rng, 3000, D, 150, 32, quantise)
+    fu, ref = _block_posterior_both(X, graph, rowptr, col, val, colscale if scaled else None,
+                                    row0=700)
+    assert fu["status"][0] == 0 and ref["status"][0] == 0
+    np.testing.assert_array_equal(fu["nu"], ref["nu"])
+    if quantise:
+        assert np.any(ref["jp"] > 0)
+    same = (fu["jp"] == ref["jp"]) & (fu["jw"] == ref["jw"])
+    assert same.mean() >= 0.9, same.mean()
+    allowed = _allowed_jitters()
+    assert all(float(v) in allowed for v in np.concatenate([fu["jp"], fu["jw"]]))
+    if not quantise:
+        bad = np.flatnonzero(~same)
+        assert same.all(), (bad, fu["jp"][bad], ref["jp"][bad], fu["jw"][bad], ref["jw"][bad],
+                            fu["nu"][bad], np.diff(rowptr)[bad])
+    for k in ("Linv", "rec"):
+        f, r = fu[k][same], ref[k][same]
+        scale = np.abs(r).max(axis=tuple(range(1, r.ndim)), keepdims=True) + 1e-30
+        np.testing.assert_allclose(f / scale, r / scale, rtol=0, atol=2e-5, err_msg=k)
+    for k in ("logdet", "trace", "b_out"):
+        np.testing.assert_allclose(fu[k][same], ref[k][same], rtol=2e-5, atol=2e-5, err_msg=k)
+    # prior factors: a jittered rank-deficient prior has pivots of order
+    # sqrt(jitter) that the summation order's rounding decides, so its factor
+    # is compared through L L^T (the jittered matrix both paths factor) and
+    # its log-determinant only where no jitter was needed
+    f, r = fu["ptril"][same].astype(np.float64), ref["ptril"][same].astype(np.float64)
+    ff, rr = f @ np.swapaxes(f, 1, 2), r @ np.swapaxes(r, 1, 2)
+    scale = np.abs(rr).max(axis=(1, 2), keepdims=True) + 1e-30
+    np.testing.assert_allclose(ff / scale, rr / scale, rtol=0, atol=2e-5, err_msg="ptril")
+    full = same & (ref["jp"] == 0)
+    np.testing.assert_allclose(fu["ldp"][full], ref["ldp"][full], rtol=2e-5, atol=2e-5)
+
+
+def test_block_posterior_shared_prior():
+    """Shared-prior mode of lakde_block_posterior (hierarchical model) against
+    the separate kernels, D = 4 (fused) and D = 9 (separate)."""
+    for D in (4, 9):
+        rng = np.random.default_rng(300 + D)
+        X, graph, rowptr, col, val, _ = _block_inputs(rng, 2000, D, 100, 16, False)
+        C = rng.standard_normal((D, D)).astype(np.float32)
+        prior = (C @ C.T * 0.1 + np.eye(D, dtype=np.float32)).astype(np.float32)
+        fu, ref = _block_posterior_both(X, None, rowptr, col, val, None, row0=0,
+                                        shared_prior=prior)
+        np.testing.assert_array_equal(fu["nu"], ref["nu"])
+        np.testing.assert_array_equal(fu["jw"], ref["jw"])
+        for k in ("logdet", "trace", "b_out"):
+            np.testing.assert_allclose(fu[k], ref[k], rtol=2e-5, atol=2e-5, err_msg=k)
+        np.testing.assert_allclose(fu["rec"], ref["rec"], rtol=2e-5, atol=2e-5)
