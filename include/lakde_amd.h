@@ -183,7 +183,11 @@ int lakde_colsum(const int32_t *col, const float *val, const int32_t *nnz_dev, i
  * column ascending).  Responsibilities r > threshold are kept (threshold 0:
  * r != 0).  Output arrays hold `cap` entries; *nnz_dev receives the true
  * count and nothing is written past cap (the caller grows and re-runs
- * lakde_estep_compact).  Workspace: lakde_estep_workspace() bytes. */
+ * lakde_estep_compact).  Workspace: lakde_estep_workspace() bytes, zero-filled
+ * before its first use; it holds counters that every ESTEP_FIXED/UPDATE
+ * launch of lakde_estep_main accumulates and lakde_estep_finish clears, so a
+ * main launch in those modes must be followed by lakde_estep_finish on the
+ * same workspace. */
 typedef struct {
   int64_t N, D, nb, row0;
   const float *X;          /* N x D */

@@ -119,7 +119,8 @@ def colsum(col, val, nnz_dev, N, out):
 class EstepWorkspace:
     def __init__(self, N, nb, device):
         nbytes = int(_lib.load().lakde_estep_workspace(N, nb))
-        self.buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        # zero-filled: the E-step's per-segment counters start (and end) at zero
+        self.buf = torch.zeros(nbytes, dtype=torch.uint8, device=device)
 
 
 def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,

@@ -204,6 +204,7 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
     cm = valid ? p.c[m] : 0.f;
   }
   if (MODE == LAKDE_ESTEP_ACCUM) return;
+  const int64_t cseg = chunk / p.seg_len;
 
   // block-uniform decision (the recompute path below synchronises the block)
   const int ovf = MODE == LAKDE_ESTEP_UPDATE ? __syncthreads_or(cnt > kEQueue) : 1;
@@ -221,7 +222,11 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
       }
       __builtin_amdgcn_wave_barrier();
       const int64_t n = wb + lane;
-      if (n < nb && chunk < p.nchunks) p.masks[chunk * p.nb + n] = sm_win[w][lane];
+      if (n < nb && chunk < p.nchunks) {
+        const uint64_t mk = sm_win[w][lane];
+        p.masks[chunk * p.nb + n] = mk;
+        count_mask(p, cseg, n, mk);
+      }
       __builtin_amdgcn_wave_barrier();
     }
     return;
@@ -239,7 +244,10 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
     float r;
     const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
     const uint64_t mask = __ballot(keep);
-    if (lane == 0 && chunk < p.nchunks) p.masks[chunk * p.nb + n] = mask;
+    if (lane == 0 && chunk < p.nchunks) {
+      p.masks[chunk * p.nb + n] = mask;
+      count_mask(p, cseg, n, mask);
+    }
     __syncthreads();
   }
 }

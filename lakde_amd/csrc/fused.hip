@@ -40,28 +40,46 @@ static int dispatch_pred_main(int64_t D, const PParams &p, hipStream_t s) {
 }
 
 // ----------------------------------------------------------- row scans
-// Kept entries per (kernel n, chunk segment): a wave per (64 kernels,
-// segment), lane = kernel; coalesced reads of the chunk-major masks.
-__global__ void __launch_bounds__(256) estep_seg_count_kernel(EParams p) {
-  const int lane = threadIdx.x & (kWave - 1);
+// The main kernel adds every written mask's kept entries to its (segment,
+// kernel) count.  estep_seg_end_kernel turns the counts into inclusive
+// per-segment ends (the compaction's write offsets within a row) and row
+// totals and clears the counts for the next E-step (blocks of 16 waves x 64
+// kernels, a wave per 8 segments); estep_rowptr_kernel scans the row totals
+// into the row pointer and the entry count (one block).  Two launches
+// instead of a last-block scan: a grid-wide handoff would need an
+// agent-scope fence, which writes back the L2 still holding the masks.
+constexpr int kOffWaves = 16;
+static_assert(kESeg % kOffWaves == 0, "segments per wave");
+
+__global__ void __launch_bounds__(1024) estep_seg_end_kernel(EParams p) {
+  constexpr int SPW = kESeg / kOffWaves;
+  __shared__ int32_t part[kOffWaves][kWave];
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid / kWave;
   const int64_t n = (int64_t)blockIdx.x * kWave + lane;
-  const int64_t seg = (int64_t)blockIdx.y * 4 + threadIdx.x / kWave;
-  if (n >= p.nb || seg >= kESeg) return;
-  const int64_t c0 = seg * p.seg_len;
-  const int64_t c1 = (c0 + p.seg_len) < p.nchunks ? (c0 + p.seg_len) : p.nchunks;
+  const bool ok = n < p.nb;
+  int32_t c[SPW];
+#pragma unroll
+  for (int k = 0; k < SPW; ++k) c[k] = ok ? p.seg_tot[(int64_t)(w * SPW + k) * p.nb + n] : 0;
   int32_t t = 0;
-  for (int64_t cb = c0; cb < c1; cb += 8) {
-    uint64_t v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
+  for (int k = 0; k < SPW; ++k) t += c[k];
+  part[w][lane] = t;
+  __syncthreads();
+  int32_t run = 0;
+  for (int ww = 0; ww < w; ++ww) run += part[ww][lane];
+  if (ok) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t += __popcll(v[u]);
+    for (int k = 0; k < SPW; ++k) {
+      run += c[k];
+      p.seg_end[(int64_t)(w * SPW + k) * p.nb + n] = run;
+      p.seg_tot[(int64_t)(w * SPW + k) * p.nb + n] = 0;
+    }
+    if (w == kOffWaves - 1) p.row_tot[n] = run;
   }
-  p.seg_tot[seg * p.nb + n] = t;
 }
 
-// rowptr = exclusive scan of the row totals (sums of seg_tot over the
-// segments); nnz = rowptr[nb].  One block.
 __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   __shared__ int32_t sh[1024];
   const int tid = threadIdx.x;
@@ -69,19 +87,19 @@ __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   const int64_t r0 = tid * per;
   const int64_t r1 = (r0 + per) < p.nb ? (r0 + per) : p.nb;
   int32_t s = 0;
-  for (int64_t r = r0; r < r1; ++r) s += strided_sum(p.seg_tot + r, p.nb, 0, kESeg);
+  for (int64_t r = r0; r < r1; ++r) s += p.row_tot[r];
   sh[tid] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    int32_t v = (tid >= off) ? sh[tid - off] : 0;
+    const int32_t v = (tid >= off) ? sh[tid - off] : 0;
     __syncthreads();
     sh[tid] += v;
     __syncthreads();
   }
-  int32_t run = sh[tid] - s;
+  int32_t pos = sh[tid] - s;
   for (int64_t r = r0; r < r1; ++r) {
-    p.rowptr[r] = run;
-    if (r + 1 < r1) run += strided_sum(p.seg_tot + r, p.nb, 0, kESeg);
+    p.rowptr[r] = pos;
+    pos += p.row_tot[r];
   }
   if (tid == 1023) {
     p.rowptr[p.nb] = sh[1023];
@@ -329,9 +347,20 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->old_colsum = a.old_colsum;
   p->c = a.c;
   char *ws = (char *)a.workspace;
-  const int64_t cells = a.nb * p->nchunks;
+  // The counters that must read zero on entry (seg_tot) sit at the
+  // END of the workspace, the scratch arrays at its start.  Blocks of
+  // different sizes share one workspace (the last block of a fit is
+  // smaller); with the counters anchored at the end, a launch with nb
+  // kernels uses a part of the counter region of any larger block, and its
+  // scratch lies inside the larger block's scratch, so the counters are never
+  // overwritten and every launch finds (and leaves) them zero.
+  const size_t segb = align256((size_t)kESeg * a.nb * 4);
+  const size_t maskb = align256((size_t)a.nb * p->nchunks * 8);
+  char *end = ws ? (char *)(((uintptr_t)ws + a.workspace_bytes) & ~(uintptr_t)255) : ws + 512;
+  p->seg_tot = (int32_t *)(end - 256 - segb);
   p->masks = (uint64_t *)ws;
-  p->seg_tot = (int32_t *)(ws + align256((size_t)cells * 8));
+  p->seg_end = (int32_t *)(ws + maskb);
+  p->row_tot = (int32_t *)(ws + maskb + segb);
   p->seg_len = ceil_div(p->nchunks, kESeg);
   p->rowptr = a.rowptr;
   p->out_row = a.out_row;
@@ -391,14 +420,13 @@ int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int3
   return LAKDE_OK;
 }
 
-// the row counts and row pointer of an E-step (the first half of
-// lakde_estep_finish; *nnz_dev is final afterwards)
+// the per-segment offsets, row pointer and entry count of an E-step (the
+// first half of lakde_estep_finish; *nnz_dev is final afterwards)
 int estep_count(const lakde_estep_args &a, hipStream_t s) {
   EParams p;
   int rc = estep_fill_params(a, &p);
   if (rc) return rc;
-  hipLaunchKernelGGL(estep_seg_count_kernel,
-                     dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
+  hipLaunchKernelGGL(estep_seg_end_kernel, dim3((unsigned)ceil_div(p.nb, kWave)), dim3(1024), 0,
                      s, p);
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(estep_rowptr_kernel, dim3(1), dim3(1024), 0, s, p);
@@ -414,7 +442,10 @@ extern "C" {
 
 size_t lakde_estep_workspace(int64_t N, int64_t nb) {
   const int64_t nchunks = ceil_div(N, kWave);
-  return align256((size_t)nb * nchunks * 8) + align256((size_t)kESeg * nb * 4);
+  // scratch (masks, segment ends, row totals), the segment counts, 256
+  // bytes at the end and 256 bytes of alignment slack
+  return align256((size_t)nb * nchunks * 8) + 2 * align256((size_t)kESeg * nb * 4) +
+         align256((size_t)nb * 4) + 256 + 256;
 }
 
 int lakde_estep_main(const lakde_estep_args *args, lakde_stream_t stream) {

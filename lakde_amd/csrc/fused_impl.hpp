@@ -21,6 +21,9 @@ struct EParams {
   float *c;
   uint64_t *masks;   // nchunks x nb (chunk-major: a wave's 64 kernels are contiguous)
   int32_t *seg_tot;  // kESeg x nb: kept entries of kernel n in chunk segment s
+                     // (accumulated by the main kernel, cleared by estep_seg_end_kernel)
+  int32_t *seg_end;  // kESeg x nb: kept entries of kernel n in segments 0..s
+  int32_t *row_tot;  // nb: kept entries of kernel n
   int64_t seg_len;   // chunks per segment
   int32_t *rowptr;   // nb + 1
   int32_t *out_row, *out_col;
@@ -59,20 +62,10 @@ struct PParams {
 constexpr int kEThreads = 256;  // 4 waves; lane = column, wave = kernel quarter
 constexpr int kESeg = 128;      // chunk segments of the row scan and the compaction
 
-// sum of p[k * stride] for k in [k0, k1), 8 loads in flight
-__device__ __forceinline__ int32_t strided_sum(const int32_t *p, int64_t stride, int64_t k0,
-                                               int64_t k1) {
-  int32_t t = 0;
-  int64_t k = k0;
-  for (; k + 8 <= k1; k += 8) {
-    int32_t v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = p[(k + u) * stride];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) t += v[u];
-  }
-  for (; k < k1; ++k) t += p[k * stride];
-  return t;
+// a written mask's kept entries, added to its (segment, kernel) count
+__device__ __forceinline__ void count_mask(const EParams &p, int64_t seg, int64_t n,
+                                           uint64_t mask) {
+  if (mask != 0ull) atomicAdd(&p.seg_tot[seg * p.nb + n], (int32_t)__popcll(mask));
 }
 
 // ---------------------------------------------------------------- E-step
@@ -209,6 +202,7 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
     cm = valid ? p.c[m] : 0.f;
   }
   if (MODE == LAKDE_ESTEP_ACCUM) return;
+  const int64_t cseg = chunk / p.seg_len;
 
   if (MODE == LAKDE_ESTEP_UPDATE && __ballot(cnt > kEQueue) == 0) {
     // exact test of the queued candidates, 64 kernels at a time through a
@@ -226,7 +220,11 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
       }
       __builtin_amdgcn_wave_barrier();
       const int64_t n = nbeg + wb + lane;
-      if (n < nend) p.masks[chunk * p.nb + n] = sm_win[w][lane];
+      if (n < nend) {
+        const uint64_t mk = sm_win[w][lane];
+        p.masks[chunk * p.nb + n] = mk;
+        count_mask(p, cseg, n, mk);
+      }
       __builtin_amdgcn_wave_barrier();
     }
     return;
@@ -239,13 +237,16 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
     float r;
     const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
     const uint64_t mask = __ballot(keep);
-    if (lane == 0) p.masks[chunk * p.nb + n] = mask;
+    if (lane == 0) {
+      p.masks[chunk * p.nb + n] = mask;
+      count_mask(p, cseg, n, mask);
+    }
   }
 }
 
 // Compaction: a wave per (64 kernels, chunk segment), lane = kernel n; the
 // lane walks the segment's chunks in column order and writes each kept
-// entry at rowptr[n] + (kept in earlier segments) + (kept so far),
+// entry at rowptr[n] + seg_end[seg - 1][n] + (kept so far),
 // recomputing it with the same device functions (identical bits) ->
 // canonical CSR.
 template <int D>
@@ -258,35 +259,55 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
   if (n >= p.nb || seg >= kESeg) return;
   const int64_t c0 = seg * p.seg_len;
   const int64_t c1 = (c0 + p.seg_len) < p.nchunks ? (c0 + p.seg_len) : p.nchunks;
-  if (c0 >= c1 || p.seg_tot[seg * p.nb + n] == 0) return;
+  const int32_t s_lo = seg > 0 ? p.seg_end[(seg - 1) * p.nb + n] : 0;
+  if (c0 >= c1 || p.seg_end[seg * p.nb + n] == s_lo) return;
   constexpr int64_t R = rec_floats(D);
   const float *rec = p.rec + n * R;
-  int64_t pos = (int64_t)p.rowptr[n] + strided_sum(p.seg_tot + n, p.nb, 0, seg);
-  for (int64_t cb = c0; cb < c1; cb += 8) {
-    uint64_t v[8];
+  int64_t pos = (int64_t)p.rowptr[n] + s_lo;
+  // kept columns are gathered KB at a time so their point loads are in
+  // flight together (the chain per lane is otherwise one load per entry)
+  constexpr int KB = D <= 8 ? 4 : (D <= 24 ? 2 : 1);
+  int64_t ms[KB];
+  int k = 0;
+  auto flush = [&]() {
+    float xv[KB][D];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
+    for (int i = 0; i < KB; ++i)
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      uint64_t mask = v[u];
-      while (mask) {
-        const int b = __builtin_ctzll(mask);
-        mask &= mask - 1;
-        const int64_t m = (cb + u) * kWave + b;
-        float xm[D];
+      for (int j = 0; j < D; ++j) xv[i][j] = i < k ? p.X[ms[i] * D + j] : 0.f;
 #pragma unroll
-        for (int j = 0; j < D; ++j) xm[j] = p.X[m * D + j];
-        const float x = log_rho<D>(rec, xm);
-        const float d = __fsub_rn(x, p.c[m]);
+    for (int i = 0; i < KB; ++i) {
+      if (i < k) {
+        const float x = log_rho<D>(rec, xv[i]);
+        const float d = __fsub_rn(x, p.c[ms[i]]);
         float r = 0.f;
         keep_resp(d, p.threshold, p.log_t_lo, &r);
-        p.out_row[pos] = (int32_t)n;
-        p.out_col[pos] = (int32_t)m;
-        p.out_val[pos] = r;
+        if (pos >= 0 && pos < p.cap) {  // never write outside the arena slots
+          p.out_row[pos] = (int32_t)n;
+          p.out_col[pos] = (int32_t)ms[i];
+          p.out_val[pos] = r;
+        }
         ++pos;
       }
     }
+    k = 0;
+  };
+  constexpr int MB = 16;  // mask loads in flight
+  for (int64_t cb = c0; cb < c1; cb += MB) {
+    uint64_t v[MB];
+#pragma unroll
+    for (int u = 0; u < MB; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
+#pragma unroll
+    for (int u = 0; u < MB; ++u) {
+      uint64_t mask = v[u];
+      while (mask) {
+        ms[k++] = (cb + u) * kWave + __builtin_ctzll(mask);
+        mask &= mask - 1;
+        if (k == KB) flush();
+      }
+    }
   }
+  if (k > 0) flush();
 }
 
 // ------------------------------------------------------------ predictive

@@ -94,6 +94,20 @@ __device__ __forceinline__ void tril_inv_reg(const T (&L)[D * (D + 1) / 2],
   }
 }
 
+// sum_i log(L_ii) in FP64, the logs computed one per lane and summed in
+// index order (every lane holds the same factor and gets the same sum)
+template <int D>
+__device__ __forceinline__ double diag_log_sum(const float (&L)[D * (D + 1) / 2], int lane) {
+  float v = 1.f;
+#pragma unroll
+  for (int i = 0; i < D; ++i) v = lane == i ? L[tri(i, i)] : v;
+  const double lg = log((double)v);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) s += __shfl(lg, i, kWave);
+  return s;
+}
+
 // stabilized_cholesky's schedule: the stored jitter first; on failure
 // restart from 0 and add eps * 2^k (k = 0..31, accumulated in FP32)
 template <int D>
@@ -156,32 +170,44 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
   for (int t = 0; t < P; ++t) S[t] = 0.f;
   float nu = p.nu0;
   const int32_t beg = p.rowptr[n], end = p.rowptr[n + 1];
-  for (int32_t e0 = beg; e0 < end; e0 += kWave) {
-    const int32_t e = e0 + lane;
-    float w = 0.f, d[D];
+  // EB wavefronts of entries per pass: all their loads are issued before
+  // any is used (long rows, the kernels with many kept points, would
+  // otherwise serialise one load latency per 64 entries)
+  constexpr int EB = 4;
+  for (int32_t e0 = beg; e0 < end; e0 += EB * kWave) {
+    int32_t c[EB];
+    float u[EB], g[EB], d[EB][D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) d[j] = 0.f;
-    if (e < end) {
-      const int32_t c = p.col[e];
-      const float u = p.val[e];
-      w = p.colscale != nullptr ? u * p.colscale[c] : u;
+    for (int i = 0; i < EB; ++i) {
+      const int32_t e = e0 + i * kWave + lane;
+      c[i] = e < end ? p.col[e] : -1;
+      u[i] = e < end ? p.val[e] : 0.f;
+    }
 #pragma unroll
-      for (int j = 0; j < D; ++j) d[j] = p.X[(int64_t)c * D + j] - xn[j];
-      if (p.old_colsum != nullptr) {
-        atomicAdd(&p.old_colsum[c], w);
-        atomicAdd(&p.U_old[c], (double)u);
+    for (int i = 0; i < EB; ++i) {
+      g[i] = (c[i] >= 0 && p.colscale != nullptr) ? p.colscale[c[i]] : 1.f;
+#pragma unroll
+      for (int j = 0; j < D; ++j) d[i][j] = c[i] >= 0 ? p.X[(int64_t)c[i] * D + j] - xn[j] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const float w = p.colscale != nullptr ? u[i] * g[i] : u[i];
+      if (c[i] >= 0 && p.old_colsum != nullptr) {
+        atomicAdd(&p.old_colsum[c[i]], w);
+        atomicAdd(&p.U_old[c[i]], (double)u[i]);
       }
-    }
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-      const float wd = w * d[i];
+      for (int ii = 0; ii < D; ++ii) {
+        const float wd = w * d[i][ii];
 #pragma unroll
-      for (int j = 0; j <= i; ++j) S[tri(i, j)] = fmaf(wd, d[j], S[tri(i, j)]);
+        for (int j = 0; j <= ii; ++j) S[tri(ii, j)] = fmaf(wd, d[i][j], S[tri(ii, j)]);
+      }
+      // nu in entry order, as scatter_add
+      const int32_t eb = e0 + i * kWave;
+      const int cnt = (end - eb) < kWave ? (end - eb) : kWave;
+      const int wb = __float_as_int(w);
+      for (int l = 0; l < cnt; ++l) nu += __int_as_float(__builtin_amdgcn_readlane(wb, l));
     }
-    // nu in entry order, as scatter_add
-    const int cnt = (end - e0) < kWave ? (end - e0) : kWave;
-    const int wb = __float_as_int(w);
-    for (int l = 0; l < cnt; ++l) nu += __int_as_float(__builtin_amdgcn_readlane(wb, l));
   }
   wave_sum_all<P>(S);
 
@@ -190,12 +216,10 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
   if (knn) {
     bool ok;
     const float j0 = stab_chol_reg<D>(pri, a.prior_jitter[n], T, &ok);
+    const double ld0 = diag_log_sum<D>(T, lane);
     if (lane == 0) {
       if (!ok) *a.status = 1;
       a.prior_jitter[n] = j0;
-      double ld0 = 0.0;
-#pragma unroll
-      for (int i = 0; i < D; ++i) ld0 += log((double)T[tri(i, i)]);
       a.logdet_prior[n] = (float)(2.0 * ld0);
 #pragma unroll
       for (int i = 0; i < D; ++i)
@@ -232,10 +256,7 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
   }
 
   // ---- logdet, trace = nu ||T^T Linv^T||_F^2 (lakde/linalg_util.py:30-37)
-  double lsum = 0.0;
-#pragma unroll
-  for (int i = 0; i < D; ++i) lsum += log((double)Li[tri(i, i)]);
-  const float logdet = (float)(2.0 * lsum);
+  const float logdet = (float)(2.0 * diag_log_sum<D>(Li, lane));
   float tr = 0.f;
 #pragma unroll
   for (int c = 0; c < D; ++c) {
@@ -272,8 +293,8 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
     for (int t = 0; t < P; ++t) rec[t] = Li[t];
 #pragma unroll
     for (int j = 0; j < D; ++j) rec[rec_x(D) + j] = xn[j];
-    record_consts(rec + rec_c(D), nu, logdet, D);
   }
+  record_consts_wave(a.rec + n * rec_floats(D) + rec_c(D), nu, logdet, D, lane);
 }
 
 template <int D>

@@ -35,4 +35,28 @@ __device__ inline void record_consts(float *cst, float nu, float logdet, int D) 
   cst[3] = h1;
 }
 
+// record_consts with the D digamma terms and the two lgamma calls spread
+// over the lanes of a wave (every lane holds the same nu and logdet; D <= 62);
+// the terms are summed in the same order, so the constants are identical.
+// Lane 0 writes cst.
+__device__ inline void record_consts_wave(float *cst, float nu, float logdet, int D, int lane) {
+  const float half_nu = nu / 2.f;
+  const float h1 = (nu + 1.f) / 2.f;
+  const float h2 = ((nu + 1.f) - (float)D) / 2.f;
+  double t = 0.0;
+  if (lane < D) t = digamma_d((double)(half_nu + (1.f - (float)(lane + 1)) / 2.f));
+  else if (lane <= D + 1) t = lgamma((double)(lane == D ? h1 : h2));
+  double mvd = 0.0;
+  for (int i = 0; i < D; ++i) mvd += __shfl(t, i, kWave);
+  const float lg1 = (float)__shfl(t, D, kWave);
+  const float lg2 = (float)__shfl(t, D + 1, kWave);
+  if (lane == 0) {
+    const float eld = ((float)mvd + (float)(D * 0.69314718055994530942)) + logdet;
+    cst[0] = 0.5f * eld;
+    cst[1] = -0.5f * nu;
+    cst[2] = ((lg1 - lg2) + 0.5f * logdet) - (float)(D / 2.0 * 1.14472988584940017414);
+    cst[3] = h1;
+  }
+}
+
 }  // namespace lakde

@@ -15,7 +15,7 @@
 //   [exact rebalance: old column sums]          lakde_colsum
 //   E-step main (+ optional events), with the   lakde_estep_main[_bf16]
 //     lazy unsafe-step test                       (+ lakde_rebalance_lazy_check, fused)
-//   row counts                                  first half of lakde_estep_finish
+//   segment offsets, row pointer, entry count   first half of lakde_estep_finish
 //   -- copy of the arena counts and flags --
 //   compaction, guarded lazy rebalance          lakde_estep_compact, lakde_rebalance_lazy
 //   -- wait for the copy; host-side outcome: exact fallback / lakde_rebalance --
@@ -129,7 +129,7 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
                                : lakde_estep_main(&em, s);
   if (rc) return rc;
   if (a->ev_main[1]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[1], s));
-  // counts first: the flags the host needs are final before the compaction
+  // offsets first: the flags the host needs are final before the compaction
   rc = estep_count(e, s);
   if (rc) return rc;
   LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all, (size_t)(a->num_blocks + 2) * sizeof(int32_t),
