@@ -2,11 +2,12 @@
 // kNN prior covariance, the contributions S_n and nu_n of the block's
 // responsibilities (with the lazy rebalance's old column sums), the two
 // jittered factorisations, L^{-1}, log-determinants, the trace term, the tau
-// update and the packed record, from one launch.  One wavefront per kernel
-// point: lanes gather the neighbours / entries, a butterfly reduction leaves
-// the D(D+1)/2 sums in every lane, and every lane then runs the same
-// register-resident factorisation, so nothing intermediate goes through HBM
-// or LDS.  For larger D the step uses lakde_knn_prior_cov,
+// update and the packed record, from one launch.  One workgroup of four
+// wavefronts per kernel point: lanes gather the neighbours / entries,
+// butterfly reductions leave the D(D+1)/2 sums in every lane, the waves'
+// partial sums are combined through LDS, and every lane of wave 0 then runs
+// the same register-resident factorisation, so nothing intermediate goes
+// through HBM.  For larger D the step uses lakde_knn_prior_cov,
 // lakde_posterior_contribs and lakde_posterior (posterior.hip), whose
 // arithmetic this kernel follows operation by operation: the same FP32
 // jitter decisions, the same FP64 refactorisation of jittered factors.
@@ -124,21 +125,27 @@ __device__ float stab_chol_reg(const float (&M)[D * (D + 1) / 2], float jitter_i
   return j;
 }
 
+// waves per kernel point: all share the row's entries, wave 0 also builds
+// the kNN prior and then runs the factorisation
+constexpr int kPostWaves = 4;
+
 template <int D>
-__global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
+__global__ void __launch_bounds__(kPostWaves * kWave) post_small_kernel(SmallPost p) {
   constexpr int P = D * (D + 1) / 2;
+  __shared__ float sm_part[kPostWaves][P + 1];
   const lakde_posterior_args &a = p.post;
   const int64_t n = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (p.unsafe != nullptr && n == 0 && lane == 0) *p.unsafe = 0;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (p.unsafe != nullptr && n == 0 && threadIdx.x == 0) *p.unsafe = 0;
   float xn[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) xn[j] = p.X[(p.row0 + n) * D + j];
 
-  // ---- prior covariance (lakde/utils.py:40-65, self-centred, / K)
+  // ---- prior covariance (lakde/utils.py:40-65, self-centred, / K), wave 0
   float pri[P];
   const bool knn = a.prior_mode == LAKDE_PRIOR_KNN;
-  if (knn) {
+  if (knn && w == 0) {
 #pragma unroll
     for (int t = 0; t < P; ++t) pri[t] = 0.f;
     for (int64_t k = lane; k < p.K; k += kWave) {
@@ -156,7 +163,7 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
     const float fk = (float)p.K;
 #pragma unroll
     for (int t = 0; t < P; ++t) pri[t] = pri[t] / fk;
-  } else {
+  } else if (!knn) {
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -164,17 +171,18 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
   }
 
   // ---- contributions S_n = sum_e r_e d_e d_e^T, nu_n = nu0 + sum_e r_e
-  // (lakde/kernels.py:26-39, lakde/local_adaptive_kde.py:66-81)
+  // (lakde/kernels.py:26-39, lakde/local_adaptive_kde.py:66-81).  The
+  // waves take EB wavefronts of entries each per pass, and all loads of a
+  // pass are issued before any is used: the long rows (kernels with many
+  // kept points) set the launch's time.  Partial sums are combined in a
+  // fixed order (deterministic, not the reference's sequential order).
   float S[P];
 #pragma unroll
   for (int t = 0; t < P; ++t) S[t] = 0.f;
-  float nu = p.nu0;
+  float wsum = 0.f;
   const int32_t beg = p.rowptr[n], end = p.rowptr[n + 1];
-  // EB wavefronts of entries per pass: all their loads are issued before
-  // any is used (long rows, the kernels with many kept points, would
-  // otherwise serialise one load latency per 64 entries)
   constexpr int EB = 4;
-  for (int32_t e0 = beg; e0 < end; e0 += EB * kWave) {
+  for (int32_t e0 = beg + w * EB * kWave; e0 < end; e0 += kPostWaves * EB * kWave) {
     int32_t c[EB];
     float u[EB], g[EB], d[EB][D];
 #pragma unroll
@@ -191,25 +199,33 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
     }
 #pragma unroll
     for (int i = 0; i < EB; ++i) {
-      const float w = p.colscale != nullptr ? u[i] * g[i] : u[i];
+      const float wt = p.colscale != nullptr ? u[i] * g[i] : u[i];
       if (c[i] >= 0 && p.old_colsum != nullptr) {
-        atomicAdd(&p.old_colsum[c[i]], w);
+        atomicAdd(&p.old_colsum[c[i]], wt);
         atomicAdd(&p.U_old[c[i]], (double)u[i]);
       }
 #pragma unroll
       for (int ii = 0; ii < D; ++ii) {
-        const float wd = w * d[i][ii];
+        const float wd = wt * d[i][ii];
 #pragma unroll
         for (int j = 0; j <= ii; ++j) S[tri(ii, j)] = fmaf(wd, d[i][j], S[tri(ii, j)]);
       }
-      // nu in entry order, as scatter_add
-      const int32_t eb = e0 + i * kWave;
-      const int cnt = (end - eb) < kWave ? (end - eb) : kWave;
-      const int wb = __float_as_int(w);
-      for (int l = 0; l < cnt; ++l) nu += __int_as_float(__builtin_amdgcn_readlane(wb, l));
+      wsum += wt;
     }
   }
   wave_sum_all<P>(S);
+  wsum = wave_sum(wsum);
+  if (lane == 0) {
+#pragma unroll
+    for (int t = 0; t < P; ++t) sm_part[w][t] = S[t];
+    sm_part[w][P] = wsum;
+  }
+  __syncthreads();
+  if (w != 0) return;
+#pragma unroll
+  for (int t = 0; t < P; ++t) S[t] = ((sm_part[0][t] + sm_part[1][t]) + sm_part[2][t]) + sm_part[3][t];
+  const float nu =
+      p.nu0 + (((sm_part[0][P] + sm_part[1][P]) + sm_part[2][P]) + sm_part[3][P]);
 
   // ---- prior factor with its persistent jitter (kNN)
   float T[P];  // lower factor of the prior (kNN); the shared prior is read in the trace
@@ -299,7 +315,8 @@ __global__ void __launch_bounds__(64) post_small_kernel(SmallPost p) {
 
 template <int D>
 static int launch_post_small(const SmallPost &p, hipStream_t s) {
-  hipLaunchKernelGGL((post_small_kernel<D>), dim3((unsigned)p.post.nb), dim3(kWave), 0, s, p);
+  hipLaunchKernelGGL((post_small_kernel<D>), dim3((unsigned)p.post.nb), dim3(kPostWaves * kWave),
+                     0, s, p);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }

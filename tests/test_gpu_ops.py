@@ -329,7 +329,7 @@ def _block_inputs(rng, N, D, nb, K, quantise):
         graph[:5] = graph[:5, :1]  # every neighbour the same point: zero covariance
     # ragged CSR rows: empty, short, and longer than one wavefront
     lens = rng.integers(0, 90, size=nb)
-    lens[0], lens[1], lens[2] = 0, 1, 200
+    lens[0], lens[1], lens[2] = 0, 1, 1500  # 1500: several passes of every wave
     rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     col = np.concatenate([np.sort(rng.choice(N, size=l, replace=False)) for l in lens])
     val = rng.random(col.size).astype(np.float32) * 0.5
@@ -392,10 +392,9 @@ def _block_posterior_both(X, graph, rowptr, col, val, colscale, row0, shared_pri
 def test_block_posterior_matches_separate_kernels(D, quantise, scaled):
     """lakde_block_posterior (one fused kernel for D <= 6; the separate
     kernels for D = 7) against lakde_knn_prior_cov + lakde_posterior_contribs
-    + lakde_posterior on ragged CSR rows (empty, 1 entry, 200 entries),
-    jittered priors and per-column factors: nu bit-exact (entry order),
-    factors and constants to 2e-5 relative (the sums are reduced in a
-    different order).  Jitter decisions are equal except where a rank-deficient
+    + lakde_posterior on ragged CSR rows (empty, 1 entry, 1500 entries),
+    jittered priors and per-column factors: nu to 2e-6 and factors and
+    constants to 2e-5 relative (the sums are reduced in a different order).  Jitter decisions are equal except where a rank-deficient
     prior sits on the FP32 decision boundary (rounding of the differently
     ordered sums decides); such rows must be few and are checked to carry a
     jitter of the reference schedule."""
@@ -404,7 +403,7 @@ def test_block_posterior_matches_separate_kernels(D, quantise, scaled):
     fu, ref = _block_posterior_both(X, graph, rowptr, col, val, colscale if scaled else None,
                                     row0=700)
     assert fu["status"][0] == 0 and ref["status"][0] == 0
-    np.testing.assert_array_equal(fu["nu"], ref["nu"])
+    np.testing.assert_allclose(fu["nu"], ref["nu"], rtol=2e-6)
     if quantise:
         assert np.any(ref["jp"] > 0)
     same = (fu["jp"] == ref["jp"]) & (fu["jw"] == ref["jw"])
@@ -443,7 +442,7 @@ def test_block_posterior_shared_prior():
         prior = (C @ C.T * 0.1 + np.eye(D, dtype=np.float32)).astype(np.float32)
         fu, ref = _block_posterior_both(X, None, rowptr, col, val, None, row0=0,
                                         shared_prior=prior)
-        np.testing.assert_array_equal(fu["nu"], ref["nu"])
+        np.testing.assert_allclose(fu["nu"], ref["nu"], rtol=2e-6)
         np.testing.assert_array_equal(fu["jw"], ref["jw"])
         for k in ("logdet", "trace", "b_out"):
             np.testing.assert_allclose(fu[k], ref[k], rtol=2e-5, atol=2e-5, err_msg=k)
