@@ -25,7 +25,7 @@ PEAK_HBM_GBS = 8000.0
 # size 512: rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs, KiB per
 # dispatch) scaled by what they read for a plain 512 MiB elementwise copy
 # (factors 1.9999 and 1.0000); summary in profiles/estep_mix32_d6_hbm_traffic.md
-MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 7367287, "write": 13587724}}
+MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 8432194, "write": 13862526}}
 
 
 def _parse():
@@ -130,20 +130,53 @@ def cpu_baseline(X, graph, block_size, threshold, seconds):
                       f"{dt:.3f} s each, extrapolated to {nb} steps per VI iteration"}
 
 
+def pred_fp64(rec, D, Y, kchunk=2048, pchunk=2000):
+    """The predictive density evaluated in float64 arithmetic (torch on the
+    GPU) from the same float32 kernel records the HIP kernels read:
+    log p(y) = -ln N + LSE_n [coeff_n - h_n log1p(||Linv_n (y - x_n)||^2)]
+    (lakde/local_adaptive_kde.py:209-223).  The difference to a kernel's
+    output is that kernel's arithmetic error, in nats."""
+    import torch
+    N = rec.size(0)
+    P = D * (D + 1) // 2
+    ti = torch.tril_indices(D, D, device=rec.device)
+    Y64 = Y.double()
+    M = Y.size(0)
+    m_run = torch.full((M,), float("-inf"), dtype=torch.float64, device=Y.device)
+    s_run = torch.zeros(M, dtype=torch.float64, device=Y.device)
+    for k0 in range(0, N, kchunk):
+        r = rec[k0:k0 + kchunk].double()
+        kc = r.size(0)
+        L = torch.zeros(kc, D, D, dtype=torch.float64, device=Y.device)
+        L[:, ti[0], ti[1]] = r[:, :P]
+        Lt = L.transpose(1, 2).contiguous()
+        xc, coeff, h = r[:, P:P + D], r[:, P + D + 2], r[:, P + D + 3]
+        for p0 in range(0, M, pchunk):
+            d = (Y64[None, p0:p0 + pchunk, :] - xc[:, None, :]).contiguous()
+            z = torch.bmm(d, Lt)
+            lp = coeff[:, None] - h[:, None] * torch.log1p((z * z).sum(-1))
+            mx = torch.maximum(lp.max(0).values, m_run[p0:p0 + pchunk])
+            s_run[p0:p0 + pchunk] = (s_run[p0:p0 + pchunk] * torch.exp(m_run[p0:p0 + pchunk] - mx)
+                                     + torch.exp(lp - mx[None]).sum(0))
+            m_run[p0:p0 + pchunk] = mx
+    return m_run + torch.log(s_run) - math.log(N)
+
+
 def precision_form(seconds_budget=None):
     """The precision comparison on the full-scale d=63 workload
     (mix128_d63: N=1,000,000, D=63, k=128, threshold 1/128, block size 512):
     initialise once (BF16 form), then time partial steps of the FP32 and the
     BF16 E-step from the same state and extrapolate to a VI iteration
-    (num_blocks partial steps); predictive density on 5,000 held-out points
-    with both forms on the same posterior."""
+    (num_blocks partial steps); predictive density on 10,000 held-out points
+    with both forms on the same posterior; both against float64 arithmetic
+    on the same records (10,000 held-out points)."""
     import torch
 
     from lakde_amd import LocalKNearestKDE, synthetic
     from lakde_amd.utils import compute_knn_graph
 
     w = synthetic.WORKLOADS["mix128_d63"]
-    Xh, Yh = synthetic.make("mix128_d63", n_test=5000)
+    Xh, Yh = synthetic.make("mix128_d63", n_test=10000)
     N, D = Xh.shape
     dev = torch.device("cuda:0")
     X = torch.from_numpy(Xh).to(dev)
@@ -211,6 +244,17 @@ def precision_form(seconds_budget=None):
     d = (lps["bf16"] - lps["fp32"]).abs()
     res["pred_abs_err_bf16_vs_fp32_nats"] = {"mean": float(d.mean()), "max": float(d.max())}
     res["mean_log_density_fp32"] = float(lps["fp32"].mean())
+    # both forms against float64 arithmetic on the same records, on the
+    # 10,000 held-out points
+    t0 = time.perf_counter()
+    lp64 = pred_fp64(model.rec, D, Y)
+    torch.cuda.synchronize()
+    for prec in ("fp32", "bf16"):
+        e = (lps[prec].double() - lp64).abs()
+        res[prec]["pred_err_vs_fp64_nats"] = {"mean": float(e.mean()), "max": float(e.max()),
+                                              "points": int(Y.size(0))}
+    print(f"# precision form: float64 predictive on {Y.size(0)} points in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     del model, X, Y, graph
     torch.cuda.empty_cache()
     return res
