@@ -327,7 +327,21 @@ typedef struct {
   int32_t *host_flags;     /* pinned host, num_blocks + 2 */
   void *ev_main[2];
   void *ev_flags;          /* optional hipEvent_t: the wait is for the flags copy only */
+  /* optional, from lakde_flag_words_alloc: the row-pointer kernel writes this
+   * block's entry count and the status / unsafe flags there (host-coherent
+   * memory, sequence-tagged words) and the driver waits for them instead of a
+   * copy and an event; it stores them into host_flags[block],
+   * host_flags[num_blocks], host_flags[num_blocks + 1] */
+  void *flag_words;
 } lakde_step_args;
+
+/* A 256-byte host-coherent, device-mapped buffer (hipHostMalloc,
+ * hipHostMallocCoherent | hipHostMallocMapped) for lakde_step_args.flag_words:
+ * words 0-2 receive the flags, word 3 is the driver's step counter, word 4
+ * holds the device-side address; one per concurrently driven model.  No reference counterpart: it replaces the per-step device-to-host
+ * copy of the flags and the event the host waits on. */
+int lakde_flag_words_alloc(void **out);
+int lakde_flag_words_free(void *words);
 
 int lakde_partial_step(const lakde_step_args *args, lakde_stream_t stream);
 int lakde_partial_step_resume(const lakde_step_args *args, lakde_stream_t stream);

@@ -1,6 +1,7 @@
 """Thin wrappers of the fused entry points and the block arena that holds the
 responsibilities on the device."""
 
+import ctypes
 import math
 
 import torch
@@ -64,6 +65,24 @@ class Arena:
 
     def total_nnz(self):
         return sum(self.nnz)
+
+
+class FlagWords:
+    """The step driver's host-coherent flag words (lakde_flag_words_alloc):
+    the row-pointer kernel writes each partial step's entry count and flags
+    there, so the host waits for them without a copy or an event."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+        p = ctypes.c_void_p()
+        check(self._lib.lakde_flag_words_alloc(ctypes.byref(p)), "flag_words_alloc")
+        self.ptr = p.value
+
+    def __del__(self):
+        ptr_ = getattr(self, "ptr", None)
+        if ptr_:
+            self._lib.lakde_flag_words_free(ctypes.c_void_p(ptr_))
+            self.ptr = None
 
 
 def knn_prior_cov(X, row0, graph_blk, out):

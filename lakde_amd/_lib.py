@@ -55,7 +55,7 @@ class StepArgs(ctypes.Structure):
         ("nnz_all", _vp), ("cols_all", _vp), ("vals_all", _vp),
         ("G", _vp), ("U", _vp), ("U_old", _vp), ("lazy_ws", _vp), ("rebal_ws", _vp),
         ("fold_first", _int), ("bf16_frag", _vp), ("host_flags", _vp), ("ev_main", _vp * 2),
-        ("ev_flags", _vp),
+        ("ev_flags", _vp), ("flag_words", _vp),
     ]
 
 
@@ -85,6 +85,8 @@ _SIGNATURES = {
     "lakde_rebalance_lazy": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _vp]),
     "lakde_rebalance_fold": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lakde_flag_words_alloc": (_int, [ctypes.POINTER(_vp)]),
+    "lakde_flag_words_free": (_int, [_vp]),
     "lakde_partial_step": (_int, [ctypes.POINTER(StepArgs), _vp]),
     "lakde_partial_step_resume": (_int, [ctypes.POINTER(StepArgs), _vp]),
     "lakde_partial_sweep": (_int, [ctypes.POINTER(StepArgs), _i64, _i64, ctypes.POINTER(_vp),

@@ -2,6 +2,7 @@
 // their dimension-independent kernels (row scans, rebalance passes, the
 // split combine).  The D-templated kernels live in fused_impl.hpp.
 #include <cfloat>
+#include <cstring>
 
 #include "fused_impl.hpp"
 
@@ -104,6 +105,14 @@ __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   if (tid == 1023) {
     p.rowptr[p.nb] = sh[1023];
     *p.nnz_dev = sh[1023];
+    if (p.flag_words != nullptr) {
+      const unsigned long long tag = (unsigned long long)p.flag_seq << 32;
+      const unsigned long long w[3] = {tag | (uint32_t)sh[1023], tag | (uint32_t)*p.status_flag,
+                                       tag | (uint32_t)*p.unsafe_flag};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        __hip_atomic_store(&p.flag_words[i], w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
@@ -370,6 +379,9 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->nnz_dev = a.nnz_dev;
   p->lazy_G = nullptr;
   p->lazy_unsafe = nullptr;
+  p->flag_words = nullptr;
+  p->status_flag = p->unsafe_flag = nullptr;
+  p->flag_seq = 0;
   if (a.mode == LAKDE_ESTEP_UPDATE && a.lazy_unsafe != nullptr) {
     LAKDE_CHECK_ARG(a.lazy_G != nullptr, "lazy_unsafe needs lazy_G");
     p->lazy_G = a.lazy_G;
@@ -422,10 +434,15 @@ int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int3
 
 // the per-segment offsets, row pointer and entry count of an E-step (the
 // first half of lakde_estep_finish; *nnz_dev is final afterwards)
-int estep_count(const lakde_estep_args &a, hipStream_t s) {
+int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *flag_words,
+                const int32_t *status_flag, const int32_t *unsafe_flag, uint32_t flag_seq) {
   EParams p;
   int rc = estep_fill_params(a, &p);
   if (rc) return rc;
+  p.flag_words = flag_words;
+  p.status_flag = status_flag;
+  p.unsafe_flag = unsafe_flag;
+  p.flag_seq = flag_seq;
   hipLaunchKernelGGL(estep_seg_end_kernel, dim3((unsigned)ceil_div(p.nb, kWave)), dim3(1024), 0,
                      s, p);
   LAKDE_LAUNCHED();
@@ -550,6 +567,28 @@ int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(fold_reset_kernel, dim3(gN), dim3(256), 0, s, N, G);
   LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int lakde_flag_words_alloc(void **out) {
+  LAKDE_CHECK_ARG(out != nullptr, "null pointer");
+  *out = nullptr;
+  void *h = nullptr, *d = nullptr;
+  LAKDE_HIP(hipHostMalloc(&h, 256, hipHostMallocCoherent | hipHostMallocMapped));
+  std::memset(h, 0, 256);
+  const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(h);
+    LAKDE_HIP(e);
+  }
+  // word 4 keeps the device-side address of the words for the kernels
+  reinterpret_cast<unsigned long long *>(h)[4] = (unsigned long long)(uintptr_t)d;
+  *out = h;
+  return LAKDE_OK;
+}
+
+int lakde_flag_words_free(void *words) {
+  if (words != nullptr) LAKDE_HIP(hipHostFree(words));
   return LAKDE_OK;
 }
 

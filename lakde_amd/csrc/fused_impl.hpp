@@ -32,6 +32,12 @@ struct EParams {
   int32_t *nnz_dev;
   const float *lazy_G;   // optional unsafe-step test of the lazy rebalance
   int32_t *lazy_unsafe;
+  // optional (step driver): the row-pointer kernel also writes the entry
+  // count and the status / unsafe flags into host-coherent memory as 64-bit
+  // words (seq << 32 | value), each valid on its own when its tag is seq
+  unsigned long long *flag_words;
+  const int32_t *status_flag, *unsafe_flag;
+  uint32_t flag_seq;
 };
 
 // the lazy rebalance's unsafe-step test for column m of an UPDATE step
@@ -474,7 +480,9 @@ struct LazyGuard {
 };
 
 // shared host helpers (fused.hip)
-int estep_count(const lakde_estep_args &a, hipStream_t s);
+int estep_count(const lakde_estep_args &a, hipStream_t s,
+                unsigned long long *flag_words = nullptr, const int32_t *status_flag = nullptr,
+                const int32_t *unsafe_flag = nullptr, uint32_t flag_seq = 0);
 int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
                            const float *c_prev, float *c, float *G, double *U, double *U_old,
                            float *old_colsum, double *ws, LazyGuard guard, hipStream_t s);

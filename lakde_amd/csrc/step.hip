@@ -42,6 +42,37 @@ hipError_t wait_event(hipEvent_t ev) {
   return q;
 }
 
+// Wait for the flag words of step `seq` (written by the row-pointer kernel
+// into host-coherent memory, each word seq << 32 | value) and copy them into
+// host_flags.  If the stream completes without the words having arrived,
+// the flags are read from device memory instead, so the wait always ends.
+int wait_flag_words(const lakde_step_args *a, uint32_t seq, hipStream_t s) {
+  volatile unsigned long long *wd = reinterpret_cast<volatile unsigned long long *>(a->flag_words);
+  for (uint64_t it = 0;; ++it) {
+    const unsigned long long w0 = wd[0], w1 = wd[1], w2 = wd[2];
+    if ((uint32_t)(w0 >> 32) == seq && (uint32_t)(w1 >> 32) == seq &&
+        (uint32_t)(w2 >> 32) == seq) {
+      a->host_flags[a->block] = (int32_t)(uint32_t)w0;
+      a->host_flags[a->num_blocks] = (int32_t)(uint32_t)w1;
+      a->host_flags[a->num_blocks + 1] = (int32_t)(uint32_t)w2;
+      return LAKDE_OK;
+    }
+    if ((it & 1023) == 1023) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess) {
+        const unsigned long long v0 = wd[0], v1 = wd[1], v2 = wd[2];
+        if ((uint32_t)(v0 >> 32) == seq && (uint32_t)(v1 >> 32) == seq &&
+            (uint32_t)(v2 >> 32) == seq)
+          continue;
+        LAKDE_HIP(hipMemcpy(a->host_flags, a->nnz_all,
+                            (size_t)(a->num_blocks + 2) * sizeof(int32_t), hipMemcpyDeviceToHost));
+        return LAKDE_OK;
+      }
+      if (q != hipErrorNotReady) LAKDE_HIP(q);
+    }
+  }
+}
+
 int step_rebalance(const lakde_step_args *a, hipStream_t s) {
   const lakde_estep_args &e = a->estep;
   const int64_t N = e.N;
@@ -129,12 +160,28 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
                                : lakde_estep_main(&em, s);
   if (rc) return rc;
   if (a->ev_main[1]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[1], s));
-  // offsets first: the flags the host needs are final before the compaction
-  rc = estep_count(e, s);
-  if (rc) return rc;
-  LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all, (size_t)(a->num_blocks + 2) * sizeof(int32_t),
-                           hipMemcpyDeviceToHost, s));
-  if (a->ev_flags) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_flags, s));
+  // offsets first: the flags the host needs are final before the compaction;
+  // with flag_words the row-pointer kernel hands them to the host itself,
+  // otherwise one copy (and optionally an event to wait on)
+  uint32_t seq = 0;
+  if (a->flag_words != nullptr) {
+    volatile unsigned long long *wd =
+        reinterpret_cast<volatile unsigned long long *>(a->flag_words);
+    seq = (uint32_t)wd[3] + 1u;
+    if (seq == 0u) seq = 1u;
+    wd[3] = seq;
+    // the kernel writes through the words' device-side address (word 4)
+    auto *dev_words = reinterpret_cast<unsigned long long *>((uintptr_t)wd[4]);
+    rc = estep_count(e, s, dev_words, a->nnz_all + a->num_blocks, unsafe, seq);
+    if (rc) return rc;
+  } else {
+    rc = estep_count(e, s);
+    if (rc) return rc;
+    LAKDE_HIP(hipMemcpyAsync(a->host_flags, a->nnz_all,
+                             (size_t)(a->num_blocks + 2) * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, s));
+    if (a->ev_flags) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_flags, s));
+  }
   // the compaction (skips itself when the entries do not fit) and the usual
   // rebalance outcome run while the host reads the flags and prepares the
   // next step; the lazy kernels skip themselves for an unsafe or
@@ -147,7 +194,12 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
                                 LazyGuard{e.nnz_dev, a->cap, unsafe}, s);
     if (rc) return rc;
   }
-  LAKDE_HIP(a->ev_flags ? wait_event((hipEvent_t)a->ev_flags) : wait_stream(s));
+  if (a->flag_words != nullptr) {
+    rc = wait_flag_words(a, seq, s);
+    if (rc) return rc;
+  } else {
+    LAKDE_HIP(a->ev_flags ? wait_event((hipEvent_t)a->ev_flags) : wait_stream(s));
+  }
   if (a->host_flags[a->num_blocks] != 0) {
     set_error("lakde_partial_step: Stabilization did not converge");
     return LAKDE_ESTABILIZE;

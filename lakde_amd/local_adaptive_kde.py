@@ -124,6 +124,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._flags = self._host_flags.numpy()
         self._flag_event = torch.cuda.Event()  # the driver waits on it for the flags copy
         self._flag_event.record()
+        self._flag_words = F.FlagWords()  # ... or, when given, on these flag words
         self._step_cache = {}
         self._ews = F.EstepWorkspace(N, bs, dev)
         if self.precision == "bf16":
@@ -295,6 +296,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
             a.rebal_ws = self._rebal_ws.data_ptr()
         a.host_flags = self._host_flags.data_ptr()
         a.ev_flags = self._flag_event.cuda_event
+        a.flag_words = self._flag_words.ptr
         self._step_cache["built"][b] = True
         return a
 
