@@ -42,48 +42,47 @@ static int dispatch_pred_main(int64_t D, const PParams &p, hipStream_t s) {
 
 // ----------------------------------------------------------- row scans
 // The main kernel adds every written mask's kept entries to its (segment,
-// kernel) count.  estep_seg_end_kernel turns the counts into inclusive
-// per-segment ends (the compaction's write offsets within a row) and row
-// totals and clears the counts for the next E-step (blocks of 16 waves x 64
-// kernels, a wave per 8 segments); estep_rowptr_kernel scans the row totals
-// into the row pointer and the entry count (one block).  Two launches
-// instead of a last-block scan: a grid-wide handoff would need an
-// agent-scope fence, which writes back the L2 still holding the masks.
+// kernel) count and to the kernel's row total.  One launch then finishes the
+// offsets: blocks 0 .. G-1 (16 waves x 64 kernels, a wave per 8 segments)
+// turn the counts into inclusive per-segment ends (the compaction's write
+// offsets within a row); block G scans the row totals into the row pointer
+// and the entry count.  Both halves read only what the main kernel wrote,
+// so the blocks need no handoff, and each clears its counters for the next
+// E-step.
 constexpr int kOffWaves = 16;
 static_assert(kESeg % kOffWaves == 0, "segments per wave");
 
-__global__ void __launch_bounds__(1024) estep_seg_end_kernel(EParams p) {
-  constexpr int SPW = kESeg / kOffWaves;
+__global__ void __launch_bounds__(1024) estep_offsets_kernel(EParams p) {
   __shared__ int32_t part[kOffWaves][kWave];
-  const int tid = threadIdx.x;
-  const int lane = tid & (kWave - 1);
-  const int w = tid / kWave;
-  const int64_t n = (int64_t)blockIdx.x * kWave + lane;
-  const bool ok = n < p.nb;
-  int32_t c[SPW];
-#pragma unroll
-  for (int k = 0; k < SPW; ++k) c[k] = ok ? p.seg_tot[(int64_t)(w * SPW + k) * p.nb + n] : 0;
-  int32_t t = 0;
-#pragma unroll
-  for (int k = 0; k < SPW; ++k) t += c[k];
-  part[w][lane] = t;
-  __syncthreads();
-  int32_t run = 0;
-  for (int ww = 0; ww < w; ++ww) run += part[ww][lane];
-  if (ok) {
-#pragma unroll
-    for (int k = 0; k < SPW; ++k) {
-      run += c[k];
-      p.seg_end[(int64_t)(w * SPW + k) * p.nb + n] = run;
-      p.seg_tot[(int64_t)(w * SPW + k) * p.nb + n] = 0;
-    }
-    if (w == kOffWaves - 1) p.row_tot[n] = run;
-  }
-}
-
-__global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   __shared__ int32_t sh[1024];
   const int tid = threadIdx.x;
+  if (blockIdx.x + 1 < gridDim.x) {
+    constexpr int SPW = kESeg / kOffWaves;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    const int64_t n = (int64_t)blockIdx.x * kWave + lane;
+    const bool ok = n < p.nb;
+    int32_t c[SPW];
+#pragma unroll
+    for (int k = 0; k < SPW; ++k) c[k] = ok ? p.seg_tot[(int64_t)(w * SPW + k) * p.nb + n] : 0;
+    int32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < SPW; ++k) t += c[k];
+    part[w][lane] = t;
+    __syncthreads();
+    int32_t run = 0;
+    for (int ww = 0; ww < w; ++ww) run += part[ww][lane];
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < SPW; ++k) {
+        run += c[k];
+        p.seg_end[(int64_t)(w * SPW + k) * p.nb + n] = run;
+        p.seg_tot[(int64_t)(w * SPW + k) * p.nb + n] = 0;
+      }
+    }
+    return;
+  }
+  // last block: exclusive scan of the row totals
   const int64_t per = ceil_div(p.nb, 1024);
   const int64_t r0 = tid * per;
   const int64_t r1 = (r0 + per) < p.nb ? (r0 + per) : p.nb;
@@ -101,6 +100,7 @@ __global__ void __launch_bounds__(1024) estep_rowptr_kernel(EParams p) {
   for (int64_t r = r0; r < r1; ++r) {
     p.rowptr[r] = pos;
     pos += p.row_tot[r];
+    p.row_tot[r] = 0;
   }
   if (tid == 1023) {
     p.rowptr[p.nb] = sh[1023];
@@ -356,7 +356,7 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->old_colsum = a.old_colsum;
   p->c = a.c;
   char *ws = (char *)a.workspace;
-  // The counters that must read zero on entry (seg_tot) sit at the
+  // The counters that must read zero on entry (seg_tot, row_tot) sit at the
   // END of the workspace, the scratch arrays at its start.  Blocks of
   // different sizes share one workspace (the last block of a fit is
   // smaller); with the counters anchored at the end, a launch with nb
@@ -364,12 +364,13 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   // scratch lies inside the larger block's scratch, so the counters are never
   // overwritten and every launch finds (and leaves) them zero.
   const size_t segb = align256((size_t)kESeg * a.nb * 4);
+  const size_t rowb = align256((size_t)a.nb * 4);
   const size_t maskb = align256((size_t)a.nb * p->nchunks * 8);
   char *end = ws ? (char *)(((uintptr_t)ws + a.workspace_bytes) & ~(uintptr_t)255) : ws + 512;
-  p->seg_tot = (int32_t *)(end - 256 - segb);
+  p->seg_tot = (int32_t *)(end - 256 - segb - rowb);
+  p->row_tot = (int32_t *)(end - 256 - rowb);
   p->masks = (uint64_t *)ws;
   p->seg_end = (int32_t *)(ws + maskb);
-  p->row_tot = (int32_t *)(ws + maskb + segb);
   p->seg_len = ceil_div(p->nchunks, kESeg);
   p->rowptr = a.rowptr;
   p->out_row = a.out_row;
@@ -382,6 +383,7 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->flag_words = nullptr;
   p->status_flag = p->unsafe_flag = nullptr;
   p->flag_seq = 0;
+  p->lazy_Rnew = p->lazy_Unew = nullptr;
   if (a.mode == LAKDE_ESTEP_UPDATE && a.lazy_unsafe != nullptr) {
     LAKDE_CHECK_ARG(a.lazy_G != nullptr, "lazy_unsafe needs lazy_G");
     p->lazy_G = a.lazy_G;
@@ -432,6 +434,25 @@ int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int3
   return LAKDE_OK;
 }
 
+// the compaction of a lazy-rebalance step, storing u = r / (G coeff) and
+// accumulating the column sums of r and u unless the step is unsafe
+// (lakde_rebalance_lazy's first half, fused)
+int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsafe, double *Rnew,
+                       double *Unew, hipStream_t s) {
+  LAKDE_CHECK_ARG(a.mode == LAKDE_ESTEP_UPDATE && a.c_old && G && unsafe && Rnew && Unew,
+                  "bad lazy compaction args");
+  EParams p;
+  int rc = estep_fill_params(a, &p);
+  if (rc) return rc;
+  p.lazy_G = G;
+  p.lazy_unsafe = unsafe;
+  p.lazy_Rnew = Rnew;
+  p.lazy_Unew = Unew;
+  rc = dispatch_estep_compact(a.D, p, s);
+  if (rc) set_error("estep_compact_lazy: launch failed");
+  return rc;
+}
+
 // the per-segment offsets, row pointer and entry count of an E-step (the
 // first half of lakde_estep_finish; *nnz_dev is final afterwards)
 int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *flag_words,
@@ -443,10 +464,8 @@ int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *fl
   p.status_flag = status_flag;
   p.unsafe_flag = unsafe_flag;
   p.flag_seq = flag_seq;
-  hipLaunchKernelGGL(estep_seg_end_kernel, dim3((unsigned)ceil_div(p.nb, kWave)), dim3(1024), 0,
-                     s, p);
-  LAKDE_LAUNCHED();
-  hipLaunchKernelGGL(estep_rowptr_kernel, dim3(1), dim3(1024), 0, s, p);
+  hipLaunchKernelGGL(estep_offsets_kernel, dim3((unsigned)ceil_div(p.nb, kWave) + 1u), dim3(1024),
+                     0, s, p);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }
@@ -459,8 +478,8 @@ extern "C" {
 
 size_t lakde_estep_workspace(int64_t N, int64_t nb) {
   const int64_t nchunks = ceil_div(N, kWave);
-  // scratch (masks, segment ends, row totals), the segment counts, 256
-  // bytes at the end and 256 bytes of alignment slack
+  // scratch (masks, segment ends), the counters (segment counts, row
+  // totals), 256 bytes at the end and 256 bytes of alignment slack
   return align256((size_t)nb * nchunks * 8) + 2 * align256((size_t)kESeg * nb * 4) +
          align256((size_t)nb * 4) + 256 + 256;
 }

@@ -21,9 +21,9 @@ struct EParams {
   float *c;
   uint64_t *masks;   // nchunks x nb (chunk-major: a wave's 64 kernels are contiguous)
   int32_t *seg_tot;  // kESeg x nb: kept entries of kernel n in chunk segment s
-                     // (accumulated by the main kernel, cleared by estep_seg_end_kernel)
-  int32_t *seg_end;  // kESeg x nb: kept entries of kernel n in segments 0..s
   int32_t *row_tot;  // nb: kept entries of kernel n
+                     // (both accumulated by the main kernel, cleared by estep_offsets_kernel)
+  int32_t *seg_end;  // kESeg x nb: kept entries of kernel n in segments 0..s
   int64_t seg_len;   // chunks per segment
   int32_t *rowptr;   // nb + 1
   int32_t *out_row, *out_col;
@@ -32,6 +32,10 @@ struct EParams {
   int32_t *nnz_dev;
   const float *lazy_G;   // optional unsafe-step test of the lazy rebalance
   int32_t *lazy_unsafe;
+  // optional (compaction of a lazy-rebalance step): unless the step is
+  // unsafe, store u = r / (G coeff) instead of r and accumulate the column
+  // sums of r and u (the work of lazy_block_kernel)
+  double *lazy_Rnew, *lazy_Unew;
   // optional (step driver): the row-pointer kernel also writes the entry
   // count and the status / unsafe flags into host-coherent memory as 64-bit
   // words (seq << 32 | value), each valid on its own when its tag is seq
@@ -68,10 +72,15 @@ struct PParams {
 constexpr int kEThreads = 256;  // 4 waves; lane = column, wave = kernel quarter
 constexpr int kESeg = 128;      // chunk segments of the row scan and the compaction
 
-// a written mask's kept entries, added to its (segment, kernel) count
+// a written mask's kept entries, added to its (segment, kernel) count and
+// to the kernel's row total
 __device__ __forceinline__ void count_mask(const EParams &p, int64_t seg, int64_t n,
                                            uint64_t mask) {
-  if (mask != 0ull) atomicAdd(&p.seg_tot[seg * p.nb + n], (int32_t)__popcll(mask));
+  if (mask != 0ull) {
+    const int32_t c = (int32_t)__popcll(mask);
+    atomicAdd(&p.seg_tot[seg * p.nb + n], c);
+    atomicAdd(&p.row_tot[n], c);
+  }
 }
 
 // ---------------------------------------------------------------- E-step
@@ -270,6 +279,7 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
   constexpr int64_t R = rec_floats(D);
   const float *rec = p.rec + n * R;
   int64_t pos = (int64_t)p.rowptr[n] + s_lo;
+  const bool to_stored = p.lazy_Rnew != nullptr && *p.lazy_unsafe == 0;
   // kept columns are gathered KB at a time so their point loads are in
   // flight together (the chain per lane is otherwise one load per entry)
   constexpr int KB = D <= 8 ? 4 : (D <= 24 ? 2 : 1);
@@ -285,13 +295,21 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
     for (int i = 0; i < KB; ++i) {
       if (i < k) {
         const float x = log_rho<D>(rec, xv[i]);
-        const float d = __fsub_rn(x, p.c[ms[i]]);
+        const float cm = p.c[ms[i]];
+        const float d = __fsub_rn(x, cm);
         float r = 0.f;
         keep_resp(d, p.threshold, p.log_t_lo, &r);
+        float v = r;
+        if (to_stored) {
+          const float coeff = clamp_nan(expf(p.c_old[ms[i]] - cm), FLT_MIN, FLT_MAX);
+          v = r / (p.lazy_G[ms[i]] * coeff);
+          atomicAdd(&p.lazy_Rnew[ms[i]], (double)r);
+          atomicAdd(&p.lazy_Unew[ms[i]], (double)v);
+        }
         if (pos >= 0 && pos < p.cap) {  // never write outside the arena slots
           p.out_row[pos] = (int32_t)n;
           p.out_col[pos] = (int32_t)ms[i];
-          p.out_val[pos] = r;
+          p.out_val[pos] = v;
         }
         ++pos;
       }
@@ -480,6 +498,8 @@ struct LazyGuard {
 };
 
 // shared host helpers (fused.hip)
+int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsafe, double *Rnew,
+                       double *Unew, hipStream_t s);
 int estep_count(const lakde_estep_args &a, hipStream_t s,
                 unsigned long long *flag_words = nullptr, const int32_t *status_flag = nullptr,
                 const int32_t *unsafe_flag = nullptr, uint32_t flag_seq = 0);

@@ -17,7 +17,8 @@
 //     lazy unsafe-step test                       (+ lakde_rebalance_lazy_check, fused)
 //   segment offsets, row pointer, entry count   first half of lakde_estep_finish
 //   -- copy of the arena counts and flags --
-//   compaction, guarded lazy rebalance          lakde_estep_compact, lakde_rebalance_lazy
+//   compaction (lazy: storing u, with the       lakde_estep_compact (+ the first half of
+//     block's column sums), guarded lazy norm     lakde_rebalance_lazy, fused), lazy_norm
 //   -- wait for the copy; host-side outcome: exact fallback / lakde_rebalance --
 #include "fused_impl.hpp"
 
@@ -83,8 +84,10 @@ int step_rebalance(const lakde_step_args *a, hipStream_t s) {
   }
   const int32_t unsafe = a->host_flags[a->num_blocks + 1];
   if (!unsafe) {
-    return lakde_rebalance_lazy(N, e.out_col, e.out_val, e.nnz_dev, e.c_old, e.c, a->G, a->U,
-                                a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws, s);
+    // the compaction stored u and accumulated the block's sums: normalise
+    return rebalance_lazy_guarded(N, nullptr, nullptr, nullptr, e.c_old, e.c, a->G, a->U,
+                                  a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws,
+                                  LazyGuard{nullptr, 0, nullptr}, s);
   }
   // the stored form cannot represent this step: fold, rebalance the true
   // values exactly, recompute the totals, clear the per-step sums
@@ -186,12 +189,18 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
   // rebalance outcome run while the host reads the flags and prepares the
   // next step; the lazy kernels skip themselves for an unsafe or
   // overflowing step
-  rc = lakde_estep_compact(&e, s);
-  if (rc) return rc;
   if (a->G != nullptr) {
-    rc = rebalance_lazy_guarded(N, e.out_col, e.out_val, e.nnz_dev, e.c_old, e.c, a->G, a->U,
+    // the compaction also stores u = r / (G coeff) and the block's column
+    // sums unless the step is unsafe; the normalisation skips itself for an
+    // unsafe or overflowing step
+    rc = estep_compact_lazy(e, a->G, unsafe, a->lazy_ws, a->lazy_ws + N, s);
+    if (rc) return rc;
+    rc = rebalance_lazy_guarded(N, nullptr, nullptr, nullptr, e.c_old, e.c, a->G, a->U,
                                 a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws,
                                 LazyGuard{e.nnz_dev, a->cap, unsafe}, s);
+    if (rc) return rc;
+  } else {
+    rc = lakde_estep_compact(&e, s);
     if (rc) return rc;
   }
   if (a->flag_words != nullptr) {
@@ -241,7 +250,10 @@ int lakde_partial_step_resume(const lakde_step_args *a, lakde_stream_t stream) {
                       a->estep.cap == a->cap,
                   "the block's entries must be its arena slots");
   hipStream_t s = as_stream(stream);
-  int rc = lakde_estep_compact(&a->estep, s);
+  int rc = a->G != nullptr
+               ? estep_compact_lazy(a->estep, a->G, a->nnz_all + a->num_blocks + 1, a->lazy_ws,
+                                    a->lazy_ws + a->estep.N, s)
+               : lakde_estep_compact(&a->estep, s);
   if (rc) return rc;
   return step_rebalance(a, s);
 }
