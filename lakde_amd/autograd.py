@@ -102,15 +102,20 @@ class FactorInverse(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gLinv):
         (Linv,) = ctx.saved_tensors
-        Li = Linv.double()
-        D = Li.size(-1)
-        eye = torch.eye(D, dtype=torch.float64, device=Li.device).expand_as(Li)
-        L = torch.linalg.solve_triangular(Li, eye, upper=False)
-        gL = -torch.tril(Li.mT @ gLinv.double() @ Li.mT)
-        P = torch.tril(L.mT @ gL)
-        P = P - 0.5 * torch.diag_embed(torch.diagonal(P, dim1=-2, dim2=-1))
-        S = Li.mT @ P @ Li
-        return 0.5 * (S + S.mT), None
+        return _factor_inverse_vjp(Linv.double(), gLinv.double())[0], None
+
+
+def _factor_inverse_vjp(Li, gLi):
+    """(gA, L) for Linv = chol(A)^{-1}: the float64 VJP of FactorInverse and
+    the Cholesky factor L = Linv^{-1} it computes on the way."""
+    D = Li.size(-1)
+    eye = torch.eye(D, dtype=torch.float64, device=Li.device).expand_as(Li)
+    L = torch.linalg.solve_triangular(Li, eye, upper=False)
+    gL = -torch.tril(Li.mT @ gLi @ Li.mT)
+    P = torch.tril(L.mT @ gL)
+    P = P - 0.5 * torch.diag_embed(torch.diagonal(P, dim1=-2, dim2=-1))
+    S = Li.mT @ P @ Li
+    return 0.5 * (S + S.mT), L
 
 
 def _scatter(model, X, b):
@@ -161,6 +166,66 @@ def log_pred_density(model, X, Y, prior_factor=None):
     return mixture_log_density(Y, Linv, X, nu)
 
 
+def prior_scale_objective(model, X, Y, log_scale=0.0, grad=True):
+    """Held-out objective of a fitted LocalKNearestKDE as a function of a
+    multiplicative scale s = exp(log_scale) of its kNN prior covariances,
+    J(s) = mean_m log p(y_m | s): the posteriors are those of
+    lakde/local_adaptive_kde.py:83-96 with W_inv_n = s nu0 E[tau_n]
+    Sigma0j_n + S_n (responsibilities, nu and the persistent jitters held at
+    the fitted values).  Returns (J, dJ/dlog s) (the derivative None when
+    grad is False).
+
+    The gradient is the HIP backward of the predictive density (w.r.t. the
+    factors L_n^{-1}), chained through each posterior factorisation with the
+    float64 Cholesky VJP and contracted with the prior term of W_inv, block by
+    block, so no N x D x D float64 tensor is ever formed.  The model's b,
+    jitters and records are restored afterwards."""
+    require_gpu(X, Y)
+    if not hasattr(model, "sigma_0_jitter"):
+        raise TypeError("prior_scale_objective needs a LocalKNearestKDE")
+    s = float(torch.exp(torch.tensor(float(log_scale), dtype=torch.float64)))
+    saved = (model.b.clone(), model.W_inv_jitter.clone(), model.sigma_0_jitter.clone())
+    try:
+        # s multiplies the prior term nu0 a/b Sigma0j: the posterior kernels
+        # see it as b / s (b_out is not written by this recompute)
+        model.b.div_(s)
+        Linv, nu = posterior_factors(model, X)
+        jit = model.W_inv_jitter.clone()
+        if not grad:
+            with torch.no_grad():
+                return float(mixture_log_density(Y, Linv, X, nu).mean()), None
+        Linv.requires_grad_(True)
+        J = mixture_log_density(Y, Linv, X, nu).mean()
+        J.backward()
+        gLinv = Linv.grad
+        N, D = X.shape
+        g = torch.zeros((), dtype=torch.float64, device=X.device)
+        for b in model._blocks(N):
+            st, e, nb = model._span(b, N)
+            S_b, _ = _scatter(model, X, b)
+            gA, L = _factor_inverse_vjp(Linv[st:e].detach().double(), gLinv[st:e].double())
+            W_j = L @ L.mT                                   # W_inv + jitter I
+            prior_term = W_j - S_b.double() - torch.diag_embed(
+                jit[st:e].double()[:, None].expand(nb, D))   # s nu0 E[tau] Sigma0j
+            g += (gA * prior_term).sum()
+        return float(J.detach()), float(g)
+    finally:
+        model.b.copy_(saved[0])
+        model.W_inv_jitter.copy_(saved[1])
+        model.sigma_0_jitter.copy_(saved[2])
+        _recompute_records(model, X)  # records of the fitted model again
+
+
+def _recompute_records(model, X):
+    N = X.size(0)
+    for b in model._blocks(N):
+        s, e, nb = model._span(b, N)
+        model._block_posterior(X, b, model._nu_s[:nb], model._logdet_s[:nb], model._trace_s[:nb],
+                               model._logdet_prior_s[:nb] if hasattr(model, "sigma_0_jitter")
+                               else None, update_b=False)
+    model._sync(None)
+
+
 def posterior_factors(model, X):
     """(Linv (N x D x D), nu (N)) of every kernel from the HIP posterior kernel
     (the persistent jitters are updated as by log_pred_density)."""
@@ -179,4 +244,4 @@ def posterior_factors(model, X):
 
 
 __all__ = ["MixtureLogDensity", "FactorInverse", "mixture_log_density", "log_pred_density",
-           "posterior_factors"]
+           "posterior_factors", "prior_scale_objective"]
