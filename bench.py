@@ -263,10 +263,12 @@ def precision_form(seconds_budget=None):
         print(f"# precision form {prec}: {dt * 1e3:.1f} ms/partial step, E-step main {k_ms:.2f} ms",
               file=sys.stderr, flush=True)
     # executed matrix-core work of the bf16 E-step main kernel: one pass of
-    # 3 (bf16x3) x 2 x Dp x Kp flop per pair (the UPDATE step tests its
-    # queued candidates without a second pass)
-    dp, kp = 64, 64
-    mfma_flop = 3 * 2 * dp * kp * 512 * N
+    # 3 (bf16x3) x 2 x 32 x 16 flop per pair for every 32-row x 16-column
+    # tile of the padded Linv that is not all zeros (the UPDATE step tests
+    # its queued candidates without a second pass)
+    rt_n, ks_n = (D + 31) // 32, (D + 15) // 16
+    tiles = sum(1 for rt in range(rt_n) for s_ in range(ks_n) if s_ * 16 <= rt * 32 + 31)
+    mfma_flop = 3 * 2 * 32 * 16 * tiles * 512 * N
     res["bf16"]["mfma_executed_tflops"] = round(mfma_flop / (res["bf16"]["estep_main_ms"] * 1e-3) / 1e12, 1)
     res["bf16"]["mfma_frac_of_2500_tflops"] = round(res["bf16"]["mfma_executed_tflops"] / 2500.0, 4)
     res["speedup_bf16_over_fp32"] = round(res["fp32"]["partial_step_ms"] / res["bf16"]["partial_step_ms"], 2)

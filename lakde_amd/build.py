@@ -58,9 +58,14 @@ def _gen_inst(lo, hi):
     return path
 
 
+# per-source flags: the matrix-core kernels keep their MFMA accumulators in
+# ordinary VGPRs (the epilogue's squares read them without v_accvgpr_read)
+EXTRA_FLAGS = {"bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + CFLAGS + EXTRA_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     t = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

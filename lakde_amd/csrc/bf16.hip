@@ -21,7 +21,20 @@
 namespace lakde {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// bf16 hi/lo split of two floats (hi = bf16(d), lo = bf16(d - hi), both
+// round-to-nearest-even), packed two per 32-bit word: the hi pair is
+// converted once and widened back with a shift and a mask
+__device__ __forceinline__ void split2(float d0, float d1, uint32_t &hi, uint32_t &lo) {
+  const uint32_t hb = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){d0, d1}, bf16x2));
+  const float h0 = __uint_as_float(hb << 16), h1 = __uint_as_float(hb & 0xffff0000u);
+  hi = hb;
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){d0 - h0, d1 - h1}, bf16x2));
+}
 
 __host__ __device__ constexpr int bf_kp(int D) { return (D + 15) / 16 * 16; }
 __host__ __device__ constexpr int bf_dp(int D) { return (D + 31) / 32 * 32; }
@@ -93,14 +106,18 @@ struct BfTile {
 #pragma unroll
       for (int e = 0; e < 8; ++e) c[e] = xn[s * 16 + 8 * (lane >> 5) + e];
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct)
+      for (int ct = 0; ct < 2; ++ct) {
+        u32x4 h, l;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const float d = xf[ct][s][e] - c[e];
-          const __bf16 h = (__bf16)d;
-          dh[ct][s][e] = h;
-          dl[ct][s][e] = (__bf16)(d - (float)h);
+        for (int e = 0; e < 4; ++e) {
+          uint32_t hh, ll;
+          split2(xf[ct][s][2 * e] - c[2 * e], xf[ct][s][2 * e + 1] - c[2 * e + 1], hh, ll);
+          h[e] = hh;
+          l[e] = ll;
         }
+        dh[ct][s] = __builtin_bit_cast(bf16x8, h);
+        dl[ct][s] = __builtin_bit_cast(bf16x8, l);
+      }
     }
     float q0 = 0.f, q1 = 0.f;
 #pragma unroll
@@ -108,6 +125,9 @@ struct BfTile {
       f32x16 acc0 = {}, acc1 = {};
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
+        // Linv is lower triangular: a K-step whose first column lies right
+        // of the tile's last row multiplies only zeros
+        if (s * 16 > rt * 32 + 31) continue;
         const int off = (rt * 32 + (lane & 31)) * Kp + s * 16 + 8 * (lane >> 5);
         const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(hi + off);
         const bf16x8 al = *reinterpret_cast<const bf16x8 *>(lo + off);
@@ -131,13 +151,29 @@ struct BfTile {
   }
 };
 
-// cooperative copy of one fragment block (FB bytes, 16-byte granules) into LDS
+// Cooperative copy of one fragment block (FB bytes, 16-byte granules, 256
+// threads) into LDS, split into its global loads, all issued together into
+// registers, and its LDS stores.  The loops load the next kernel's block
+// before the current kernel's matrix-core work and store it after, so the
+// load latency is hidden behind the MFMAs instead of being waited for
+// granule by granule.
 template <int FB>
-__device__ __forceinline__ void stage_frag(const uint8_t *__restrict__ src, uint8_t *dst) {
-  const int4 *s = reinterpret_cast<const int4 *>(src);
-  int4 *d = reinterpret_cast<int4 *>(dst);
-  for (int t = threadIdx.x; t < FB / 16; t += blockDim.x) d[t] = s[t];
-}
+struct FragStage {
+  static constexpr int kG = FB / 16, kFull = kG / 256, kTail = kG % 256;
+  u32x4 r[kFull + 1];
+  __device__ __forceinline__ void load(const uint8_t *__restrict__ src) {
+    const u32x4 *s = reinterpret_cast<const u32x4 *>(src) + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < kFull; ++i) r[i] = s[i * 256];
+    if (kTail != 0 && threadIdx.x < kTail) r[kFull] = s[kFull * 256];
+  }
+  __device__ __forceinline__ void store(uint8_t *dst) const {
+    u32x4 *d = reinterpret_cast<u32x4 *>(dst) + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < kFull; ++i) d[i * 256] = r[i];
+    if (kTail != 0 && threadIdx.x < kTail) d[kFull * 256] = r[kFull];
+  }
+};
 
 template <int RT, int KS, int MODE>
 __global__ void __launch_bounds__(256)
@@ -174,10 +210,14 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
     }
     Lse l;
     l.init();
-    stage_frag<T::FB>(frag, lds[0]);
+    FragStage<T::FB> st;
+    st.load(frag);
+    st.store(lds[0]);
     __syncthreads();
     for (int64_t n = 0; n < nb; ++n) {
-      if (n + 1 < nb) stage_frag<T::FB>(frag + (n + 1) * T::FB, lds[(n + 1) & 1]);
+      // the next block (the last one again on the last kernel) goes into the
+      // other buffer, last read in iteration n - 1, which ended at a barrier
+      st.load(frag + (n + 1 < nb ? n + 1 : n) * T::FB);
       const float q = tile.quad(lds[n & 1], lane);
       const float *rc = rec + n * R + cofs;
       float x = __fadd_rn(__fmul_rn(q, rc[1]), rc[0]);
@@ -190,6 +230,7 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
         }
         ++cnt;
       }
+      st.store(lds[(n + 1) & 1]);
       __syncthreads();
     }
     const float lse = l.value();
@@ -232,10 +273,12 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
     return;
   }
 
-  stage_frag<T::FB>(frag, lds[0]);
+  FragStage<T::FB> st;
+  st.load(frag);
+  st.store(lds[0]);
   __syncthreads();
   for (int64_t n = 0; n < nb; ++n) {
-    if (n + 1 < nb) stage_frag<T::FB>(frag + (n + 1) * T::FB, lds[(n + 1) & 1]);
+    st.load(frag + (n + 1 < nb ? n + 1 : n) * T::FB);
     const float q = tile.quad(lds[n & 1], lane);
     const float *rc = rec + n * R + cofs;
     float x = __fadd_rn(__fmul_rn(q, rc[1]), rc[0]);
@@ -248,6 +291,7 @@ estep_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ X,
       p.masks[chunk * p.nb + n] = mask;
       count_mask(p, cseg, n, mask);
     }
+    st.store(lds[(n + 1) & 1]);
     __syncthreads();
   }
 }
@@ -270,14 +314,17 @@ pred_bf16_kernel(const uint8_t *__restrict__ frag, const float *__restrict__ Y,
   const int cofs = (int)rec_c(Dreal);
   Lse l;
   l.init();
-  stage_frag<T::FB>(frag + sbeg * T::FB, lds[0]);
+  FragStage<T::FB> st;
+  st.load(frag + sbeg * T::FB);
+  st.store(lds[0]);
   __syncthreads();
   for (int64_t n = sbeg; n < send; ++n) {
     const int64_t k = n - sbeg;
-    if (n + 1 < send) stage_frag<T::FB>(frag + (n + 1) * T::FB, lds[(k + 1) & 1]);
+    st.load(frag + (n + 1 < send ? n + 1 : n) * T::FB);
     const float q = tile.quad(lds[k & 1], lane);
     const float *rc = rec + n * R + cofs;
     l.add1(__fsub_rn(rc[2], __fmul_rn(rc[3], log1pf(q))));
+    st.store(lds[(k + 1) & 1]);
     __syncthreads();
   }
   if (m < p.M) {
