@@ -94,54 +94,73 @@ struct BfTile {
     }
   }
 
-  // Q for this lane's column from one kernel's fragment block in LDS
+  // hi/lo split of this lane's differences x_m - x_n for K-step s, both
+  // column tiles
+  __device__ __forceinline__ void split_step(const float *xn, int s, int lane, bf16x8 (&dh)[2],
+                                             bf16x8 (&dl)[2]) const {
+    float c[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) c[e] = xn[s * 16 + 8 * (lane >> 5) + e];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      u32x4 h, l;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        uint32_t hh, ll;
+        split2(xf[ct][s][2 * e] - c[2 * e], xf[ct][s][2 * e + 1] - c[2 * e + 1], hh, ll);
+        h[e] = hh;
+        l[e] = ll;
+      }
+      dh[ct] = __builtin_bit_cast(bf16x8, h);
+      dl[ct] = __builtin_bit_cast(bf16x8, l);
+    }
+  }
+
+  // Q for this lane's column from one kernel's fragment block in LDS.
+  // K-step by K-step: the split of step s + 1 and the epilogue of a row
+  // tile whose last nonzero step is done are independent of step s's
+  // MFMAs, so their vector instructions issue while the matrix core works.
+  // Every accumulator sees the same MFMA sequence in the same order
+  // (per step: hi*hi, hi*lo, lo*hi), and q sums row tiles in order.
   __device__ __forceinline__ float quad(const uint8_t *fb, int lane) const {
     const __bf16 *hi = reinterpret_cast<const __bf16 *>(fb);
     const __bf16 *lo = hi + Dp * Kp;
     const float *xn = reinterpret_cast<const float *>(lo + Dp * Kp);
-    bf16x8 dh[2][KS], dl[2][KS];
+    bf16x8 dh[KS][2], dl[KS][2];
+    f32x16 acc[RT][2];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) acc[rt][0] = acc[rt][1] = (f32x16){};
+    float q0 = 0.f, q1 = 0.f;
+    split_step(xn, 0, lane, dh[0], dl[0]);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      float c[8];
+      if (s + 1 < KS) split_step(xn, s + 1, lane, dh[s + 1], dl[s + 1]);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) c[e] = xn[s * 16 + 8 * (lane >> 5) + e];
-#pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        u32x4 h, l;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          uint32_t hh, ll;
-          split2(xf[ct][s][2 * e] - c[2 * e], xf[ct][s][2 * e + 1] - c[2 * e + 1], hh, ll);
-          h[e] = hh;
-          l[e] = ll;
-        }
-        dh[ct][s] = __builtin_bit_cast(bf16x8, h);
-        dl[ct][s] = __builtin_bit_cast(bf16x8, l);
-      }
-    }
-    float q0 = 0.f, q1 = 0.f;
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-      f32x16 acc0 = {}, acc1 = {};
-#pragma unroll
-      for (int s = 0; s < KS; ++s) {
+      for (int rt = 0; rt < RT; ++rt) {
         // Linv is lower triangular: a K-step whose first column lies right
         // of the tile's last row multiplies only zeros
         if (s * 16 > rt * 32 + 31) continue;
         const int off = (rt * 32 + (lane & 31)) * Kp + s * 16 + 8 * (lane >> 5);
         const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(hi + off);
         const bf16x8 al = *reinterpret_cast<const bf16x8 *>(lo + off);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dh[0][s], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dh[1][s], acc1, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dl[0][s], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dl[1][s], acc1, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, dh[0][s], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, dh[1][s], acc1, 0, 0, 0);
+        acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dh[s][0], acc[rt][0], 0, 0, 0);
+        acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dh[s][1], acc[rt][1], 0, 0, 0);
+        acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dl[s][0], acc[rt][0], 0, 0, 0);
+        acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, dl[s][1], acc[rt][1], 0, 0, 0);
+        acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, dh[s][0], acc[rt][0], 0, 0, 0);
+        acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, dh[s][1], acc[rt][1], 0, 0, 0);
       }
+      // row tile rt is complete after step min(KS - 1, 2 rt + 1); tiles
+      // complete in increasing rt, so q sums them in order
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        q0 = fmaf(acc0[r], acc0[r], q0);
-        q1 = fmaf(acc1[r], acc1[r], q1);
+      for (int rt = 0; rt < RT; ++rt) {
+        const int last = (2 * rt + 1 < KS - 1) ? 2 * rt + 1 : KS - 1;
+        if (s != last) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          q0 = fmaf(acc[rt][0][r], acc[rt][0][r], q0);
+          q1 = fmaf(acc[rt][1][r], acc[rt][1][r], q1);
+        }
       }
     }
     const bool upper = lane >= 32;
