@@ -366,6 +366,19 @@ int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, i
                        float *out, void *workspace, size_t workspace_bytes,
                        lakde_stream_t stream);
 
+/* k-nearest-neighbour graph for initialisation (lakde/utils.py:77-116,
+ * _compute_knn_graph_generic with metric "l2"; called from
+ * lakde/local_knn_kde.py:113-127 and lakde/local_hierarchical_kde.py at
+ * init).  X: N x D float32 row-major.  out: N x k (LAKDE_I32 or LAKDE_I64)
+ * neighbour indices, each row sorted by squared distance (FP32, summed in
+ * coordinate order), equal distances by index; n == m is excluded unless
+ * include_self.  dist (optional, N x k): the Euclidean distances.
+ * Workspace: lakde_knn_graph_workspace() bytes, no initial contents. */
+size_t lakde_knn_graph_workspace(int64_t N, int64_t k);
+int lakde_knn_graph(const float *X, int64_t N, int64_t D, int64_t k, int include_self, void *out,
+                    int out_dtype, float *dist, void *workspace, size_t workspace_bytes,
+                    lakde_stream_t stream);
+
 /* ------------------------------------------------------------------ *
  * Precision form (8 <= D <= 63): BF16 matrix cores, split precision    *
  * ------------------------------------------------------------------ */

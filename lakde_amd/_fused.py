@@ -231,6 +231,26 @@ def pred_density(rec, Y, out):
     return out
 
 
+def knn_graph(X, k, include_self=False, dtype=torch.int32, return_dists=False):
+    """k nearest neighbours of every row of X (N x D float32 on the device),
+    rows sorted by distance with equal distances by index
+    (lakde/utils.py:77-116)."""
+    X = X.contiguous()
+    if X.dtype != torch.float32:
+        raise TypeError("knn_graph: X must be float32")
+    N, D = X.shape
+    out = torch.empty(N, k, dtype=dtype, device=X.device)
+    dist = torch.empty(N, k, dtype=torch.float32, device=X.device) if return_dists else None
+    lib = _lib.load()
+    nbytes = int(lib.lakde_knn_graph_workspace(N, k))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=X.device)
+    check(lib.lakde_knn_graph(ptr(X), N, D, k, int(bool(include_self)), ptr(out), idx_code(dtype),
+                              ptr(dist) if dist is not None else None, ptr(ws), nbytes,
+                              stream_of(X)),
+          "knn_graph")
+    return (dist, out) if return_dists else out
+
+
 def bf16_frag_bytes(D):
     return int(_lib.load().lakde_bf16_frag_bytes(D))
 

@@ -95,6 +95,8 @@ _SIGNATURES = {
     "lakde_pred_workspace": (_sz, [_i64, _i64]),
     "lakde_pred_density": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "lakde_fill_f32": (_int, [_vp, _i64, _f32, _vp]),
+    "lakde_knn_graph_workspace": (_sz, [_i64, _i64]),
+    "lakde_knn_graph": (_int, [_vp, _i64, _i64, _i64, _int, _vp, _int, _vp, _vp, _sz, _vp]),
     "lakde_bf16_frag_bytes": (_i64, [_i64]),
     "lakde_pack_frag_bf16": (_int, [_vp, _i64, _i64, _vp, _vp]),
     "lakde_estep_main_bf16": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp]),

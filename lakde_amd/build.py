@@ -34,7 +34,7 @@ CFLAGS = [
 ]
 
 SOURCES = ["ops_ref.hip", "posterior.hip", "post_small.hip", "fused.hip", "bf16.hip",
-           "backward.hip", "step.hip"]
+           "backward.hip", "step.hip", "knn.hip"]
 D_GROUPS = [(1, 6), (7, 12), (13, 18), (19, 24), (25, 32), (33, 40), (41, 48), (49, 56),
             (57, 63)]
 

@@ -19,14 +19,10 @@ def cov(X, dtype=torch.float64):
 
 
 def compute_knn_graph(X, k, bsize=1024, include_self=False, dtype=torch.int32):
-    """k nearest neighbours by L2 distance, blocked over rows
-    (lakde/utils.py:77-116, the generic path).  Used only at initialisation
-    when the model is given an integer k."""
-    N = X.size(0)
-    out = torch.empty(N, k, dtype=dtype, device=X.device)
-    for i in range(0, N, bsize):
-        d = torch.cdist(X[i:i + bsize], X)
-        if not include_self:
-            torch.diagonal(d, i).fill_(float("inf"))
-        out[i:i + bsize] = d.topk(k, dim=1, largest=False, sorted=True)[1].to(dtype)
-    return out
+    """k nearest neighbours by L2 distance (lakde/utils.py:77-116, the
+    generic path), on the HIP kernel: rows sorted by distance, equal
+    distances by index.  Used only at initialisation when the model is given
+    an integer k.  ``bsize`` is accepted for signature compatibility; the
+    kernel tiles by itself."""
+    from . import _fused
+    return _fused.knn_graph(X.to(torch.float32), int(k), include_self=include_self, dtype=dtype)
