@@ -367,9 +367,9 @@ int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, i
                        lakde_stream_t stream);
 
 /* k-nearest-neighbour graph for initialisation (lakde/utils.py:77-116,
- * _compute_knn_graph_generic with metric "l2"; called from
- * lakde/local_knn_kde.py:113-127 and lakde/local_hierarchical_kde.py at
- * init).  X: N x D float32 row-major.  out: N x k (LAKDE_I32 or LAKDE_I64)
+ * _compute_knn_graph_generic with metric "l2"; called at init from
+ * lakde/local_knn_kde.py:95 and lakde/local_hierarchical_kde.py:149).
+ * X: N x D float32 row-major.  out: N x k (LAKDE_I32 or LAKDE_I64)
  * neighbour indices, each row sorted by squared distance (FP32, summed in
  * coordinate order), equal distances by index; n == m is excluded unless
  * include_self.  dist (optional, N x k): the Euclidean distances.

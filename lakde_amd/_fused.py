@@ -74,6 +74,7 @@ class FlagWords:
 
     def __init__(self):
         self._lib = _lib.load()
+        self._vp = ctypes.c_void_p  # kept: module globals may be gone at interpreter exit
         p = ctypes.c_void_p()
         check(self._lib.lakde_flag_words_alloc(ctypes.byref(p)), "flag_words_alloc")
         self.ptr = p.value
@@ -81,7 +82,7 @@ class FlagWords:
     def __del__(self):
         ptr_ = getattr(self, "ptr", None)
         if ptr_:
-            self._lib.lakde_flag_words_free(ctypes.c_void_p(ptr_))
+            self._lib.lakde_flag_words_free(self._vp(ptr_))
             self.ptr = None
 
 
