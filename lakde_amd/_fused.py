@@ -144,7 +144,9 @@ class EstepWorkspace:
 
 
 def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
-               arena=None, b=None):
+               arena=None, b=None, order=None):
+    """``order``: None, or (col_perm, chunk_box) of utils.column_order — the
+    column order of the pair-skipping E-step."""
     N, D = X.shape
     nb = rec_blk.size(0)
     if arena is not None:
@@ -152,17 +154,19 @@ def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=
                arena.cap, ptr(arena.nnz_dev[b:b + 1]))
     else:
         out = (ptr(None), ptr(None), ptr(None), ptr(None), 0, ptr(None))
+    perm, box = order if order is not None else (None, None)
     return EstepArgs(N, D, nb, row0, ptr(X), ptr(rec_blk), mode, float(threshold or 0.0),
                      ptr(c_old), ptr(old_colsum), ptr(c), out[0], out[1], out[2], out[3], out[4],
-                     out[5], ptr(ws.buf), ws.buf.numel(), None, None)
+                     out[5], ptr(ws.buf), ws.buf.numel(), None, None, ptr(perm), ptr(box))
 
 
 def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
-          arena=None, b=None, compact_only=False, bf16=None):
+          arena=None, b=None, compact_only=False, bf16=None, order=None):
     """One E-step launch sequence.  ``bf16=frag`` (the block's fragment
     blocks) runs the main (quadratic form / LSE / ballot) kernel on the
     matrix cores."""
-    args = estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old, old_colsum, arena, b)
+    args = estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old, old_colsum, arena, b,
+                      order)
     lib = _lib.load()
     st = stream_of(X)
     if compact_only:

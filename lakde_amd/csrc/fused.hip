@@ -52,6 +52,22 @@ static int dispatch_pred_main(int64_t D, const PParams &p, hipStream_t s) {
 constexpr int kOffWaves = 16;
 static_assert(kESeg % kOffWaves == 0, "segments per wave");
 
+// With the skipping layout (col_perm) the main kernel sets mask bits from
+// scattered columns and only adds the row totals; the per-(segment, kernel)
+// counts are the masks' popcounts.  Wave per (64 kernels, segment), lane =
+// kernel; each count is written once (estep_offsets_kernel clears it).
+__global__ void __launch_bounds__(256) estep_segcount_kernel(EParams p) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t n = (int64_t)blockIdx.x * kWave + lane;
+  const int64_t seg = (int64_t)blockIdx.y * 4 + threadIdx.x / kWave;
+  if (n >= p.nb || seg >= kESeg) return;
+  const int64_t c0 = seg * p.seg_len;
+  const int64_t c1 = (c0 + p.seg_len) < p.nchunks ? (c0 + p.seg_len) : p.nchunks;
+  int32_t t = 0;
+  for (int64_t c = c0; c < c1; ++c) t += (int32_t)__popcll(p.masks[c * p.nb + n]);
+  p.seg_tot[seg * p.nb + n] = t;
+}
+
 __global__ void __launch_bounds__(1024) estep_offsets_kernel(EParams p) {
   __shared__ int32_t part[kOffWaves][kWave];
   __shared__ int32_t sh[1024];
@@ -365,7 +381,27 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   // overwritten and every launch finds (and leaves) them zero.
   const size_t segb = align256((size_t)kESeg * a.nb * 4);
   const size_t rowb = align256((size_t)a.nb * 4);
-  const size_t maskb = align256((size_t)a.nb * p->nchunks * 8);
+  // The masks start the workspace and the segment ends follow the masks of
+  // the LARGEST block the workspace fits (nb_cap), so no block's segment
+  // ends ever lie where another block's masks go: with the skipping layout
+  // (col_perm) the masks must read zero on entry, and only the compaction
+  // clears them.
+  int64_t nb_cap = a.nb;
+  if (ws != nullptr && a.workspace_bytes >= lakde_estep_workspace(a.N, a.nb)) {
+    // lo fits; grow hi until it does not, then bisect
+    int64_t lo = a.nb, hi = a.nb + 1;
+    while (hi < (1LL << 30) && lakde_estep_workspace(a.N, hi) <= a.workspace_bytes) {
+      lo = hi;
+      hi *= 2;
+    }
+    while (hi - lo > 1) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      if (lakde_estep_workspace(a.N, mid) <= a.workspace_bytes) lo = mid;
+      else hi = mid;
+    }
+    nb_cap = lo;
+  }
+  const size_t maskb = align256((size_t)nb_cap * p->nchunks * 8);
   char *end = ws ? (char *)(((uintptr_t)ws + a.workspace_bytes) & ~(uintptr_t)255) : ws + 512;
   p->seg_tot = (int32_t *)(end - 256 - segb - rowb);
   p->row_tot = (int32_t *)(end - 256 - rowb);
@@ -384,6 +420,16 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->status_flag = p->unsafe_flag = nullptr;
   p->flag_seq = 0;
   p->lazy_Rnew = p->lazy_Unew = nullptr;
+  p->col_perm = a.col_perm;
+  p->chunk_box = a.chunk_box;
+  // a skipped pair is at least skip_margin nats below its column's
+  // normaliser: far below FP32 resolution of the normaliser's sum (nb
+  // terms), and below any threshold (threshold 0 keeps r != 0, i.e. down to
+  // 2^-149 = e^-103.3)
+  p->skip_margin = a.threshold > 0.f ? fmaxf(40.f, 8.f - logf(a.threshold)) : 110.f;
+  if (a.col_perm != nullptr)
+    LAKDE_CHECK_ARG(a.chunk_box != nullptr && a.D <= kSkipMaxD,
+                    "col_perm needs chunk_box and D <= 16");
   if (a.mode == LAKDE_ESTEP_UPDATE && a.lazy_unsafe != nullptr) {
     LAKDE_CHECK_ARG(a.lazy_G != nullptr, "lazy_unsafe needs lazy_G");
     p->lazy_G = a.lazy_G;
@@ -464,6 +510,12 @@ int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *fl
   p.status_flag = status_flag;
   p.unsafe_flag = unsafe_flag;
   p.flag_seq = flag_seq;
+  if (p.col_perm != nullptr) {
+    hipLaunchKernelGGL(estep_segcount_kernel,
+                       dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
+                       s, p);
+    LAKDE_LAUNCHED();
+  }
   hipLaunchKernelGGL(estep_offsets_kernel, dim3((unsigned)ceil_div(p.nb, kWave) + 1u), dim3(1024),
                      0, s, p);
   LAKDE_LAUNCHED();

@@ -42,7 +42,19 @@ struct EParams {
   unsigned long long *flag_words;
   const int32_t *status_flag, *unsafe_flag;
   uint32_t flag_seq;
+  // optional (UPDATE, D <= kSkipMaxD): workgroup g takes the columns
+  // col_perm[64 g .. 64 g + 63] (a spatially compact chunk whose bounding
+  // box is chunk_box[g] = lo[D], hi[D]) and skips every kernel whose log rho
+  // is provably below (min lb of the chunk) - skip_margin on the whole box.
+  // Kept pairs are then OR-ed into the masks at their own columns (the masks
+  // must be zero on entry; the compaction clears what it reads) and the
+  // segment counts are taken from the masks after the launch.
+  const int32_t *col_perm;
+  const float *chunk_box;
+  float skip_margin;
 };
+
+constexpr int kSkipMaxD = 16;
 
 // the lazy rebalance's unsafe-step test for column m of an UPDATE step
 // (lakde_rebalance_lazy_check): G[m] clamp(exp(c_old - c)) outside
@@ -83,6 +95,39 @@ __device__ __forceinline__ void count_mask(const EParams &p, int64_t seg, int64_
   }
 }
 
+// Upper bound of log rho_n(x) over the box [lo, hi] (box = lo[D], hi[D]):
+// interval arithmetic on y = Linv (x - x_n) row by row gives a lower bound
+// of Q = |y|^2 (each y_i's distance from 0), and log rho = c + e Q with
+// e = -nu/2 < 0.  The bound is shrunk by 0.1% against FP32 rounding of the
+// interval sums; the callers' margins are tens of nats.  r: the kernel's
+// record (loaded per lane).
+template <int D>
+__device__ __forceinline__ float log_rho_box_ub(const float *__restrict__ r,
+                                                const float *__restrict__ box) {
+  float dl[D], dh[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    dl[j] = box[j] - r[rec_x(D) + j];
+    dh[j] = box[D + j] - r[rec_x(D) + j];
+  }
+  float lbq = 0.f;
+  int t = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    float lo = 0.f, hi = 0.f;
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      const float a = r[t + j] * dl[j], b = r[t + j] * dh[j];
+      lo += fminf(a, b);
+      hi += fmaxf(a, b);
+    }
+    t += i + 1;
+    const float g = fmaxf(fmaxf(lo, -hi), 0.f);
+    lbq = fmaf(g, g, lbq);
+  }
+  return r[rec_c(D)] + r[rec_c(D) + 1] * (0.999f * lbq);
+}
+
 // ---------------------------------------------------------------- E-step
 // One workgroup per chunk of 64 columns (points m) and all nb kernels of the
 // block.  Pass 1 (ACCUM/UPDATE): streaming log-sum-exp over the kernels,
@@ -101,20 +146,24 @@ __device__ __forceinline__ void count_mask(const EParams &p, int64_t seg, int64_
 // queued more than kEQueue pairs falls back to recomputing all of its pairs.
 constexpr int kEQueue = 8;
 
-template <int D, int MODE>
+template <int D, int MODE, bool SKIP = false>
 __global__ void __launch_bounds__(kEThreads)
 estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EParams p) {
   __shared__ float sm_m[4][kWave], sm_s[4][kWave], sm_c[kWave];
   __shared__ int32_t q_n[MODE == LAKDE_ESTEP_UPDATE ? kEQueue : 1][kEThreads];
   __shared__ float q_x[MODE == LAKDE_ESTEP_UPDATE ? kEQueue : 1][kEThreads];
   __shared__ uint64_t sm_win[4][kWave];
+  __shared__ int32_t sm_col[SKIP ? kWave : 1];
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   // wave index made explicitly uniform so the record loads below are scalar
   const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int64_t chunk = blockIdx.x;
-  const int64_t m = chunk * kWave + lane;
-  const bool valid = m < p.N;
+  // SKIP: the workgroup's columns are the chunk-th 64 of the order col_perm
+  const int64_t slot = chunk * kWave + lane;
+  const bool valid = slot < p.N;
+  const int64_t m = SKIP ? (valid ? (int64_t)p.col_perm[slot] : p.N) : slot;
+  if (SKIP && w == 0) sm_col[lane] = (int32_t)m;
   float xm[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) xm[j] = valid ? X[m * D + j] : 0.f;
@@ -149,47 +198,85 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
     // only the waves whose columns overlap the block's own rows hold the
     // excluded diagonal pair (n == m - row0)
     const int64_t col0 = chunk * kWave - p.row0;
-    const bool has_diag = col0 < nend && col0 + kWave > nbeg;
-    auto scan = [&](auto diag_tag) {
+    const bool has_diag = SKIP ? __ballot(valid && diag_n >= nbeg && diag_n < nend) != 0ull
+                               : (col0 < nend && col0 + kWave > nbeg);
+    // four kernels n0..n3 (wave-uniform), then one kernel
+    auto group4 = [&](auto diag_tag, int64_t n0, int64_t n1, int64_t n2, int64_t n3) {
       constexpr bool DIAG = decltype(diag_tag)::value;
-      int64_t n = nbeg;
-      if (D <= 24) {
-        for (; n + 4 <= nend; n += 4) {
-          const float *r0 = rec + (n + 0) * R, *r1 = rec + (n + 1) * R;
-          const float *r2 = rec + (n + 2) * R, *r3 = rec + (n + 3) * R;
-          const float q0 = qf_rec<D>(r0, xm), q1 = qf_rec<D>(r1, xm);
-          const float q2 = qf_rec<D>(r2, xm), q3 = qf_rec<D>(r3, xm);
-          float x0 = log_rho_q<D>(r0, q0), x1 = log_rho_q<D>(r1, q1);
-          float x2 = log_rho_q<D>(r2, q2), x3 = log_rho_q<D>(r3, q3);
-          if (DIAG) {
-            if (n + 0 == diag_n) x0 = neg_inf();
-            if (n + 1 == diag_n) x1 = neg_inf();
-            if (n + 2 == diag_n) x2 = neg_inf();
-            if (n + 3 == diag_n) x3 = neg_inf();
-          }
-          const float g = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
-          if (l.fold(g)) B = fmaxf(lb, g);
-          l.s += (l.term(x0) + l.term(x1)) + (l.term(x2) + l.term(x3));
-          if (MODE == LAKDE_ESTEP_UPDATE) {
-            // one pre-test on the group's largest value (fl(x - B) is
-            // monotone in x), then the group's pairs
-            if (valid && __fsub_rn(g, B) > p.log_t_lo) {
-              cand(x0, n + 0, B);
-              cand(x1, n + 1, B);
-              cand(x2, n + 2, B);
-              cand(x3, n + 3, B);
-            }
-          }
+      const float *r0 = rec + n0 * R, *r1 = rec + n1 * R;
+      const float *r2 = rec + n2 * R, *r3 = rec + n3 * R;
+      const float q0 = qf_rec<D>(r0, xm), q1 = qf_rec<D>(r1, xm);
+      const float q2 = qf_rec<D>(r2, xm), q3 = qf_rec<D>(r3, xm);
+      float x0 = log_rho_q<D>(r0, q0), x1 = log_rho_q<D>(r1, q1);
+      float x2 = log_rho_q<D>(r2, q2), x3 = log_rho_q<D>(r3, q3);
+      if (DIAG) {
+        if (n0 == diag_n) x0 = neg_inf();
+        if (n1 == diag_n) x1 = neg_inf();
+        if (n2 == diag_n) x2 = neg_inf();
+        if (n3 == diag_n) x3 = neg_inf();
+      }
+      const float g = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+      if (l.fold(g)) B = fmaxf(lb, g);
+      l.s += (l.term(x0) + l.term(x1)) + (l.term(x2) + l.term(x3));
+      if (MODE == LAKDE_ESTEP_UPDATE) {
+        // one pre-test on the group's largest value (fl(x - B) is
+        // monotone in x), then the group's pairs
+        if (valid && __fsub_rn(g, B) > p.log_t_lo) {
+          cand(x0, n0, B);
+          cand(x1, n1, B);
+          cand(x2, n2, B);
+          cand(x3, n3, B);
         }
       }
-      for (; n < nend; ++n) {
-        const float *r = rec + n * R;
-        float x = log_rho<D>(r, xm);
-        if (DIAG && n == diag_n) x = neg_inf();
-        if (l.fold(x)) B = fmaxf(lb, x);
-        l.s += l.term(x);
-        if (MODE == LAKDE_ESTEP_UPDATE && valid) cand(x, n, B);
+    };
+    auto one = [&](auto diag_tag, int64_t n) {
+      constexpr bool DIAG = decltype(diag_tag)::value;
+      float x = log_rho<D>(rec + n * R, xm);
+      if (DIAG && n == diag_n) x = neg_inf();
+      if (l.fold(x)) B = fmaxf(lb, x);
+      l.s += l.term(x);
+      if (MODE == LAKDE_ESTEP_UPDATE && valid) cand(x, n, B);
+    };
+    auto scan = [&](auto diag_tag) {
+      if constexpr (SKIP) {
+        // Kernels whose log rho stays below (the chunk's smallest lb) -
+        // skip_margin on the chunk's box contribute less than
+        // nb exp(-skip_margin) of the normaliser (below FP32 resolution) and
+        // can never be kept: only the others are evaluated.  lb <= the final
+        // normaliser, and -inf (no other mass) skips nothing.
+        float lbmin = valid ? lb : __builtin_huge_valf();
+#pragma unroll
+        for (int off = kWave / 2; off >= 1; off >>= 1)
+          lbmin = fminf(lbmin, __shfl_xor(lbmin, off, kWave));
+        const float limit = lbmin - p.skip_margin;
+        const float *box = p.chunk_box + chunk * 2 * D;
+        for (int64_t wb = nbeg; wb < nend; wb += kWave) {
+          const int64_t kn = wb + lane;
+          const bool need = kn < nend && !(log_rho_box_ub<D>(rec + kn * R, box) < limit);
+          uint64_t todo = __ballot(need);
+          while (__popcll(todo) >= 4) {
+            const int64_t n0 = wb + __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const int64_t n1 = wb + __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const int64_t n2 = wb + __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const int64_t n3 = wb + __builtin_ctzll(todo);
+            todo &= todo - 1;
+            group4(diag_tag, n0, n1, n2, n3);
+          }
+          while (todo) {
+            const int64_t n = wb + __builtin_ctzll(todo);
+            todo &= todo - 1;
+            one(diag_tag, n);
+          }
+        }
+        return;
       }
+      int64_t n = nbeg;
+      if (D <= 24)
+        for (; n + 4 <= nend; n += 4) group4(diag_tag, n, n + 1, n + 2, n + 3);
+      for (; n < nend; ++n) one(diag_tag, n);
     };
     if (has_diag) scan(std::true_type{});
     else scan(std::false_type{});
@@ -237,8 +324,22 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
       const int64_t n = nbeg + wb + lane;
       if (n < nend) {
         const uint64_t mk = sm_win[w][lane];
-        p.masks[chunk * p.nb + n] = mk;
-        count_mask(p, cseg, n, mk);
+        if constexpr (SKIP) {
+          // the chunk's columns are scattered: each kept pair sets its own
+          // bit; the kernel's row total once per wave
+          if (mk != 0ull) {
+            atomicAdd(&p.row_tot[n], (int32_t)__popcll(mk));
+            uint64_t bits = mk;
+            while (bits) {
+              const int64_t mc = sm_col[__builtin_ctzll(bits)];
+              bits &= bits - 1;
+              atomicOr((unsigned long long *)&p.masks[(mc >> 6) * p.nb + n], 1ull << (mc & 63));
+            }
+          }
+        } else {
+          p.masks[chunk * p.nb + n] = mk;
+          count_mask(p, cseg, n, mk);
+        }
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -252,7 +353,10 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
     float r;
     const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
     const uint64_t mask = __ballot(keep);
-    if (lane == 0) {
+    if constexpr (SKIP) {
+      if (keep) atomicOr((unsigned long long *)&p.masks[(m >> 6) * p.nb + n], 1ull << (m & 63));
+      if (lane == 0 && mask != 0ull) atomicAdd(&p.row_tot[n], (int32_t)__popcll(mask));
+    } else if (lane == 0) {
       p.masks[chunk * p.nb + n] = mask;
       count_mask(p, cseg, n, mask);
     }
@@ -321,6 +425,12 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
     uint64_t v[MB];
 #pragma unroll
     for (int u = 0; u < MB; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
+    if (p.col_perm != nullptr) {
+      // the next skipping E-step ORs into the masks: leave them zero
+#pragma unroll
+      for (int u = 0; u < MB; ++u)
+        if (v[u] != 0ull) p.masks[(cb + u) * p.nb + n] = 0ull;
+    }
 #pragma unroll
     for (int u = 0; u < MB; ++u) {
       uint64_t mask = v[u];
@@ -460,7 +570,13 @@ int launch_estep_main(const EParams &p, int mode, hipStream_t s) {
     hipLaunchKernelGGL((estep_main_kernel<D, LAKDE_ESTEP_ACCUM>), grid, dim3(kEThreads), 0, s, p.rec, p.X, p);
   else if (mode == LAKDE_ESTEP_FIXED)
     hipLaunchKernelGGL((estep_main_kernel<D, LAKDE_ESTEP_FIXED>), grid, dim3(kEThreads), 0, s, p.rec, p.X, p);
-  else
+  else if (p.col_perm != nullptr) {
+    if constexpr (D <= kSkipMaxD)
+      hipLaunchKernelGGL((estep_main_kernel<D, LAKDE_ESTEP_UPDATE, true>), grid, dim3(kEThreads), 0,
+                         s, p.rec, p.X, p);
+    else
+      return LAKDE_EINVAL;
+  } else
     hipLaunchKernelGGL((estep_main_kernel<D, LAKDE_ESTEP_UPDATE>), grid, dim3(kEThreads), 0, s, p.rec, p.X, p);
   return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
 }

@@ -34,7 +34,7 @@ from . import _lib
 from ._lib import (ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE, LAKDE_ECAPACITY, LAKDE_ESTABILIZE,
                    LAKDE_OK, LakdeError, check, require_gpu, stream_of)
 from .kdes import AbstractKDE
-from .utils import tensor_as
+from .utils import column_order, tensor_as
 
 
 class LocallyAdaptiveKDE(AbstractKDE):
@@ -44,7 +44,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
     """
 
     def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32",
-                 rebalance="lazy"):
+                 rebalance="lazy", estep_skip=True):
         super().__init__(block_size, verbose, logs)
         if precision not in ("fp32", "bf16"):
             raise ValueError("precision must be 'fp32' or 'bf16'")
@@ -52,6 +52,8 @@ class LocallyAdaptiveKDE(AbstractKDE):
             raise ValueError("rebalance must be 'lazy' or 'exact'")
         self.precision = precision
         self.rebalance = rebalance
+        self.estep_skip = bool(estep_skip)
+        self._col_order = None
         self._lazy = None
         self.nu_0 = nu_0
         self.a_0 = a_0
@@ -127,6 +129,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._flag_words = F.FlagWords()  # ... or, when given, on these flag words
         self._step_cache = {}
         self._ews = F.EstepWorkspace(N, bs, dev)
+        # pair skipping (FP32, D <= 16): spatially compact column chunks
+        self._col_order = None
+        if self.estep_skip and self.precision == "fp32" and D <= 16:
+            self._col_order = column_order(X)
         if self.precision == "bf16":
             if not 8 <= D <= 63:
                 raise ValueError("the bf16 precision form needs 8 <= D <= 63")
@@ -220,13 +226,13 @@ class LocallyAdaptiveKDE(AbstractKDE):
         s, e, nb = self._span(b, N)
         c = self.rnm_log_consts
         F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, arena=self.arena, b=b,
-                bf16=self._bf16_operands(b, N))
+                bf16=self._bf16_operands(b, N), order=self._col_order)
         nnz = self._sync(b)
         if nnz > self.arena.cap:
             self.arena.grow(nnz)
             self._step_cache = {}
             F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, arena=self.arena, b=b,
-                    compact_only=True)
+                    compact_only=True, order=self._col_order)
         old = self.arena.nnz[b]
         self.arena.nnz[b] = nnz
         return old, nnz
@@ -278,7 +284,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         a.estep = F.estep_args(X, self.rec[s:e], s, ESTEP_UPDATE, 0.0, self._c_new, self._ews,
                                c_old=self.rnm_log_consts,
                                old_colsum=lazy.old_colsum if lazy is not None else self._colsum,
-                               arena=arena, b=b)
+                               arena=arena, b=b, order=self._col_order)
         if graph is not None:
             a.knn_graph = graph.data_ptr()
             a.knn_graph_dtype = _lib.idx_code(graph.dtype)

@@ -22,8 +22,10 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
     def __init__(self, nu_0: Union[torch.Tensor, float] = None,
                  k_or_knn_graph: Union[torch.Tensor, int] = None, block_size: int = 512,
                  a_0: Union[torch.Tensor, float] = 1e-6, b_0: Union[torch.Tensor, float] = 1e-6,
-                 verbose=False, logs=False, precision="fp32", rebalance="lazy"):
-        super().__init__(nu_0, a_0, b_0, block_size, verbose, logs, precision, rebalance)
+                 verbose=False, logs=False, precision="fp32", rebalance="lazy",
+                 estep_skip=True):
+        super().__init__(nu_0, a_0, b_0, block_size, verbose, logs, precision, rebalance,
+                         estep_skip)
         self.k_or_knn_graph = k_or_knn_graph
         self.k = None
         self.knn_graph = None

@@ -232,6 +232,62 @@ def test_lazy_rebalance_matches_exact_mid_sweep():
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5)
 
 
+def test_estep_skip_matches_full_evaluation():
+    """The pair-skipping E-step (Morton-ordered column chunks, kernels bounded
+    on each chunk's box and skipped when provably tens of nats below the
+    normaliser) against the full evaluation on the same inputs: log
+    normalisers within 1e-5 relative, values within 1e-4 relative on the
+    common entries, index sets equal up to a few threshold-crossing entries,
+    identical posterior jitters and the same held-out density."""
+    from lakde_amd import LocalKNearestKDE
+    from lakde_amd import synthetic
+    X, Y = synthetic.make("mix32_d6", 20000, 500)
+    graph = O.compute_knn_graph(X, 64)
+    t = 1 / 64
+    Xd = _t(X)
+    ms = {}
+    for skip in (True, False):
+        m = LocalKNearestKDE(k_or_knn_graph=_t(graph), block_size=512, estep_skip=skip)
+        m.fit(Xd, iterations=3, sparsity_threshold=t, callbacks=[])
+        ms[skip] = m
+    assert ms[True]._col_order is not None and ms[False]._col_order is None
+    cs = ms[True].rnm_log_consts.cpu().numpy()
+    cf = ms[False].rnm_log_consts.cpu().numpy()
+    np.testing.assert_allclose(cs, cf, rtol=1e-5, atol=1e-5)
+    ps, pf = _pairs(ms[True]), _pairs(ms[False])
+    N = X.shape[0]
+    ks = ps[0] * N + ps[1]
+    kf = pf[0] * N + pf[1]
+    common, i_s, i_f = np.intersect1d(ks, kf, return_indices=True)
+    np.testing.assert_allclose(ps[2][i_s], pf[2][i_f], rtol=1e-4, atol=1e-9)
+    ndiff = ks.size + kf.size - 2 * common.size
+    assert ndiff <= 1e-4 * common.size + 5, (ndiff, common.size)
+    np.testing.assert_array_equal(ms[True].W_inv_jitter.cpu().numpy(),
+                                  ms[False].W_inv_jitter.cpu().numpy())
+    np.testing.assert_allclose(ms[True].nu.cpu().numpy(), ms[False].nu.cpu().numpy(),
+                               rtol=1e-5)
+    a = ms[True].log_pred_density(Xd, _t(Y)).cpu().numpy()
+    b = ms[False].log_pred_density(Xd, _t(Y)).cpu().numpy()
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5)
+    print(f"common {common.size}, differing {ndiff}")
+
+
+def test_column_order_chunks_and_boxes():
+    """utils.column_order: a permutation whose 64-column chunks have the
+    returned boxes as their exact coordinate-wise min / max."""
+    from lakde_amd.utils import column_order
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((1000, 5)).astype(np.float32)
+    perm, box = column_order(_t(X))
+    perm, box = perm.cpu().numpy(), box.cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(1000))
+    assert box.shape == (16, 2, 5)
+    for c in range(16):
+        pts = X[perm[c * 64:(c + 1) * 64]]
+        np.testing.assert_array_equal(box[c, 0], pts.min(0))
+        np.testing.assert_array_equal(box[c, 1], pts.max(0))
+
+
 def test_arena_growth_inside_sweep_matches():
     """A step whose entries outgrow the arena inside the native sweep loop
     (the driver returns, the arena grows, the step is finished by the
