@@ -212,17 +212,18 @@ typedef struct {
   int32_t *lazy_unsafe;
   /* optional (1 <= D <= 16): column order for pair skipping.  col_perm (N
    * int32) lists the columns so that every 64 consecutive entries form a
-   * spatially compact chunk (e.g. a Morton order); chunk_box holds each
-   * chunk's bounding box (ceil(N/64) x 2 x D floats: lo then hi).  An UPDATE
-   * launch then takes its columns chunk by chunk and skips every kernel
-   * whose log rho is provably more than max(40, 8 - log threshold) nats
-   * (110 for threshold 0) below the chunk's smallest lower bound of the new
-   * normaliser on the whole box.  Such pairs change no normaliser at FP32
-   * resolution and are never kept, so the outputs are those of the full
-   * evaluation.  With col_perm set, the workspace's masks must be zero on
-   * entry to a launch in any mode: a zero-filled workspace whose every
-   * compaction ran with col_perm set satisfies this (the compaction clears
-   * the masks it reads).  NULL: columns in index order, no skipping. */
+   * spatially compact chunk (e.g. a k-d ordering with 64-point leaves);
+   * chunk_box holds each chunk's bounding box (ceil(N/64) x 2 x D floats:
+   * lo then hi).  An UPDATE launch then takes its columns chunk by chunk and
+   * skips every kernel whose log rho is provably more than
+   * max(30 + log(nb/512), 8 - log threshold) nats (110 for threshold 0)
+   * below the chunk's smallest lower bound of the new normaliser on the
+   * whole box.  Such pairs change no normaliser at FP32 resolution and are
+   * never kept, so the outputs are those of the full evaluation.  With
+   * col_perm set, the workspace's masks must be zero on entry to a launch in
+   * any mode: a zero-filled workspace whose every compaction ran with
+   * col_perm set satisfies this (the compaction clears the masks it reads).
+   * NULL: columns in index order, no skipping. */
   const int32_t *col_perm;
   const float *chunk_box;
 } lakde_estep_args;

@@ -29,23 +29,35 @@ def compute_knn_graph(X, k, bsize=1024, include_self=False, dtype=torch.int32):
 
 
 def column_order(X, chunk=64):
-    """Column order for the pair-skipping E-step: the points sorted by their
-    Morton (Z-order) code on a 2^b grid per coordinate (b = min(10, 62 // D)),
-    so that every 64 consecutive columns form a spatially compact chunk, and
-    each chunk's bounding box.  Returns (col_perm int32 [N], chunk_box
-    float32 [ceil(N/64), 2, D]: lo then hi).  Setup glue (one sort), run
-    once per fit."""
+    """Column order for the pair-skipping E-step: a balanced k-d ordering of
+    the points whose leaves are the 64-column chunks (each node is split on
+    its widest coordinate at half its chunks, so the splits fall on chunk
+    boundaries), and each chunk's bounding box.  Returns (col_perm int32
+    [N], chunk_box float32 [ceil(N/64), 2, D]: lo then hi).  Setup glue (two
+    sorts per tree level), run once per fit."""
     N, D = X.shape
-    bits = max(1, min(10, 62 // D))
-    lo = X.amin(0)
-    span = (X.amax(0) - lo).clamp_min(1e-30)
-    q = ((X - lo) / span * (2 ** bits - 1)).round().long().clamp_(0, 2 ** bits - 1)
-    code = torch.zeros(N, dtype=torch.long, device=X.device)
-    for b in range(bits - 1, -1, -1):
-        for d in range(D):
-            code = (code << 1) | ((q[:, d] >> b) & 1)
-    perm = torch.argsort(code, stable=True)
+    dev = X.device
     nch = (N + chunk - 1) // chunk
+    perm = torch.arange(N, device=dev)
+    pos = torch.arange(N, device=dev)
+    start = torch.zeros(N, dtype=torch.long, device=dev)  # node = its first chunk
+    cnt = torch.full((N,), nch, dtype=torch.long, device=dev)  # the node's chunks
+    while bool((cnt > 1).any()):
+        active = cnt > 1
+        Xp = X[perm]
+        idx = start[:, None].expand(N, D)
+        hi = torch.full((nch, D), -float("inf"), device=dev).scatter_reduce(0, idx, Xp, "amax")
+        lo = torch.full((nch, D), float("inf"), device=dev).scatter_reduce(0, idx, Xp, "amin")
+        dim = (hi - lo).argmax(1)
+        val = Xp.gather(1, dim[start][:, None]).squeeze(1)
+        # within each node by the split coordinate (nodes stay contiguous)
+        o = torch.argsort(val, stable=True)
+        o = o[torch.argsort(start[o], stable=True)]
+        perm = perm[o]
+        left = cnt // 2
+        right = (pos - start * chunk) >= left * chunk
+        start, cnt = (torch.where(active & right, start + left, start),
+                      torch.where(active, torch.where(right, cnt - left, left), cnt))
     Xp = X[perm]
     if nch * chunk > N:  # pad the last chunk with its own last point
         Xp = torch.cat([Xp, Xp[-1:].expand(nch * chunk - N, D)])
