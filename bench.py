@@ -21,11 +21,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per launch of estep_main_kernel<6,UPDATE> on mix32_d6 with block
-# size 512: rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs, KiB per
-# dispatch) scaled by what they read for a plain 512 MiB elementwise copy
-# (factors 1.9999 and 1.0000); summary in profiles/estep_mix32_d6_hbm_traffic.md
-MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 8432194, "write": 13862526}}
+# HBM bytes per launch of the pair-skipping estep_main_kernel<6,UPDATE> on
+# mix32_d6 with block size 512: rocprofv3 FETCH_SIZE and WRITE_SIZE (separate
+# runs, KiB per dispatch) scaled by what they read for a plain 512 MiB
+# elementwise copy (factors 1.9999 and 1.0000); summary in
+# profiles/estep_mix32_d6_hbm_traffic.md
+MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 44619852, "write": 7872717}}
 
 
 def _parse():
@@ -385,7 +386,9 @@ def main():
     mean_lp = float(lp.mean())
 
     # roofline of the E-step main kernel (UPDATE mode): algorithmic work is
-    # D(D+1) FLOP per (kernel, point) pair, nb * N pairs per launch
+    # D(D+1) FLOP per (kernel, point) pair, nb * N pairs per launch -- the
+    # reference's pairs; with pair skipping (the default for D <= 16) the
+    # kernel evaluates only the pairs its bounds cannot exclude
     bs = model.block_size
     pairs_per_launch = (N * N) / nblocks
     flop = pairs_per_launch * D * (D + 1)
@@ -395,7 +398,7 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "traffic": (tr["fetch"] + tr["write"]) if tr else None,
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)",
-                "kernel": f"estep_main_kernel<{D},UPDATE>",
+                "kernel": f"estep_main_kernel<{D},UPDATE{',SKIP' if model._col_order is not None else ''}>",
                 "kernel_ms": round(k_ms, 5), "pairs_per_launch": int(pairs_per_launch)}
 
     prec = None

@@ -63,8 +63,15 @@ __global__ void __launch_bounds__(256) estep_segcount_kernel(EParams p) {
   if (n >= p.nb || seg >= kESeg) return;
   const int64_t c0 = seg * p.seg_len;
   const int64_t c1 = (c0 + p.seg_len) < p.nchunks ? (c0 + p.seg_len) : p.nchunks;
+  // 8 mask loads in flight per lane
   int32_t t = 0;
-  for (int64_t c = c0; c < c1; ++c) t += (int32_t)__popcll(p.masks[c * p.nb + n]);
+  for (int64_t cb = c0; cb < c1; cb += 8) {
+    uint64_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += (int32_t)__popcll(v[u]);
+  }
   p.seg_tot[seg * p.nb + n] = t;
 }
 
