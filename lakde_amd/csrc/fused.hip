@@ -52,29 +52,6 @@ static int dispatch_pred_main(int64_t D, const PParams &p, hipStream_t s) {
 constexpr int kOffWaves = 16;
 static_assert(kESeg % kOffWaves == 0, "segments per wave");
 
-// With the skipping layout (col_perm) the main kernel sets mask bits from
-// scattered columns and only adds the row totals; the per-(segment, kernel)
-// counts are the masks' popcounts.  Wave per (64 kernels, segment), lane =
-// kernel; each count is written once (estep_offsets_kernel clears it).
-__global__ void __launch_bounds__(256) estep_segcount_kernel(EParams p) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t n = (int64_t)blockIdx.x * kWave + lane;
-  const int64_t seg = (int64_t)blockIdx.y * 4 + threadIdx.x / kWave;
-  if (n >= p.nb || seg >= kESeg) return;
-  const int64_t c0 = seg * p.seg_len;
-  const int64_t c1 = (c0 + p.seg_len) < p.nchunks ? (c0 + p.seg_len) : p.nchunks;
-  // 8 mask loads in flight per lane
-  int32_t t = 0;
-  for (int64_t cb = c0; cb < c1; cb += 8) {
-    uint64_t v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) t += (int32_t)__popcll(v[u]);
-  }
-  p.seg_tot[seg * p.nb + n] = t;
-}
-
 __global__ void __launch_bounds__(1024) estep_offsets_kernel(EParams p) {
   __shared__ int32_t part[kOffWaves][kWave];
   __shared__ int32_t sh[1024];
@@ -519,12 +496,6 @@ int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *fl
   p.status_flag = status_flag;
   p.unsafe_flag = unsafe_flag;
   p.flag_seq = flag_seq;
-  if (p.col_perm != nullptr) {
-    hipLaunchKernelGGL(estep_segcount_kernel,
-                       dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
-                       s, p);
-    LAKDE_LAUNCHED();
-  }
   hipLaunchKernelGGL(estep_offsets_kernel, dim3((unsigned)ceil_div(p.nb, kWave) + 1u), dim3(1024),
                      0, s, p);
   LAKDE_LAUNCHED();

@@ -47,8 +47,8 @@ struct EParams {
   // box is chunk_box[g] = lo[D], hi[D]) and skips every kernel whose log rho
   // is provably below (min lb of the chunk) - skip_margin on the whole box.
   // Kept pairs are then OR-ed into the masks at their own columns (the masks
-  // must be zero on entry; the compaction clears what it reads) and the
-  // segment counts are taken from the masks after the launch.
+  // must be zero on entry; the compaction clears what it reads) and counted
+  // in their own segments one by one.
   const int32_t *col_perm;
   const float *chunk_box;
   float skip_margin;
@@ -326,7 +326,8 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
         const uint64_t mk = sm_win[w][lane];
         if constexpr (SKIP) {
           // the chunk's columns are scattered: each kept pair sets its own
-          // bit; the kernel's row total once per wave
+          // bit and counts in its own segment; the kernel's row total once
+          // per wave
           if (mk != 0ull) {
             atomicAdd(&p.row_tot[n], (int32_t)__popcll(mk));
             uint64_t bits = mk;
@@ -334,6 +335,7 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
               const int64_t mc = sm_col[__builtin_ctzll(bits)];
               bits &= bits - 1;
               atomicOr((unsigned long long *)&p.masks[(mc >> 6) * p.nb + n], 1ull << (mc & 63));
+              atomicAdd(&p.seg_tot[((mc >> 6) / p.seg_len) * p.nb + n], 1);
             }
           }
         } else {
@@ -354,7 +356,10 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
     const bool keep = valid && keep_resp(d, p.threshold, p.log_t_lo, &r);
     const uint64_t mask = __ballot(keep);
     if constexpr (SKIP) {
-      if (keep) atomicOr((unsigned long long *)&p.masks[(m >> 6) * p.nb + n], 1ull << (m & 63));
+      if (keep) {
+        atomicOr((unsigned long long *)&p.masks[(m >> 6) * p.nb + n], 1ull << (m & 63));
+        atomicAdd(&p.seg_tot[((m >> 6) / p.seg_len) * p.nb + n], 1);
+      }
       if (lane == 0 && mask != 0ull) atomicAdd(&p.row_tot[n], (int32_t)__popcll(mask));
     } else if (lane == 0) {
       p.masks[chunk * p.nb + n] = mask;

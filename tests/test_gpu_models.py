@@ -272,6 +272,37 @@ def test_estep_skip_matches_full_evaluation():
     print(f"common {common.size}, differing {ndiff}")
 
 
+@pytest.mark.parametrize("D,threshold", [(12, 1 / 32), (3, 0.0)])
+def test_estep_skip_matches_full_evaluation_other_shapes(D, threshold):
+    """Pair skipping against full evaluation at another dimension (the
+    largest skipping kernels are instantiated up to D=16) and with
+    threshold 0 (keep r != 0: the skip margin is 110 nats), on a ragged last
+    block (N not a multiple of the block size)."""
+    from lakde_amd import LocalKNearestKDE
+    rng = np.random.default_rng(D)
+    centres = rng.standard_normal((16, D)) * 4
+    X = (centres[rng.integers(0, 16, 6000)] + rng.standard_normal((6000, D))).astype(np.float32)
+    Y = (centres[rng.integers(0, 16, 300)] + rng.standard_normal((300, D))).astype(np.float32)
+    graph = O.compute_knn_graph(X, 24)
+    Xd = _t(X)
+    ms = {}
+    for skip in (True, False):
+        m = LocalKNearestKDE(k_or_knn_graph=_t(graph), block_size=512, estep_skip=skip)
+        m.fit(Xd, iterations=2, sparsity_threshold=threshold, callbacks=[])
+        ms[skip] = m
+    np.testing.assert_allclose(ms[True].rnm_log_consts.cpu().numpy(),
+                               ms[False].rnm_log_consts.cpu().numpy(), rtol=1e-5, atol=1e-5)
+    ps, pf = _pairs(ms[True]), _pairs(ms[False])
+    N = X.shape[0]
+    ks, kf = ps[0] * N + ps[1], pf[0] * N + pf[1]
+    common, i_s, i_f = np.intersect1d(ks, kf, return_indices=True)
+    np.testing.assert_allclose(ps[2][i_s], pf[2][i_f], rtol=1e-4, atol=1e-9)
+    assert ks.size + kf.size - 2 * common.size <= 1e-4 * common.size + 5
+    a = ms[True].log_pred_density(Xd, _t(Y)).cpu().numpy()
+    b = ms[False].log_pred_density(Xd, _t(Y)).cpu().numpy()
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5)
+
+
 def test_column_order_chunks_and_boxes():
     """utils.column_order: a permutation whose 64-column chunks have the
     returned boxes as their exact coordinate-wise min / max."""
