@@ -26,7 +26,7 @@ PEAK_HBM_GBS = 8000.0
 # runs, KiB per dispatch) scaled by what they read for a plain 512 MiB
 # elementwise copy (factors 1.9999 and 1.0000); summary in
 # profiles/estep_mix32_d6_hbm_traffic.md
-MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 44619852, "write": 7872717}}
+MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 47249802, "write": 8159539}}
 
 
 def _parse():
