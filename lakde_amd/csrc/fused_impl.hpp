@@ -82,7 +82,7 @@ struct PParams {
 };
 
 constexpr int kEThreads = 256;  // 4 waves; lane = column, wave = kernel quarter
-constexpr int kESeg = 128;      // chunk segments of the row scan and the compaction
+constexpr int kESeg = 256;      // chunk segments of the row scan and the compaction
 
 // a written mask's kept entries, added to its (segment, kernel) count and
 // to the kernel's row total
