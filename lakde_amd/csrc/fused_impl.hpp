@@ -96,33 +96,34 @@ __device__ __forceinline__ void count_mask(const EParams &p, int64_t seg, int64_
 }
 
 // Upper bound of log rho_n(x) over the box [lo, hi] (box = lo[D], hi[D]):
-// interval arithmetic on y = Linv (x - x_n) row by row gives a lower bound
-// of Q = |y|^2 (each y_i's distance from 0), and log rho = c + e Q with
-// e = -nu/2 < 0.  The bound is shrunk by 0.1% against FP32 rounding of the
-// interval sums; the callers' margins are tens of nats.  r: the kernel's
-// record (loaded per lane).
+// on the box each y_i = (Linv (x - x_n))_i lies within C_i +- R_i, with C_i
+// the row applied to the box centre minus x_n and R_i = sum_j |L_ij| h_j
+// (h = half-widths, inflated by 1e-6 against the rounding of centre and
+// width), so |y_i| >= max(|C_i| - R_i, 0) and Q = |y|^2 has a lower bound
+// LB; log rho = c + e Q with e = -nu/2 < 0.  LB is shrunk by 0.1% against
+// the rounding of the sums; the callers' margins are tens of nats.  Two
+// multiply-adds per factor entry.  r: the kernel's record (loaded per lane).
 template <int D>
 __device__ __forceinline__ float log_rho_box_ub(const float *__restrict__ r,
                                                 const float *__restrict__ box) {
-  float dl[D], dh[D];
+  float dm[D], hw[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    dl[j] = box[j] - r[rec_x(D) + j];
-    dh[j] = box[D + j] - r[rec_x(D) + j];
+    dm[j] = 0.5f * (box[j] + box[D + j]) - r[rec_x(D) + j];
+    hw[j] = 0.5f * (box[D + j] - box[j]) * 1.000001f;
   }
   float lbq = 0.f;
   int t = 0;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    float lo = 0.f, hi = 0.f;
+    float c = 0.f, rad = 0.f;
 #pragma unroll
     for (int j = 0; j <= i; ++j) {
-      const float a = r[t + j] * dl[j], b = r[t + j] * dh[j];
-      lo += fminf(a, b);
-      hi += fmaxf(a, b);
+      c = fmaf(r[t + j], dm[j], c);
+      rad = fmaf(fabsf(r[t + j]), hw[j], rad);
     }
     t += i + 1;
-    const float g = fmaxf(fmaxf(lo, -hi), 0.f);
+    const float g = fmaxf(fabsf(c) - rad, 0.f);
     lbq = fmaf(g, g, lbq);
   }
   return r[rec_c(D)] + r[rec_c(D) + 1] * (0.999f * lbq);
