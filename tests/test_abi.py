@@ -87,3 +87,49 @@ def test_product_refuses_cpu_tensors():
         kernels.ind2ptr(torch.zeros(3, dtype=torch.int32), 4)
     with pytest.raises(LakdeError):
         kernels.chol_quad_form(torch.eye(2)[None], torch.zeros(1, 2), torch.zeros(3, 2))
+
+
+def test_knn_workspace_query_and_argument_checks(lib):
+    """lakde_knn_graph: the workspace size query and the host-side checks of
+    k (no launch happens for rejected arguments or empty inputs)."""
+    lib.lakde_knn_graph_workspace.restype = ctypes.c_size_t
+    lib.lakde_knn_graph_workspace.argtypes = [ctypes.c_int64, ctypes.c_int64]
+    # one split at N = 1M (>= 2048 blocks of 256 queries): N k 8-byte entries
+    # plus N list sizes, each rounded to 256 bytes
+    n, k = 1_000_000, 128
+    assert lib.lakde_knn_graph_workspace(n, k) == (n * k * 8 + 255) // 256 * 256 + (n * 4 + 255) // 256 * 256
+    vp, i64 = ctypes.c_void_p, ctypes.c_int64
+    lib.lakde_knn_graph.argtypes = [vp, i64, i64, i64, ctypes.c_int, vp, ctypes.c_int, vp, vp,
+                                    ctypes.c_size_t, vp]
+    lib.lakde_knn_graph.restype = ctypes.c_int
+    dummy = ctypes.c_void_p(16)
+    # k > N - 1 without self, D out of range, bad dtype: EINVAL before any launch
+    assert lib.lakde_knn_graph(dummy, 10, 3, 10, 0, dummy, 0, None, dummy, 1 << 20, None) == 1
+    assert lib.lakde_knn_graph(dummy, 10, 64, 3, 0, dummy, 0, None, dummy, 1 << 20, None) == 1
+    assert lib.lakde_knn_graph(dummy, 10, 3, 3, 0, dummy, 7, None, dummy, 1 << 20, None) == 1
+    # N = 0: nothing to do
+    assert lib.lakde_knn_graph(dummy, 0, 3, 3, 0, dummy, 0, None, dummy, 0, None) == 0
+
+
+def test_column_order_cpu():
+    """utils.column_order (the pair-skipping E-step's column order): a
+    permutation whose 64-column chunks are boxed exactly, the partial chunk
+    last, and a k-d split on each node's widest coordinate (a chunk is much
+    tighter than a random one)."""
+    import numpy as np
+    import torch
+    from lakde_amd.utils import column_order
+    rng = np.random.default_rng(4)
+    X = torch.from_numpy((rng.standard_normal((5000, 4)) * [1, 3, 0.5, 2]).astype(np.float32))
+    perm, box = column_order(X)
+    p = perm.long()
+    assert torch.equal(torch.sort(p).values, torch.arange(5000))
+    nch = (5000 + 63) // 64
+    assert box.shape == (nch, 2, 4)
+    for c in range(nch):
+        pts = X[p[c * 64:(c + 1) * 64]]
+        assert torch.equal(box[c, 0], pts.amin(0)) and torch.equal(box[c, 1], pts.amax(0))
+    widths = (box[:, 1] - box[:, 0]).log().sum(1).mean()
+    rnd = X[torch.randperm(5000)[:4992]].view(78, 64, 4)
+    rwidths = (rnd.amax(1) - rnd.amin(1)).log().sum(1).mean()
+    assert widths < rwidths - 3.0
