@@ -349,6 +349,14 @@ typedef struct {
    * copy and an event; it stores them into host_flags[block],
    * host_flags[num_blocks], host_flags[num_blocks + 1] */
   void *flag_words;
+  /* optional (with flag_words; FP32, lazy rebalance, D <= 6): a device int
+   * that the row-pointer kernel of every step sets to 1 when the step needs
+   * the host (entries beyond cap, a failed stabilisation, an unsafe lazy
+   * step) and to 0 otherwise.  lakde_partial_sweep then enqueues the next
+   * step's posterior before it waits for this step's flags; that posterior
+   * does nothing when the word is 1, and is enqueued again after the host
+   * has handled the step.  NULL: no posterior is enqueued ahead. */
+  int32_t *abort_flag;
 } lakde_step_args;
 
 /* A 256-byte host-coherent, device-mapped buffer (hipHostMalloc,

@@ -22,9 +22,10 @@ class Arena:
     within a row.  This is the data of the reference's per-block
     ``rnm_blocks[b] = (indices (2, nnz) int32, values)``
     (lakde/local_adaptive_kde.py:145-153) with the row pointer kept alongside.
-    ``nnz_dev`` carries two extra slots: the posterior status flag and the
+    ``nnz_dev`` carries three extra slots: the posterior status flag and the
     lazy rebalance's unsafe-step flag, read with the entry counts in one
-    copy.
+    copy, and the step driver's abort word (set when a partial step needs
+    the host, so a posterior of the next step enqueued ahead does nothing).
     """
 
     def __init__(self, num_blocks, block_size, cap, device):
@@ -37,7 +38,7 @@ class Arena:
         self.col = torch.zeros(num_blocks, self.cap, dtype=i32, device=device)
         self.val = torch.zeros(num_blocks, self.cap, dtype=torch.float32, device=device)
         self.rowptr = torch.zeros(num_blocks, block_size + 1, dtype=i32, device=device)
-        self.nnz_dev = torch.zeros(num_blocks + 2, dtype=i32, device=device)
+        self.nnz_dev = torch.zeros(num_blocks + 3, dtype=i32, device=device)
         self.nnz = [0] * num_blocks
 
     @property
@@ -46,7 +47,11 @@ class Arena:
 
     @property
     def unsafe_ptr(self):
-        return self.nnz_dev[self.num_blocks + 1:]
+        return self.nnz_dev[self.num_blocks + 1:self.num_blocks + 2]
+
+    @property
+    def abort_ptr(self):
+        return self.nnz_dev[self.num_blocks + 2:]
 
     def grow(self, need):
         new_cap = max(int(need * 1.25), int(self.cap * 1.5)) + 255

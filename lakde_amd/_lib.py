@@ -55,7 +55,7 @@ class StepArgs(ctypes.Structure):
         ("nnz_all", _vp), ("cols_all", _vp), ("vals_all", _vp),
         ("G", _vp), ("U", _vp), ("U_old", _vp), ("lazy_ws", _vp), ("rebal_ws", _vp),
         ("fold_first", _int), ("bf16_frag", _vp), ("host_flags", _vp), ("ev_main", _vp * 2),
-        ("ev_flags", _vp), ("flag_words", _vp),
+        ("ev_flags", _vp), ("flag_words", _vp), ("abort_flag", _vp),
     ]
 
 

@@ -94,7 +94,8 @@ int posterior_contribs_lazy(const int32_t *rowptr, const int32_t *col, const flo
 int block_posterior(const lakde_posterior_args &post, const float *X, int64_t N, int64_t row0,
                     const int32_t *rowptr, const int32_t *col, const float *val,
                     const float *colscale, const void *graph, int graph_dtype, int64_t K,
-                    float nu0, float *old_colsum, double *U_old, int32_t *unsafe, hipStream_t s);
+                    float nu0, float *old_colsum, double *U_old, int32_t *unsafe, hipStream_t s,
+                    const int32_t *abort = nullptr);
 
 // Packed record layout (floats): [ Linv packed lower row-major: T = D(D+1)/2 ]
 // [ x_n: D ] [ estep_const, estep_expo, pred_coeff, pred_expo ], padded to 4.

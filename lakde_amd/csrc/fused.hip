@@ -105,6 +105,8 @@ __global__ void __launch_bounds__(1024) estep_offsets_kernel(EParams p) {
   if (tid == 1023) {
     p.rowptr[p.nb] = sh[1023];
     *p.nnz_dev = sh[1023];
+    if (p.abort_flag != nullptr)
+      *p.abort_flag = (sh[1023] > p.cap || *p.status_flag != 0 || *p.unsafe_flag != 0) ? 1 : 0;
     if (p.flag_words != nullptr) {
       const unsigned long long tag = (unsigned long long)p.flag_seq << 32;
       const unsigned long long w[3] = {tag | (uint32_t)sh[1023], tag | (uint32_t)*p.status_flag,
@@ -403,6 +405,7 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->flag_words = nullptr;
   p->status_flag = p->unsafe_flag = nullptr;
   p->flag_seq = 0;
+  p->abort_flag = nullptr;
   p->lazy_Rnew = p->lazy_Unew = nullptr;
   p->col_perm = a.col_perm;
   p->chunk_box = a.chunk_box;
@@ -488,7 +491,8 @@ int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsaf
 // the per-segment offsets, row pointer and entry count of an E-step (the
 // first half of lakde_estep_finish; *nnz_dev is final afterwards)
 int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *flag_words,
-                const int32_t *status_flag, const int32_t *unsafe_flag, uint32_t flag_seq) {
+                const int32_t *status_flag, const int32_t *unsafe_flag, uint32_t flag_seq,
+                int32_t *abort_flag) {
   EParams p;
   int rc = estep_fill_params(a, &p);
   if (rc) return rc;
@@ -496,6 +500,9 @@ int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *fl
   p.status_flag = status_flag;
   p.unsafe_flag = unsafe_flag;
   p.flag_seq = flag_seq;
+  p.abort_flag = (flag_words != nullptr && status_flag != nullptr && unsafe_flag != nullptr)
+                     ? abort_flag
+                     : nullptr;
   hipLaunchKernelGGL(estep_offsets_kernel, dim3((unsigned)ceil_div(p.nb, kWave) + 1u), dim3(1024),
                      0, s, p);
   LAKDE_LAUNCHED();

@@ -42,6 +42,9 @@ struct EParams {
   unsigned long long *flag_words;
   const int32_t *status_flag, *unsafe_flag;
   uint32_t flag_seq;
+  // optional (with flag_words): 1 when the step needs the host (entries
+  // beyond cap, status or unsafe set), else 0
+  int32_t *abort_flag;
   // optional (UPDATE, D <= kSkipMaxD): workgroup g takes the columns
   // col_perm[64 g .. 64 g + 63] (a spatially compact chunk whose bounding
   // box is chunk_box[g] = lo[D], hi[D]) and skips every kernel whose log rho
@@ -624,7 +627,8 @@ int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsaf
                        double *Unew, hipStream_t s);
 int estep_count(const lakde_estep_args &a, hipStream_t s,
                 unsigned long long *flag_words = nullptr, const int32_t *status_flag = nullptr,
-                const int32_t *unsafe_flag = nullptr, uint32_t flag_seq = 0);
+                const int32_t *unsafe_flag = nullptr, uint32_t flag_seq = 0,
+                int32_t *abort_flag = nullptr);
 int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
                            const float *c_prev, float *c, float *G, double *U, double *U_old,
                            float *old_colsum, double *ws, LazyGuard guard, hipStream_t s);

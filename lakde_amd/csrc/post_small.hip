@@ -35,6 +35,7 @@ struct SmallPost {
   float *old_colsum;            // optional lazy rebalance old column sums (true values)
   double *U_old;                //   and stored values
   int32_t *unsafe;              //   unsafe-step flag, cleared by the launch
+  const int32_t *abort;         // optional: the launch does nothing when *abort != 0
 };
 
 __host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // j <= i
@@ -137,6 +138,9 @@ __global__ void __launch_bounds__(kPostWaves * kWave) post_small_kernel(SmallPos
   const int64_t n = blockIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  // a posterior enqueued ahead of the previous step's flags: that step needs
+  // the host, which enqueues this one again after handling it
+  if (p.abort != nullptr && *p.abort != 0) return;
   if (p.unsafe != nullptr && n == 0 && threadIdx.x == 0) *p.unsafe = 0;
   float xn[D];
 #pragma unroll
@@ -326,7 +330,8 @@ static int launch_post_small(const SmallPost &p, hipStream_t s) {
 int block_posterior(const lakde_posterior_args &post, const float *X, int64_t N, int64_t row0,
                     const int32_t *rowptr, const int32_t *col, const float *val,
                     const float *colscale, const void *graph, int graph_dtype, int64_t K,
-                    float nu0, float *old_colsum, double *U_old, int32_t *unsafe, hipStream_t s) {
+                    float nu0, float *old_colsum, double *U_old, int32_t *unsafe, hipStream_t s,
+                    const int32_t *abort) {
   const int64_t D = post.D, nb = post.nb;
   LAKDE_CHECK_ARG(D >= 1 && D <= 63 && nb >= 0 && row0 >= 0 && row0 + nb <= N, "bad sizes");
   LAKDE_CHECK_ARG(X && rowptr && col && val, "null pointer");
@@ -344,7 +349,7 @@ int block_posterior(const lakde_posterior_args &post, const float *X, int64_t N,
                         post.logdet_W && post.trace && post.rec && post.status,
                     "null pointer");
     SmallPost p{post, X, row0, rowptr, col, val, colscale, graph, graph_dtype, K, nu0,
-                old_colsum, U_old, unsafe};
+                old_colsum, U_old, unsafe, abort};
     switch (D) {
       case 1: return launch_post_small<1>(p, s);
       case 2: return launch_post_small<2>(p, s);
@@ -354,6 +359,7 @@ int block_posterior(const lakde_posterior_args &post, const float *X, int64_t N,
       case 6: return launch_post_small<6>(p, s);
     }
   }
+  LAKDE_CHECK_ARG(abort == nullptr, "a guarded posterior needs D <= 6");
   int rc;
   if (graph != nullptr) {
     rc = lakde_knn_prior_cov(X, N, D, row0, nb, graph, graph_dtype, K,

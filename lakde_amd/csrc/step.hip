@@ -20,6 +20,8 @@
 //   compaction (lazy: storing u, with the       lakde_estep_compact (+ the first half of
 //     block's column sums), guarded lazy norm     lakde_rebalance_lazy, fused), lazy_norm
 //   -- wait for the copy; host-side outcome: exact fallback / lakde_rebalance --
+#include <chrono>
+
 #include "fused_impl.hpp"
 
 using namespace lakde;
@@ -47,8 +49,15 @@ hipError_t wait_event(hipEvent_t ev) {
 // into host-coherent memory, each word seq << 32 | value) and copy them into
 // host_flags.  If the stream completes without the words having arrived,
 // the flags are read from device memory instead, so the wait always ends.
+// The stream is queried only once the words are late (kFlagPatience): a
+// query of a busy stream enqueues a signalled marker, and the device then
+// idles ~5 us between the kernels either side of it, every step.
+constexpr auto kFlagPatience = std::chrono::milliseconds(20);
+
 int wait_flag_words(const lakde_step_args *a, uint32_t seq, hipStream_t s) {
   volatile unsigned long long *wd = reinterpret_cast<volatile unsigned long long *>(a->flag_words);
+  const auto t0 = std::chrono::steady_clock::now();
+  bool late = false;
   for (uint64_t it = 0;; ++it) {
     const unsigned long long w0 = wd[0], w1 = wd[1], w2 = wd[2];
     if ((uint32_t)(w0 >> 32) == seq && (uint32_t)(w1 >> 32) == seq &&
@@ -59,6 +68,10 @@ int wait_flag_words(const lakde_step_args *a, uint32_t seq, hipStream_t s) {
       return LAKDE_OK;
     }
     if ((it & 1023) == 1023) {
+      if (!late) {
+        late = std::chrono::steady_clock::now() - t0 > kFlagPatience;
+        if (!late) continue;
+      }
       const hipError_t q = hipStreamQuery(s);
       if (q == hipSuccess) {
         const unsigned long long v0 = wd[0], v1 = wd[1], v2 = wd[2];
@@ -105,11 +118,8 @@ int step_rebalance(const lakde_step_args *a, hipStream_t s) {
   return LAKDE_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
+// argument checks of one step
+int step_check(const lakde_step_args *a) {
   LAKDE_CHECK_ARG(a != nullptr, "null args");
   const lakde_estep_args &e = a->estep;
   const lakde_posterior_args &p = a->post;
@@ -126,23 +136,28 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(a->G != nullptr || a->rebal_ws != nullptr, "no rebalance workspace");
   LAKDE_CHECK_ARG(a->G == nullptr || (a->U && a->U_old && a->lazy_ws && a->rebal_ws),
                   "lazy rebalance state incomplete");
-  hipStream_t s = as_stream(stream);
+  return LAKDE_OK;
+}
+
+// [fold], prior, contributions (+ the lazy rebalance's old column sums) and
+// factorisation [, old column sums, bf16 fragments]; abort: see
+// lakde_step_args.abort_flag
+int step_begin(const lakde_step_args *a, hipStream_t s, const int32_t *abort) {
+  const lakde_estep_args &e = a->estep;
+  const lakde_posterior_args &p = a->post;
   const int64_t N = e.N, D = e.D, nb = e.nb;
   int32_t *unsafe = a->nnz_all + a->num_blocks + 1;
   int rc;
-
   if (a->G != nullptr && a->fold_first) {
     rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all,
                               a->G, a->U, s);
     if (rc) return rc;
   }
-  // prior, contributions (+ the lazy rebalance's old column sums) and
-  // factorisation
   const bool lazy = a->G != nullptr;
   rc = block_posterior(p, e.X, N, e.row0, e.rowptr, e.out_col, e.out_val, a->G, a->knn_graph,
                        a->knn_graph_dtype, a->knn_k, a->nu0,
                        lazy ? const_cast<float *>(e.old_colsum) : nullptr,
-                       lazy ? a->U_old : nullptr, lazy ? unsafe : nullptr, s);
+                       lazy ? a->U_old : nullptr, lazy ? unsafe : nullptr, s, abort);
   if (rc) return rc;
   if (a->G == nullptr) {
     rc = lakde_colsum(e.out_col, e.out_val, e.nnz_dev, N, const_cast<float *>(e.old_colsum), s);
@@ -152,6 +167,16 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
     rc = lakde_pack_frag_bf16(e.rec, nb, D, a->bf16_frag, s);
     if (rc) return rc;
   }
+  return LAKDE_OK;
+}
+
+// E-step, offsets (+ flags), compaction and the guarded lazy normalisation;
+// *seq: the flag words' tag of this step (0 without flag words)
+int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
+  const lakde_estep_args &e = a->estep;
+  const int64_t N = e.N;
+  int32_t *unsafe = a->nnz_all + a->num_blocks + 1;
+  int rc;
   // the E-step main kernel also runs the lazy rebalance's unsafe-step test
   lakde_estep_args em = e;
   if (a->G != nullptr) {
@@ -175,7 +200,7 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
     wd[3] = seq;
     // the kernel writes through the words' device-side address (word 4)
     auto *dev_words = reinterpret_cast<unsigned long long *>((uintptr_t)wd[4]);
-    rc = estep_count(e, s, dev_words, a->nnz_all + a->num_blocks, unsafe, seq);
+    rc = estep_count(e, s, dev_words, a->nnz_all + a->num_blocks, unsafe, seq, a->abort_flag);
     if (rc) return rc;
   } else {
     rc = estep_count(e, s);
@@ -203,6 +228,13 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
     rc = lakde_estep_compact(&e, s);
     if (rc) return rc;
   }
+  *seq_out = seq;
+  return LAKDE_OK;
+}
+
+// wait for the step's flags; the host's share of the step
+int step_wait(const lakde_step_args *a, hipStream_t s, uint32_t seq) {
+  int rc;
   if (a->flag_words != nullptr) {
     rc = wait_flag_words(a, seq, s);
     if (rc) return rc;
@@ -221,20 +253,75 @@ int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
   return step_rebalance(a, s);
 }
 
+// whether the next step's posterior may be enqueued before this step's
+// flags (the fused D <= 6 posterior honours the abort word)
+bool can_begin_ahead(const lakde_step_args *a) {
+  return a->G != nullptr && a->bf16_frag == nullptr && a->flag_words != nullptr &&
+         a->abort_flag != nullptr && a->estep.D <= 6;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lakde_partial_step(const lakde_step_args *a, lakde_stream_t stream) {
+  int rc = step_check(a);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  rc = step_begin(a, s, nullptr);
+  if (rc) return rc;
+  uint32_t seq = 0;
+  rc = step_rest(a, s, &seq);
+  if (rc) return rc;
+  return step_wait(a, s, seq);
+}
+
 int lakde_partial_sweep(lakde_step_args *steps, int64_t first, int64_t count, float *const *c,
                         int *cur, int *lazy_dirty, int64_t *done, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(steps && c && c[0] && c[1] && cur && lazy_dirty && done, "null pointer");
   LAKDE_CHECK_ARG(*cur == 0 || *cur == 1, "cur must be 0 or 1");
   LAKDE_CHECK_ARG(first >= 0 && count >= 0, "bad range");
+  hipStream_t s = as_stream(stream);
   *done = 0;
+  bool begun = false;  // the current step's posterior was enqueued ahead, guarded
   for (int64_t i = 0; i < count; ++i) {
     lakde_step_args *a = steps + first + i;
     LAKDE_CHECK_ARG(a->block == first + i, "steps[b] must be the arguments of block b");
     a->estep.c_old = c[*cur];
     a->estep.c = c[1 - *cur];
     a->fold_first = (a->G != nullptr && a->block == 0 && *lazy_dirty) ? 1 : 0;
-    const int rc = lakde_partial_step(a, stream);
+    int rc = step_check(a);
     if (rc) return rc;
+    if (!begun) {
+      rc = step_begin(a, s, nullptr);
+      if (rc) return rc;
+    }
+    uint32_t seq = 0;
+    rc = step_rest(a, s, &seq);
+    if (rc) return rc;
+    // the next step's posterior goes in before the wait, so the device does
+    // not idle while the host reads this step's flags; it does nothing if
+    // this step turns out to need the host (the abort word)
+    begun = false;
+    if (i + 1 < count && can_begin_ahead(a)) {
+      lakde_step_args *nx = a + 1;
+      LAKDE_CHECK_ARG(nx->block == first + i + 1, "steps[b] must be the arguments of block b");
+      nx->estep.c_old = c[1 - *cur];
+      nx->estep.c = c[*cur];
+      nx->fold_first = 0;
+      if (can_begin_ahead(nx) && nx->abort_flag == a->abort_flag) {
+        rc = step_check(nx);
+        if (rc) return rc;
+        rc = step_begin(nx, s, a->abort_flag);
+        if (rc) return rc;
+        begun = true;
+      }
+    }
+    rc = step_wait(a, s, seq);
+    if (rc) return rc;
+    // an unsafe step was finished by the host after the guarded posterior,
+    // which did nothing: enqueue it again
+    if (a->G != nullptr && a->host_flags[a->num_blocks + 1] != 0) begun = false;
     *cur = 1 - *cur;
     if (a->G != nullptr) *lazy_dirty = a->host_flags[a->num_blocks + 1] == 0 ? 1 : 0;
     *done = i + 1;

@@ -303,6 +303,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         a.host_flags = self._host_flags.data_ptr()
         a.ev_flags = self._flag_event.cuda_event
         a.flag_words = self._flag_words.ptr
+        a.abort_flag = arena.abort_ptr.data_ptr()
         self._step_cache["built"][b] = True
         return a
 
