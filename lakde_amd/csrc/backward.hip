@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) bwd_kernel_stats(const float *__restrict_
                                                        float *__restrict__ beta,
                                                        float *__restrict__ tau) {
   __shared__ float Ls[64][65];
-  __shared__ float dl[64][64];
+  __shared__ float dl[64][65];  // padded: a wave reads 16 rows at one column
   __shared__ float al[64];
   __shared__ float red[2][256];
   const int64_t n = blockIdx.x;

@@ -16,6 +16,42 @@ namespace lakde {
 
 constexpr int kPostMaxD = 63;
 
+// leading dimension of the LDS matrices: the smallest odd value >= D
+__host__ __device__ constexpr int post_ld(int D) { return D | 1; }
+
+// Reads src[0, count) with kLoadBatch loads in flight per lane and hands
+// each value to put(t, v): a copy loop that waited for every load would
+// spend a global-memory latency per element.
+constexpr int kLoadBatch = 16;
+
+template <typename Put>
+__device__ __forceinline__ void load_batched(const float *__restrict__ src, int count, int lane,
+                                             Put put) {
+  for (int t0 = 0; t0 < count; t0 += kLoadBatch * kWave) {
+    float v[kLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int t = t0 + u * kWave + lane;
+      v[u] = t < count ? src[t] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int t = t0 + u * kWave + lane;
+      if (t < count) put(t, v[u]);
+    }
+  }
+}
+
+// row i and column r of entry t of a row-packed lower triangle
+// (t = i (i + 1) / 2 + r, r <= i); the float estimate is corrected exactly
+__device__ __forceinline__ void tri_index(int t, int *i, int *r) {
+  int k = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+  if ((k + 1) * (k + 2) / 2 <= t) ++k;
+  if (k * (k + 1) / 2 > t) --k;
+  *i = k;
+  *r = t - k * (k + 1) / 2;
+}
+
 // Lower Cholesky of M (+ jitter on the diagonal, added in FP32) into Lout.
 // Lane i owns row i.  Returns true on success (every pivot > 0).
 template <typename T>
@@ -27,6 +63,9 @@ __device__ bool chol_wave(const float *M, T *Lout, int D, int ld, float jitter) 
       float mij = M[lane * ld + j];
       if (lane == j) mij = mij + jitter;
       T acc = T(mij);
+      // unrolled so the LDS reads of 8 steps are in flight together (the
+      // multiply-adds stay in order)
+#pragma unroll 8
       for (int k = 0; k < j; ++k) acc = fma(-Lout[lane * ld + k], Lout[j * ld + k], acc);
       s = acc;
     }
@@ -51,6 +90,7 @@ __device__ void tril_inv_wave(const T *L, T *X, int D, int ld) {
   if (c < D) {
     for (int i = 0; i < D; ++i) {
       T acc = (i == c) ? T(1) : T(0);
+#pragma unroll 8
       for (int k = 0; k < i; ++k) acc = fma(-L[i * ld + k], X[k * ld + c], acc);
       X[i * ld + c] = (i < c) ? T(0) : acc / L[i * ld + i];
     }
@@ -78,7 +118,9 @@ __device__ float stabilized_chol_wave(const float *M, float *L, int D, int ld, f
 __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   extern __shared__ float smem[];
   const int D = (int)a.D;
-  const int ld = D + 1;
+  // odd row stride: lane i's row-wise reads (L[i * ld + k]) then fall in
+  // different LDS banks (an even stride, 64 at D = 63, serialises them)
+  const int ld = post_ld(D);
   const int64_t n = blockIdx.x;
   const int lane = threadIdx.x;
   float *R0 = smem;               // working matrix
@@ -89,7 +131,7 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
 
   // ---- prior
   const float *P = (a.prior_mode == LAKDE_PRIOR_KNN) ? a.prior + n * DD : a.prior;
-  for (int t = lane; t < D * D; t += kWave) R0[(t / D) * ld + (t % D)] = P[t];
+  load_batched(P, D * D, lane, [&](int t, float v) { R0[(t / D) * ld + (t % D)] = v; });
   __syncthreads();
   if (a.prior_mode == LAKDE_PRIOR_KNN) {
     bool ok;
@@ -112,10 +154,10 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   const float etau = a.a[n] / a.b[n];
   const float scale = a.nu0 * etau;
   const float *S = a.S + n * DD;
-  for (int t = lane; t < D * D; t += kWave) {
-    int i = t / D, k = t % D;
-    R0[i * ld + k] = __fadd_rn(__fmul_rn(R0[i * ld + k], scale), S[t]);
-  }
+  load_batched(S, D * D, lane, [&](int t, float v) {
+    const int i = t / D, k = t % D;
+    R0[i * ld + k] = __fadd_rn(__fmul_rn(R0[i * ld + k], scale), v);
+  });
   __syncthreads();
 
   // ---- factor with the persistent jitter (FP32 decisions)
@@ -156,16 +198,24 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   const float logdet = (float)(2.0 * lsum);
 
   // ---- trace = nu * sum_{i,j} (sum_k T[k][i] Linv[j][k])^2
+  // Linv is lower triangular (k <= j).  The kNN prior's T is its Cholesky
+  // factor, lower triangular too, so only i <= k <= j contribute there: the
+  // terms left out are exact zeros (the sums are unchanged for finite
+  // factors) and the longest lane runs D(D+1)/2 steps instead of D^2.  The
+  // shared prior's T is the dense prior covariance (all i, all k <= j).
   float *Tm = R3;
-  const float *Tsrc = (a.prior_mode == LAKDE_PRIOR_KNN) ? a.prior_tril + n * DD : a.prior;
+  const bool tri = a.prior_mode == LAKDE_PRIOR_KNN;
+  const float *Tsrc = tri ? a.prior_tril + n * DD : a.prior;
   __syncthreads();
-  for (int t = lane; t < D * D; t += kWave) Tm[(t / D) * ld + (t % D)] = Tsrc[t];
+  load_batched(Tsrc, D * D, lane, [&](int t, float v) { Tm[(t / D) * ld + (t % D)] = v; });
   __syncthreads();
   float tr = 0.f;
   if (lane < D) {
-    for (int i = 0; i < D; ++i) {
+    const int iend = tri ? lane + 1 : D;
+    for (int i = 0; i < iend; ++i) {
       float s = 0.f;
-      for (int k = 0; k <= lane; ++k) s = fmaf(Tm[k * ld + i], Linv[lane * ld + k], s);
+#pragma unroll 8
+      for (int k = tri ? i : 0; k <= lane; ++k) s = fmaf(Tm[k * ld + i], Linv[lane * ld + k], s);
       tr = fmaf(s, s, tr);
     }
   }
@@ -176,8 +226,8 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   const int64_t R = rec_floats(D);
   float *rec = a.rec + n * R;
   for (int t = lane; t < (int)rec_tri(D); t += kWave) {
-    int i = 0, r = t;
-    while (r > i) { r -= i + 1; ++i; }
+    int i, r;
+    tri_index(t, &i, &r);
     rec[t] = Linv[i * ld + r];
   }
   if (lane < D) rec[rec_x(D) + lane] = a.Xn[n * D + lane];
@@ -207,7 +257,7 @@ extern "C" int lakde_posterior(const lakde_posterior_args *args, lakde_stream_t 
   if (a.prior_mode == LAKDE_PRIOR_KNN)
     LAKDE_CHECK_ARG(a.prior_jitter && a.prior_tril && a.logdet_prior, "kNN prior outputs missing");
   if (a.nb == 0) return LAKDE_OK;
-  size_t lds = (size_t)4 * a.D * (a.D + 1) * sizeof(float);
+  size_t lds = (size_t)4 * a.D * post_ld((int)a.D) * sizeof(float);
   hipLaunchKernelGGL(posterior_kernel, dim3((unsigned)a.nb), dim3(64), lds, as_stream(stream), a);
   LAKDE_LAUNCHED();
   return LAKDE_OK;

@@ -297,6 +297,60 @@ def test_posterior_shared_prior():
     np.testing.assert_allclose(out["trace"], tr_ref, rtol=1e-4)
 
 
+@pytest.mark.parametrize("D", [20, 43, 63])
+@pytest.mark.parametrize("shared", [False, True])
+@pytest.mark.parametrize("jit", [0.0, 1e-3])
+def test_posterior_kernel_large_d(D, shared, jit):
+    """The posterior kernel at the dimensions of the d=21/43/63 workloads
+    (even and odd D give different LDS row strides): W = Linv^T Linv against
+    the float64 inverse of W_inv (+ the stored jitter: a nonzero one takes
+    the float64 refactorisation), logdet against slogdet, the trace against
+    chol_mm_trace in float64 (triangular prior factor or dense shared prior)."""
+    rng = np.random.default_rng(100 + D)
+    nb = 70
+    nu0 = np.float32(D)
+    B = rng.standard_normal((nb, D, D)).astype(np.float32)
+    S = (B @ np.swapaxes(B, 1, 2) / D).astype(np.float32)
+    nu = (D + rng.random(nb) * 50).astype(np.float32)
+    a = np.full(nb, 2.0, np.float32)
+    b = (1 + rng.random(nb)).astype(np.float32)
+    Xn = rng.standard_normal((nb, D)).astype(np.float32)
+    jw0 = np.full(nb, jit, np.float32)
+    if shared:
+        C = rng.standard_normal((D, D)).astype(np.float32)
+        prior = (C @ C.T / D + np.eye(D, dtype=np.float32)).astype(np.float32)
+        out = _run_posterior(prior, S, nu, a, b, nu0, Xn, jw0, shared=True)
+        prior_j = np.broadcast_to(prior, (nb, D, D))
+        T = prior_j.astype(np.float64)
+    else:
+        A = rng.standard_normal((nb, D, D)).astype(np.float32)
+        prior = (A @ np.swapaxes(A, 1, 2) / D + 0.1 * np.eye(D, dtype=np.float32)).astype(np.float32)
+        out = _run_posterior(prior, S, nu, a, b, nu0, Xn, jw0, np.zeros(nb, np.float32))
+        assert np.all(out["jit_p"] == 0)
+        prior_j = prior
+        Lp = out["ptril"].astype(np.float64)
+        np.testing.assert_allclose(Lp @ np.swapaxes(Lp, 1, 2), prior, rtol=1e-4, atol=1e-4)
+        assert np.all(np.triu(out["ptril"], 1) == 0)
+        T = Lp
+    assert out["status"][0] == 0
+    np.testing.assert_array_equal(out["jit_w"], jw0)
+    scale = (nu0 * (a / b).astype(np.float32)).astype(np.float32)
+    Winv = (prior_j * scale[:, None, None]).astype(np.float32) + S
+    Winv = O._add_diag(Winv, jw0)
+    Linv = out["Linv"].astype(np.float64)
+    assert np.all(np.triu(out["Linv"], 1) == 0)
+    W_ref = np.linalg.inv(Winv.astype(np.float64))
+    W_dev = np.swapaxes(Linv, 1, 2) @ Linv
+    rel = np.abs(W_dev - W_ref).max(axis=(1, 2)) / np.abs(W_ref).max(axis=(1, 2))
+    assert rel.max() < 1e-3
+    np.testing.assert_allclose(out["logdet"], -np.linalg.slogdet(Winv.astype(np.float64))[1],
+                               rtol=1e-4, atol=1e-3)
+    tr_ref = nu * O.chol_mm_trace(T, np.swapaxes(Linv, 1, 2))
+    np.testing.assert_allclose(out["trace"], tr_ref, rtol=1e-4)
+    tri = np.tril_indices(D)
+    np.testing.assert_array_equal(out["rec"][:, :D * (D + 1) // 2], out["Linv"][:, tri[0], tri[1]])
+
+
 def test_posterior_jitter_schedule_exact():
     """Rounding-independent jitter decisions: exact zero pivot -> eps;
     a -3 eps pivot -> 7 eps; a -1e-3 pivot -> (2^14 - 1) eps."""
