@@ -47,6 +47,7 @@ static int launch_ind2ptr(const void *ind, int64_t nnz, int64_t M, void *out, hi
 constexpr int kOuterThreads = 256;
 constexpr int kMaxPairs = 9;          // 256 * 9 >= 63*64/2 + 64
 constexpr int kStageElems = 4096;     // LDS staging elements per chunk
+constexpr int kStageBatch = 8;        // staging loads in flight per thread
 
 template <typename T, typename RP, typename CI, bool GRAPH>
 __global__ void __launch_bounds__(kOuterThreads)
@@ -101,10 +102,27 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
       const int64_t ne = (end - c0) < E ? (end - c0) : E;
       __syncthreads();
       const int Di = (int)D;
-      for (int t = tid; t < (int)ne * Di; t += kOuterThreads) {
-        const int e = t / Di, d = t - e * Di;
-        const int64_t cidx = (int64_t)col[c0 + e];
-        stage[t] = Xm[cidx * D + d] - xn_s[d];
+      const int nt = (int)ne * Di;
+      // kStageBatch column indices, then the kStageBatch gathered
+      // coordinates, in flight per thread (two dependent loads per element)
+      for (int t0 = tid; t0 < nt; t0 += kOuterThreads * kStageBatch) {
+        int64_t cidx[kStageBatch];
+        T xv[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int t = t0 + u * kOuterThreads;
+          cidx[u] = t < nt ? (int64_t)col[c0 + t / Di] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int t = t0 + u * kOuterThreads;
+          xv[u] = t < nt ? Xm[cidx[u] * D + (t - (t / Di) * Di)] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+          const int t = t0 + u * kOuterThreads;
+          if (t < nt) stage[t] = xv[u] - xn_s[t - (t / Di) * Di];
+        }
       }
       for (int64_t e = tid; e < ne; e += kOuterThreads) {
         T wv = GRAPH ? T(1) : value[c0 + e];
