@@ -48,6 +48,13 @@ constexpr int kOuterThreads = 256;
 constexpr int kMaxPairs = 9;          // 256 * 9 >= 63*64/2 + 64
 constexpr int kStageElems = 4096;     // LDS staging elements per chunk
 constexpr int kStageBatch = 8;        // staging loads in flight per thread
+// 3 x 3 register tiles of the upper triangle for 23 <= D <= 63: one tile per
+// thread (ceil(D/3) (ceil(D/3) + 1) / 2 <= 231 tiles), 6 LDS reads per
+// entry for 9 products instead of 2 per product.  Every output pair i <= j
+// is still fma(w * d_i, d_j, acc) over the entries in order, so the sums are
+// the same as with the pair-per-slot mapping.
+constexpr int kTileMinD = 23;         // below: P <= 256, the entry-group split
+constexpr int kTileMaxD = 63;
 
 template <typename T, typename RP, typename CI, bool GRAPH>
 __global__ void __launch_bounds__(kOuterThreads)
@@ -72,6 +79,25 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
   if (old.unsafe_reset != nullptr && n == 0 && tid == 0) *old.unsafe_reset = 0;
   T nu_acc = nu0;
 
+  // tile of this thread (tiled form): rows 3 ti .., columns 3 tj .., ti <= tj
+  const bool tiled = D >= kTileMinD && D <= kTileMaxD;
+  const int nT = (int)(D + 2) / 3;
+  int ti = 0, tj = 0;
+  const bool tile_ok = tiled && tid < nT * (nT + 1) / 2;
+  if (tile_ok) {
+    int r = tid;
+    while (r >= nT - ti) { r -= nT - ti; ++ti; }
+    tj = ti + r;
+  }
+  // tile rows/columns clamped into [0, D) for the reads (entries outside the
+  // triangle are computed and not written)
+  int ir[3], jc[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    ir[u] = 3 * ti + u < D ? 3 * ti + u : (int)D - 1;
+    jc[u] = 3 * tj + u < D ? 3 * tj + u : (int)D - 1;
+  }
+
   // pair batches (a single batch for D <= 70)
   for (int64_t pbase = 0; pbase < P; pbase += (int64_t)kOuterThreads * kMaxPairs) {
     const int64_t Pb = (P - pbase) < (int64_t)kOuterThreads * kMaxPairs
@@ -85,7 +111,7 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
     T acc[kMaxPairs];
 #pragma unroll
     for (int q = 0; q < kMaxPairs; ++q) { acc[q] = T(0); pi[q] = 0; pj[q] = 0; }
-    if (active) {
+    if (active && !tiled) {
       int64_t p0 = Pb <= kOuterThreads ? (tid % Pb) : tid;
       for (int q = 0; q < kMaxPairs; ++q) {
         int64_t p = p0 + (int64_t)q * kOuterThreads;
@@ -139,7 +165,25 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
       // weights (first pair batch only)
       if (!GRAPH && nu_out != nullptr && pbase == 0 && tid == 0)
         for (int64_t e = 0; e < ne; ++e) nu_acc += wts[e];
-      if (active) {
+      if (tiled) {
+        if (tile_ok) {
+          for (int64_t e = 0; e < ne; ++e) {
+            const T w = wts[e];
+            const T *de = stage + e * D;
+            T wi[3], dj[3];
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+              wi[u] = w * de[ir[u]];
+              dj[u] = de[jc[u]];
+            }
+#pragma unroll
+            for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+              for (int b2 = 0; b2 < 3; ++b2)
+                acc[a2 * 3 + b2] = fma(wi[a2], dj[b2], acc[a2 * 3 + b2]);
+          }
+        }
+      } else if (active) {
         for (int64_t e = g; e < ne; e += G) {
           const T w = wts[e];
           const T *de = stage + e * D;
@@ -160,7 +204,20 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
         acc[0] = s;
       }
     }
-    if (g == 0 && active) {
+    if (tile_ok) {
+#pragma unroll
+      for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 3; ++b2) {
+          const int64_t i = 3 * ti + a2, j = 3 * tj + b2;
+          if (i < D && j < D && i <= j) {
+            T v = acc[a2 * 3 + b2];
+            if (GRAPH) v = v / T(K);
+            out[(n * D + i) * D + j] = v;
+            out[(n * D + j) * D + i] = v;
+          }
+        }
+    } else if (g == 0 && active && !tiled) {
       for (int q = 0; q < np; ++q) {
         T v = acc[q];
         if (GRAPH) v = v / T(K);
