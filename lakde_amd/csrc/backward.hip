@@ -69,8 +69,9 @@ __global__ void __launch_bounds__(256) sum_splits_kernel(const float *part, int6
 
 // One workgroup per kernel n: chunks of 64 test points; 4 threads per point
 // form z = Linv d (rows strided by 4) and Q; then every thread accumulates
-// its (i <= j) entries of S and, for j < D, s1.  Deterministic order.
-constexpr int kBwdPairs = 8;  // 256 * 8 >= 63 * 64 / 2
+// a 3 x 3 tile of the upper triangle of S (ceil(D/3) (ceil(D/3) + 1) / 2 <=
+// 231 tiles: 6 LDS reads per point for 9 products) and, for j < D, s1.
+// Deterministic order.
 
 __global__ void __launch_bounds__(256) bwd_kernel_stats(const float *__restrict__ rec, int D,
                                                        const float *__restrict__ Y, int64_t M,
@@ -95,23 +96,25 @@ __global__ void __launch_bounds__(256) bwd_kernel_stats(const float *__restrict_
   const float *xn = r + rec_x(D);
   const float coeff = r[rec_c(D) + 2];
   const float h = r[rec_c(D) + 3];
-  const int P = D * (D + 1) / 2;
-  int pi[kBwdPairs], pj[kBwdPairs];
-  float acc[kBwdPairs];
-  int np = 0;
-#pragma unroll
-  for (int q = 0; q < kBwdPairs; ++q) {
-    acc[q] = 0.f;
-    pi[q] = pj[q] = 0;
-    const int p = tid + q * 256;
-    if (p < P) {
-      int pp = p, i = 0;
-      while (pp >= D - i) { pp -= D - i; ++i; }
-      pi[q] = i;
-      pj[q] = i + pp;
-      np = q + 1;
-    }
+  // this thread's tile: rows 3 ti .., columns 3 tj .., ti <= tj; reads are
+  // clamped into [0, D) (entries outside the triangle are not written)
+  const int nT = (D + 2) / 3;
+  const bool tile_ok = tid < nT * (nT + 1) / 2;
+  int ti = 0, tj = 0;
+  if (tile_ok) {
+    int r = tid;
+    while (r >= nT - ti) { r -= nT - ti; ++ti; }
+    tj = ti + r;
   }
+  int ir[3], jc[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    ir[u] = 3 * ti + u < D ? 3 * ti + u : D - 1;
+    jc[u] = 3 * tj + u < D ? 3 * tj + u : D - 1;
+  }
+  float acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.f;
   float s1acc = 0.f, bacc = 0.f, tacc = 0.f;
   const int pt = tid >> 2, qt = tid & 3;
   for (int64_t m0 = 0; m0 < M; m0 += 64) {
@@ -146,16 +149,33 @@ __global__ void __launch_bounds__(256) bwd_kernel_stats(const float *__restrict_
     __syncthreads();
     for (int p = 0; p < 64; ++p) {
       const float a = al[p];
+      if (tile_ok) {
+        float wi[3], dj[3];
 #pragma unroll
-      for (int q = 0; q < kBwdPairs; ++q)
-        if (q < np) acc[q] = fmaf(a * dl[p][pi[q]], dl[p][pj[q]], acc[q]);
+        for (int u = 0; u < 3; ++u) {
+          wi[u] = a * dl[p][ir[u]];
+          dj[u] = dl[p][jc[u]];
+        }
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+          for (int b2 = 0; b2 < 3; ++b2) acc[a2 * 3 + b2] = fmaf(wi[a2], dj[b2], acc[a2 * 3 + b2]);
+      }
       if (tid < D) s1acc = fmaf(a, dl[p][tid], s1acc);
     }
   }
   const int64_t DD = (int64_t)D * D;
-  for (int q = 0; q < np; ++q) {
-    S[n * DD + pi[q] * D + pj[q]] = acc[q];
-    S[n * DD + pj[q] * D + pi[q]] = acc[q];
+  if (tile_ok) {
+#pragma unroll
+    for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+      for (int b2 = 0; b2 < 3; ++b2) {
+        const int i = 3 * ti + a2, j = 3 * tj + b2;
+        if (i < D && j < D && i <= j) {
+          S[n * DD + i * D + j] = acc[a2 * 3 + b2];
+          S[n * DD + j * D + i] = acc[a2 * 3 + b2];
+        }
+      }
   }
   if (tid < D) s1[n * D + tid] = s1acc;
   red[0][tid] = bacc;
