@@ -41,9 +41,11 @@ __global__ void __launch_bounds__(64) pack_records_kernel(const float *__restric
   const int64_t R = rec_floats(D);
   const float *L = Linv + n * (int64_t)D * D;
   float *r = rec + n * R;
+  // 8 loads in flight per lane
+#pragma unroll 8
   for (int t = lane; t < (int)rec_tri(D); t += 64) {
-    int i = 0, c = t;
-    while (c > i) { c -= i + 1; ++i; }
+    int i, c;
+    tri_index(t, &i, &c);
     r[t] = L[i * D + c];
   }
   if (lane < D) r[rec_x(D) + lane] = Xc[n * D + lane];

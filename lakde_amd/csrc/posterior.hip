@@ -42,16 +42,6 @@ __device__ __forceinline__ void load_batched(const float *__restrict__ src, int 
   }
 }
 
-// row i and column r of entry t of a row-packed lower triangle
-// (t = i (i + 1) / 2 + r, r <= i); the float estimate is corrected exactly
-__device__ __forceinline__ void tri_index(int t, int *i, int *r) {
-  int k = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
-  if ((k + 1) * (k + 2) / 2 <= t) ++k;
-  if (k * (k + 1) / 2 > t) --k;
-  *i = k;
-  *r = t - k * (k + 1) / 2;
-}
-
 // Lower Cholesky of M (+ jitter on the diagonal, added in FP32) into Lout.
 // Lane i owns row i.  Returns true on success (every pivot > 0).
 template <typename T>
@@ -139,6 +129,7 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
     if (!ok && lane == 0) *a.status = 1;
     if (lane == 0) a.prior_jitter[n] = j0;
     double ld0 = 0.0;
+#pragma unroll 4
     for (int t = lane; t < D * D; t += kWave) {
       int i = t / D, k = t % D;
       a.prior_tril[n * DD + t] = (k <= i) ? R2[i * ld + k] : 0.f;
@@ -189,6 +180,7 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   }
 
   // ---- outputs: Linv, logdet_W
+#pragma unroll 4
   for (int t = lane; t < D * D; t += kWave) {
     int i = t / D, k = t % D;
     a.Linv[n * DD + t] = (k <= i) ? Linv[i * ld + k] : 0.f;
@@ -225,6 +217,7 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   // ---- packed record
   const int64_t R = rec_floats(D);
   float *rec = a.rec + n * R;
+#pragma unroll 4
   for (int t = lane; t < (int)rec_tri(D); t += kWave) {
     int i, r;
     tri_index(t, &i, &r);

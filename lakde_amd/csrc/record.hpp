@@ -17,6 +17,16 @@ __device__ inline double digamma_d(double x) {
   return r + log(x) - 0.5 / x + t;
 }
 
+// row i and column r of entry t of a row-packed lower triangle
+// (t = i (i + 1) / 2 + r, r <= i); the float estimate is corrected exactly
+__device__ __forceinline__ void tri_index(int t, int *i, int *r) {
+  int k = (int)((sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+  if ((k + 1) * (k + 2) / 2 <= t) ++k;
+  if (k * (k + 1) / 2 > t) --k;
+  *i = k;
+  *r = t - k * (k + 1) / 2;
+}
+
 // cst[0..3] = E-step const, E-step exponent, predictive coefficient,
 // predictive exponent, in the float32 operation order of
 // lakde/local_adaptive_kde.py:36-44 (log rho) and :209-217 (predictive).
