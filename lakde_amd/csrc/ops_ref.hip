@@ -39,9 +39,10 @@ static int launch_ind2ptr(const void *ind, int64_t nnz, int64_t M, void *out, hi
 // ------------------------------------------------- segmented outer products
 // One workgroup (256 threads) per output row n.  The row's entries are staged
 // through LDS in chunks of E rows of d_e = Xm[col_e] - Xn[n] (coalesced row
-// gathers); each thread owns up to kMaxPairs (i <= j) output pairs and, when
-// the triangle is small, thread groups split the entries and are reduced in a
-// fixed order.  No atomics; the result is deterministic.
+// gathers); each thread owns a 3 x 3 tile of the upper triangle (23 <= D <=
+// 63) or up to kMaxPairs (i <= j) output pairs and, when the triangle is
+// small, thread groups split the entries and are reduced in a fixed order.
+// No atomics; the result is deterministic.
 //   GRAPH = false: CSR rows (rowptr, col, value weights)       -> W_inv
 //   GRAPH = true : dense K-regular graph, unit weights, / K    -> kNN prior
 constexpr int kOuterThreads = 256;
