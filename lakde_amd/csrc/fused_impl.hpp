@@ -511,16 +511,21 @@ __global__ void __launch_bounds__(kEThreads)
 bwd_y_kernel(const float *__restrict__ rec, const float *__restrict__ Y,
              const float *__restrict__ lse, const float *__restrict__ g, PParams p,
              float *__restrict__ part) {
-  __shared__ float sm_g[4][kWave][D];
+  // one buffer, two uses: the lane's point y (y[j] at ys[j * 256]) during
+  // the kernel loop, then the per-wave partial gradients for the fixed-order
+  // combine.  y is read back per kernel instead of held in registers, which
+  // leaves the difference, z and the gradient (3D floats) live per lane.
+  __shared__ float sm_buf[4 * kWave * D];
   const int lane = threadIdx.x & (kWave - 1);
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t m = (int64_t)blockIdx.x * kWave + lane;
   const int64_t split = blockIdx.y;
   const bool valid = m < p.M;
-  float ym[D], gy[D];
+  float *ys = sm_buf + threadIdx.x;
+  float gy[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    ym[j] = valid ? Y[m * D + j] : 0.f;
+    ys[j * kEThreads] = valid ? Y[m * D + j] : 0.f;
     gy[j] = 0.f;
   }
   const float l_m = valid ? lse[m] : 0.f;
@@ -532,10 +537,13 @@ bwd_y_kernel(const float *__restrict__ rec, const float *__restrict__ Y,
   const int64_t nend = (nbeg + q) < send ? (nbeg + q) : send;
   constexpr int64_t R = rec_floats(D);
   for (int64_t n = nbeg; n < nend; ++n) {
+    // compiler barrier: y is re-read from LDS for every kernel, not hoisted
+    // into registers for the whole loop
+    asm volatile("" ::: "memory");
     const float *r = rec + n * R;
     float d[D], z[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) d[j] = ym[j] - r[rec_x(D) + j];
+    for (int j = 0; j < D; ++j) d[j] = ys[j * kEThreads] - r[rec_x(D) + j];
     float Q = 0.f;
     int t = 0;
 #pragma unroll
@@ -560,14 +568,18 @@ bwd_y_kernel(const float *__restrict__ rec, const float *__restrict__ Y,
       t += i + 1;
     }
   }
+  __syncthreads();  // every wave is done with its y before the buffer is reused
+  float *gs = sm_buf + ((int64_t)w * kWave + lane) * D;
 #pragma unroll
-  for (int j = 0; j < D; ++j) sm_g[w][lane][j] = gy[j];
+  for (int j = 0; j < D; ++j) gs[j] = gy[j];
   __syncthreads();
   if (w == 0 && valid) {
+    const float *g0 = sm_buf + (int64_t)lane * D;
+    constexpr int64_t WS = (int64_t)kWave * D;  // one wave's partials
 #pragma unroll
     for (int j = 0; j < D; ++j)
       part[(split * p.M + m) * D + j] =
-          (sm_g[0][lane][j] + sm_g[1][lane][j]) + (sm_g[2][lane][j] + sm_g[3][lane][j]);
+          (g0[j] + g0[WS + j]) + (g0[2 * WS + j] + g0[3 * WS + j]);
   }
 }
 
