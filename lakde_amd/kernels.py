@@ -38,26 +38,34 @@ def ind2ptr(ind, M, out=None):
     return out
 
 
-def calc_W_inv(rowptr, col, value, Xn, Xm):
-    """out[n] = sum_e value[e] (Xm[col[e]] - Xn[n]) (...)^T over row n
-    (lakde/kernels.py:102-113)."""
+def _calc_W_inv_into(rowptr, col, value, Xn, Xm, out):
+    """Write the segmented outer-product sums into ``out`` (N, D, D) in place,
+    with no other allocation (csrc/cuda/W_inv_cuda.h:5-7 semantics)."""
     assert rowptr.is_contiguous()
     assert col.is_contiguous()
     assert value.is_contiguous()
     assert Xn.is_contiguous()
     assert Xm.is_contiguous()
-    require_gpu(rowptr, col, value, Xn, Xm)
+    require_gpu(rowptr, col, value, Xn, Xm, out)
     N, D = Xn.shape
     if rowptr.numel() != N + 1 or Xm.dim() != 2 or Xm.size(1) != D:
         raise ValueError("shape mismatch between rowptr, Xn and Xm")
     if not (value.dtype == Xn.dtype == Xm.dtype):
         raise TypeError("value, Xn and Xm must share one floating dtype")
-    out = value.new_empty(N, D, D)
+    if tuple(out.shape) != (N, D, D) or not out.is_contiguous() or out.dtype != value.dtype:
+        raise ValueError("out must be a contiguous (N, D, D) tensor of the value dtype")
     lib = _lib.load()
     check(lib.lakde_w_inv(ptr(rowptr), idx_code(rowptr.dtype), ptr(col), idx_code(col.dtype),
                           ptr(value), ptr(Xn), ptr(Xm), N, Xm.size(0), D, fp_code(value.dtype),
                           ptr(out), stream_of(value)), "calc_W_inv")
     return out
+
+
+def calc_W_inv(rowptr, col, value, Xn, Xm):
+    """out[n] = sum_e value[e] (Xm[col[e]] - Xn[n]) (...)^T over row n
+    (lakde/kernels.py:102-113)."""
+    N, D = Xn.shape
+    return _calc_W_inv_into(rowptr, col, value, Xn, Xm, value.new_empty(N, D, D))
 
 
 def _chol_quad_form_into(W_triu, Xn, Xm, out):
@@ -101,8 +109,7 @@ def _register_torch_ops():
         return ind2ptr(ind, M, out=out)
 
     def _w_inv_impl(rowptr, col, value, Xn, Xm, out):
-        out.copy_(calc_W_inv(rowptr, col, value, Xn, Xm))
-        return out
+        return _calc_W_inv_into(rowptr, col, value, Xn, Xm, out)
 
     def _qf_impl(W_triu, Xn, Xm, out):
         return _chol_quad_form_into(W_triu, Xn, Xm, out)

@@ -34,7 +34,7 @@ from . import _lib
 from ._lib import (ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE, LAKDE_ECAPACITY, LAKDE_ESTABILIZE,
                    LAKDE_OK, LakdeError, check, require_gpu, stream_of)
 from .kdes import AbstractKDE
-from .utils import column_order, tensor_as
+from .utils import InterruptDelay, column_order, tensor_as
 
 
 class LocallyAdaptiveKDE(AbstractKDE):
@@ -69,6 +69,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._active_device = None
         self._step_cache = {}
         self._estep_timer = None  # optional: .pair() -> two recorded torch events or None
+        self._count_steps = False
+        # partial steps per native sweep call in fit(): bounds the delay of
+        # a SIGINT (it takes effect between calls)
+        self.sweep_chunk = 256
 
     # ------------------------------------------------------------ helpers
     def _blocks(self, N):
@@ -307,11 +311,13 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._step_cache["built"][b] = True
         return a
 
-    def _run_steps(self, X, first, count, sparsity_threshold):
+    def _run_steps(self, X, first, count, sparsity_threshold, count_steps=False):
         """Partial steps of blocks first .. first+count-1: posterior,
         partial_expectation_step and rebalance_responsibilities_
         (lakde/local_adaptive_kde.py:66-181) through the native step driver,
-        with one host synchronisation per step."""
+        with one host synchronisation per step.  ``count_steps``: add the
+        completed steps to iter_steps with the rest of the bookkeeping."""
+        self._count_steps = count_steps
         lazy = self._lazy
         t = float(sparsity_threshold or 0.0)
         frag = self._frag_blk.data_ptr() if self.precision == "bf16" else None
@@ -330,40 +336,46 @@ class LocallyAdaptiveKDE(AbstractKDE):
         arena = self.arena
         b, end = first, first + count
         while b < end:
-            cbuf = (ctypes.c_void_p * 2)(self.rnm_log_consts.data_ptr(), self._c_new.data_ptr())
-            cur = ctypes.c_int(0)
-            dirty = ctypes.c_int(int(lazy is not None and lazy.dirty))
-            done = ctypes.c_int64(0)
-            rc = lib.lakde_partial_sweep(self._step_cache["arr"], b, end - b, cbuf,
-                                         ctypes.byref(cur), ctypes.byref(dirty),
-                                         ctypes.byref(done), st)
-            self._account(b, b + done.value, cur.value, dirty.value)
-            b += done.value
-            if rc == LAKDE_OK:
-                break
-            if rc == LAKDE_ESTABILIZE:
-                raise RuntimeError("Stabilization did not converge")
-            if rc != LAKDE_ECAPACITY:
-                check(rc, "partial_sweep")
-            # the block's entries exceed the arena: grow, rebuild the
-            # arguments (same per-step fields) and finish the step
-            old = self._step_cache["arr"][b]
-            keep = (old.estep.threshold, old.post.prior, old.bf16_frag, old.ev_main[0],
-                    old.ev_main[1], old.fold_first)
-            arena.grow(int(self._flags[b]))
-            self._step_cache = {}
-            for j in range(b, end):
-                a = self._step_args(X, j)
-                a.estep.threshold, a.post.prior, a.bf16_frag = keep[0], keep[1], keep[2]
-                if j == b:
-                    a.ev_main[0], a.ev_main[1], a.fold_first = keep[3], keep[4], keep[5]
-            a = self._step_cache["arr"][b]
-            a.estep.c_old = self.rnm_log_consts.data_ptr()
-            a.estep.c = self._c_new.data_ptr()
-            check(lib.lakde_partial_step_resume(ctypes.byref(a), st), "partial_step_resume")
-            unsafe = int(self._flags[arena.num_blocks + 1]) != 0
-            self._account(b, b + 1, 1, int(lazy is not None and not unsafe))
-            b += 1
+            # the native steps and the host bookkeeping that follows them
+            # (entry counts, normaliser buffer roles, lazy state) complete
+            # together: a SIGINT in between is delivered afterwards
+            # (lakde/local_knn_kde.py:224 wraps its state writes the same way)
+            with InterruptDelay():
+                cbuf = (ctypes.c_void_p * 2)(self.rnm_log_consts.data_ptr(),
+                                             self._c_new.data_ptr())
+                cur = ctypes.c_int(0)
+                dirty = ctypes.c_int(int(lazy is not None and lazy.dirty))
+                done = ctypes.c_int64(0)
+                rc = lib.lakde_partial_sweep(self._step_cache["arr"], b, end - b, cbuf,
+                                             ctypes.byref(cur), ctypes.byref(dirty),
+                                             ctypes.byref(done), st)
+                self._account(b, b + done.value, cur.value, dirty.value)
+                b += done.value
+                if rc == LAKDE_OK:
+                    break
+                if rc == LAKDE_ESTABILIZE:
+                    raise RuntimeError("Stabilization did not converge")
+                if rc != LAKDE_ECAPACITY:
+                    check(rc, "partial_sweep")
+                # the block's entries exceed the arena: grow, rebuild the
+                # arguments (same per-step fields) and finish the step
+                old = self._step_cache["arr"][b]
+                keep = (old.estep.threshold, old.post.prior, old.bf16_frag, old.ev_main[0],
+                        old.ev_main[1], old.fold_first)
+                arena.grow(int(self._flags[b]))
+                self._step_cache = {}
+                for j in range(b, end):
+                    a = self._step_args(X, j)
+                    a.estep.threshold, a.post.prior, a.bf16_frag = keep[0], keep[1], keep[2]
+                    if j == b:
+                        a.ev_main[0], a.ev_main[1], a.fold_first = keep[3], keep[4], keep[5]
+                a = self._step_cache["arr"][b]
+                a.estep.c_old = self.rnm_log_consts.data_ptr()
+                a.estep.c = self._c_new.data_ptr()
+                check(lib.lakde_partial_step_resume(ctypes.byref(a), st), "partial_step_resume")
+                unsafe = int(self._flags[arena.num_blocks + 1]) != 0
+                self._account(b, b + 1, 1, int(lazy is not None and not unsafe))
+                b += 1
 
     def _account(self, first, stop, cur, dirty):
         """Host bookkeeping after driver steps first .. stop-1: entry counts
@@ -378,6 +390,8 @@ class LocallyAdaptiveKDE(AbstractKDE):
             self.rnm_log_consts, self._c_new = self._c_new, self.rnm_log_consts
         if self._lazy is not None and stop > first:
             self._lazy.dirty = bool(dirty)
+        if self._count_steps:
+            self.iter_steps += stop - first
 
     def _partial_step(self, X, b, sparsity_threshold):
         self._run_steps(X, b, 1, sparsity_threshold)
@@ -412,8 +426,11 @@ class LocallyAdaptiveKDE(AbstractKDE):
 
     def fit(self, X, iterations=200, callbacks=None, sparsity_threshold=None, dataset_label=None):
         """lakde/kdes.py:34-87.  Without callbacks, and where no host work
-        separates the partial steps (LocalKNearestKDE), each VI iteration is
-        one call of the native sweep loop."""
+        separates the partial steps (LocalKNearestKDE), each VI iteration
+        runs in the native sweep loop, in calls of at most ``sweep_chunk``
+        partial steps: a KeyboardInterrupt (SIGINT) takes effect after the
+        call in progress, with the bookkeeping of its steps done, and
+        ``fit`` returns a consistent model, as the reference's does."""
         if callbacks is not None or not self._sweep_fusable():
             super().fit(X, iterations, callbacks, sparsity_threshold, dataset_label)
             self.materialize()
@@ -425,12 +442,16 @@ class LocallyAdaptiveKDE(AbstractKDE):
             self.init_parameters_(X, sparsity_threshold)
             self.init_summaries_(X, sparsity_threshold)
         num_blocks = (X.size(0) - 1) // self.block_size + 1
+        chunk = max(1, int(self.sweep_chunk))
         try:
             for _ in range(iterations):
-                self._run_steps(X, 0, num_blocks, sparsity_threshold)
-                self.iter_steps += num_blocks
+                for b0 in range(0, num_blocks, chunk):
+                    self._run_steps(X, b0, min(chunk, num_blocks - b0), sparsity_threshold,
+                                    count_steps=True)
         except (KeyboardInterrupt, StopIteration):
             pass
+        finally:
+            self._count_steps = False
         self.materialize()
         return self
 

@@ -15,7 +15,7 @@ import torch
 from ._lib import PRIOR_SHARED
 from .linalg_util import stabilized_cholesky, tril_inverse
 from .local_adaptive_kde import LocallyAdaptiveKDE
-from .utils import compute_knn_graph, cov, tensor_as
+from .utils import InterruptDelay, compute_knn_graph, cov, tensor_as
 
 
 class LocalHierarchicalKDE(LocallyAdaptiveKDE):
@@ -131,17 +131,20 @@ class LocalHierarchicalKDE(LocallyAdaptiveKDE):
     def partial_step_(self, X, block_idx, sparsity_threshold=None):
         N = X.size(0)
         s, e, nb = self._span(block_idx, N)
-        self._set_sigma_bar()
-        # b[s:e] <- b0 + nu0 * trace / 2 is written by the posterior kernel
-        # after E[tau] has been read; nothing in the E-step reads b
-        self._partial_step(X, block_idx, sparsity_threshold)
-        # nu, b and the factors are not touched by the E-step
-        self.expect_lambda_contribs[block_idx] = self._lambda_contrib(nb, self.nu[s:e],
-                                                                      self.b[s:e], s, e)
-        W_shared_inv = self.calc_prior_prec()
-        W_shared_inv_tril, self.W_shared_inv_jitter = stabilized_cholesky(
-            W_shared_inv, self.W_shared_inv_jitter)
-        self.W_shared_triu = tril_inverse(W_shared_inv_tril).T
+        # the step and the shared-prior update complete together; a SIGINT
+        # in between is delivered afterwards (lakde/local_hierarchical_kde.py:299)
+        with InterruptDelay():
+            self._set_sigma_bar()
+            # b[s:e] <- b0 + nu0 * trace / 2 is written by the posterior kernel
+            # after E[tau] has been read; nothing in the E-step reads b
+            self._partial_step(X, block_idx, sparsity_threshold)
+            # nu, b and the factors are not touched by the E-step
+            self.expect_lambda_contribs[block_idx] = self._lambda_contrib(
+                nb, self.nu[s:e], self.b[s:e], s, e)
+            W_shared_inv = self.calc_prior_prec()
+            W_shared_inv_tril, self.W_shared_inv_jitter = stabilized_cholesky(
+                W_shared_inv, self.W_shared_inv_jitter)
+            self.W_shared_triu = tril_inverse(W_shared_inv_tril).T
 
     def state_dict(self):
         d = super().state_dict()

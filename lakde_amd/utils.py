@@ -1,6 +1,46 @@
-"""Initialisation helpers (device-side torch glue, init only)."""
+"""Initialisation helpers (device-side torch glue, init only) and the SIGINT
+deferral around model state updates."""
+
+import signal
+import threading
 
 import torch
+
+
+class InterruptDelay:
+    """Defer SIGINT while a model's device and host state are updated together
+    (the semantics of lakde/utils.py:240-255): a SIGINT that arrives inside
+    the block is recorded, and when the block exits the previous handler
+    receives it (the default handler raises KeyboardInterrupt there, after
+    the bookkeeping).  Outside the main thread Python delivers no signals to
+    handlers, so the block is a no-op there."""
+
+    def __init__(self):
+        self.signal_received = None
+        self.old_handler = None
+
+    def __enter__(self):
+        self.signal_received = None
+        self.old_handler = None
+        if threading.current_thread() is threading.main_thread():
+            self.old_handler = signal.signal(signal.SIGINT, self._handler)
+        return self
+
+    def _handler(self, sig, frame):
+        self.signal_received = (sig, frame)
+
+    def __exit__(self, exc_type, exc, tb):
+        if self.old_handler is None:
+            return False
+        signal.signal(signal.SIGINT, self.old_handler)
+        got = self.signal_received
+        self.signal_received = None
+        if got is not None:
+            if callable(self.old_handler):
+                self.old_handler(*got)
+            elif self.old_handler == signal.SIG_DFL:
+                signal.raise_signal(signal.SIGINT)
+        return False
 
 
 def tensor_as(x, other):

@@ -139,16 +139,33 @@ def _fit_case(name, model_cls, X, Y, graph, block_size, threshold, iterations,
     # unrolled so the state after init and after every sweep can be recorded
     m.init_parameters_(Xt, threshold)
     m.init_summaries_(Xt, threshold)
-    traj = {"init_log_consts": _np(m.rnm_log_consts).copy(), "init_nu": _np(m.nu).copy()}
+    traj = {"init_log_consts": _np(m.rnm_log_consts).copy(), "init_nu": _np(m.nu).copy(),
+            "init_W_inv_jitter": _np(m.W_inv_jitter).copy()}
+    hier = hasattr(m, "W_shared_triu")
+    if hier:
+        traj["init_W_shared_triu"] = _np(m.W_shared_triu).copy()
     num_blocks = (Xt.size(0) - 1) // m.block_size + 1
-    lc, nus = [], []
+    lc, nus, jits, bs_, wst, wsj = [], [], [], [], [], []
     for _ in range(iterations):
         for bi in range(num_blocks):
             m.partial_step_(Xt, bi, sparsity_threshold=threshold)
             m.iter_steps += 1
         lc.append(_np(m.rnm_log_consts).copy())
         nus.append(_np(m.nu).copy())
+        jits.append(_np(m.W_inv_jitter).copy())
+        bs_.append(_np(m.b).copy())
+        if hier:
+            wst.append(_np(m.W_shared_triu).copy())
+            wsj.append(float(m.W_shared_inv_jitter))
     t_fit = time.time() - t0
+    traj["traj_W_inv_jitter"] = np.stack(jits) if jits else np.zeros((0, X.shape[0]), np.float32)
+    traj["traj_b"] = np.stack(bs_) if bs_ else np.zeros((0, X.shape[0]), np.float32)
+    if hier:
+        D = X.shape[1]
+        traj["traj_W_shared_triu"] = np.stack(wst) if wst else np.zeros((0, D, D))
+        traj["traj_W_shared_inv_jitter"] = np.asarray(wsj, np.float64)
+        print(f"  {name}: posterior jitters per sweep "
+              f"{[int(np.count_nonzero(j)) for j in jits]}, shared jitter {wsj}")
     t0 = time.time()
     lp = m.log_pred_density(Xt, Yt)
     t_pred = time.time() - t0
@@ -233,6 +250,65 @@ def case_hier_smooth_small():
     _fit_case("hier_smooth_small", lakde.LocalHierarchicalKDE, X, Y, g, 256, 1 / 24, 4)
 
 
+def _latent_smooth(n, m, dims):
+    """latent_d43 without its two integer columns, first ``dims`` columns."""
+    X, Y = synthetic.make("latent_d43", n, m)
+    return [np.ascontiguousarray(np.delete(A, [3, 17], axis=1)[:, :dims]) for A in (X, Y)]
+
+
+def _hier_jitter_counts(X, k, threshold, block_size, iterations):
+    """Posterior / shared-prior jitters the reference applies per sweep."""
+    Xt = torch.from_numpy(X)
+    m = lakde.LocalHierarchicalKDE(k_or_knn_graph=torch.from_numpy(_ref_graph(X, k, True)),
+                                   block_size=block_size, verbose=False, logs=False)
+    m.init_parameters_(Xt, threshold)
+    m.init_summaries_(Xt, threshold)
+    out = [int(torch.count_nonzero(m.W_inv_jitter))]
+    nb = (X.shape[0] - 1) // m.block_size + 1
+    for _ in range(iterations):
+        for bi in range(nb):
+            m.partial_step_(Xt, bi, sparsity_threshold=threshold)
+        out.append(int(torch.count_nonzero(m.W_inv_jitter)))
+    return out, float(m.W_shared_inv_jitter)
+
+
+def probe_hier_configs():
+    """How the jitter-free hierarchical fixture was chosen: the reference's
+    jitter counts (after init, then per sweep) on candidate configurations."""
+    for n, dims, k, t in [(1200, 41, 96, 1 / 96), (1200, 20, 64, 1 / 64), (1200, 12, 48, 1 / 48),
+                          (1000, 8, 32, 1 / 32), (1500, 20, 64, 0.0), (1200, 41, 128, 0.0)]:
+        X, _ = _latent_smooth(n, 10, dims)
+        jit, sj = _hier_jitter_counts(X, k, t, 512, 4)
+        print(f"  latent_smooth N={n} D={dims} k={k} t={t:.4g}: jitters {jit}, shared {sj}",
+              flush=True)
+    for name, n, k, t in [("circle8", 1000, 20, 1 / 20), ("circle8", 1000, 40, 0.0),
+                          ("mix64_d21", 1200, 64, 1 / 64), ("mix128_d63", 1500, 160, 1 / 160)]:
+        X, _ = synthetic.make(name, n, 10)
+        jit, sj = _hier_jitter_counts(X, k, t, 512, 4)
+        print(f"  {name} N={n} k={k} t={t:.4g}: jitters {jit}, shared {sj}", flush=True)
+    rng = np.random.default_rng(12)
+    for dims, n, k in [(4, 1000, 24), (10, 1200, 48), (6, 1200, 32), (6, 1200, 96),
+                       (8, 1200, 64)]:
+        X = rng.standard_normal((n, dims)).astype(np.float32)
+        jit, sj = _hier_jitter_counts(X, k, 1 / k, 512, 4)
+        print(f"  normal D={dims} N={n} k={k}: jitters {jit}, shared {sj}", flush=True)
+
+
+def case_hier_nojit():
+    # configurations on which the reference applies no posterior and no
+    # shared-prior jitter in any sweep (probe_hier_configs: every D >= 6
+    # candidate collapses some posteriors onto the jitter path by the second
+    # sweep), so the whole trajectory is pinned to float32 rounding
+    X, Y = synthetic.make("circle8", 1000, 200)
+    g = _ref_graph(X, 20, True)
+    _fit_case("hier_circle8_nojit", lakde.LocalHierarchicalKDE, X, Y, g, 512, 1 / 20, 4)
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((1000, 4)).astype(np.float32)
+    Y = rng.standard_normal((200, 4)).astype(np.float32)
+    g = _ref_graph(X, 24, True)
+    _fit_case("hier_normal4_nojit", lakde.LocalHierarchicalKDE, X, Y, g, 512, 0.0, 4)
+
+
 def case_knn_circle_full():
     w = synthetic.WORKLOADS["circle8"]
     X, Y = synthetic.make("circle8")
@@ -245,9 +321,16 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true", help="also run the 5k-point circle fit")
     ap.add_argument("--only", default=None)
+    ap.add_argument("--probe-hier", action="store_true",
+                    help="print the reference's jitter counts on candidate hierarchical "
+                         "configurations (writes nothing)")
     args = ap.parse_args()
+    if args.probe_hier:
+        probe_hier_configs()
+        sys.exit(0)
     cases = [case_ops, case_knn_circle_small, case_knn_mix6_small, case_knn_degenerate_small,
-             case_hier_latent_small, case_hier_mix6_small, case_hier_smooth_small]
+             case_hier_latent_small, case_hier_mix6_small, case_hier_smooth_small,
+             case_hier_nojit]
     if args.full:
         cases.append(case_knn_circle_full)
     for c in cases:

@@ -150,6 +150,23 @@ def test_torch_ops_registered(K):
     np.testing.assert_allclose(_np(q), _np(K.chol_quad_form(Wt, Xn, Xm)))
 
 
+def test_w_inv_op_writes_out_in_place(K, golden):
+    """torch.ops.lakde.W_inv_cuda writes the caller's ``out`` itself and
+    returns it (csrc/cuda/W_inv_cuda.h:5-7): no new tensor is copied in."""
+    g = golden("ops")
+    N, D = g["winv_Xn"].shape
+    out = torch.full((N, D, D), float("nan"), device=DEV)
+    addr = out.data_ptr()
+    ret = torch.ops.lakde.W_inv_cuda(_t(g["winv_rowptr"]), _t(g["winv_col"]), _t(g["winv_val"]),
+                                     _t(g["winv_Xn"]), _t(g["winv_Xm"]), out)
+    assert ret.data_ptr() == addr and out.data_ptr() == addr
+    np.testing.assert_allclose(_np(out), g["winv_S"], rtol=1e-5, atol=1e-6)
+    with pytest.raises((ValueError, RuntimeError)):
+        torch.ops.lakde.W_inv_cuda(_t(g["winv_rowptr"]), _t(g["winv_col"]), _t(g["winv_val"]),
+                                   _t(g["winv_Xn"]), _t(g["winv_Xm"]),
+                                   torch.empty(N, D, D + 1, device=DEV))
+
+
 # ------------------------------------------------------- fused pieces
 def test_knn_prior_cov_matches_oracle():
     from lakde_amd import _fused as F

@@ -200,23 +200,73 @@ def test_knn_fit_matches_reference(golden, name):
     np.testing.assert_allclose(vals, d["rnm_val"], rtol=1e-4, atol=1e-7)
 
 
-@pytest.mark.parametrize("name,init_rtol", [("hier_smooth_small", 1e-4),
-                                            ("hier_latent_small", 5e-3)])
-def test_hierarchical_fit_matches_reference(golden, name, init_rtol):
-    """The hierarchical model drives many posterior scale matrices to the
-    FP32 jitter threshold (the reference itself jitters ~1/3 of the points),
-    so independent FP32 implementations make different jitter decisions.
-    Pinned: the state after initialisation (1e-4 relative; 5e-3 for the
-    43-D data with integer columns, whose posterior scales are ill
-    conditioned already at initialisation) and the held-out mean log density
-    to 2e-3 relative."""
+def _check_hier_state(m, d, it, tag):
+    """One sweep's state of a hierarchical fit against the reference
+    fixture (it = -1: after initialisation): log normalisers and nu within
+    1e-4 (relative, 1e-4 absolute floor), b within 1e-4 relative, the
+    shared prior factor within 1e-5 of its largest entry, equal jitters."""
+    if it < 0:
+        lc, nu, jit, wst = (d["init_log_consts"], d["init_nu"], d["init_W_inv_jitter"],
+                            d["init_W_shared_triu"])
+        b = None
+    else:
+        lc, nu, jit, wst, b = (d["traj_log_consts"][it], d["traj_nu"][it],
+                               d["traj_W_inv_jitter"][it], d["traj_W_shared_triu"][it],
+                               d["traj_b"][it])
+    np.testing.assert_allclose(m.rnm_log_consts, lc, rtol=1e-4, atol=1e-4, err_msg=tag)
+    np.testing.assert_allclose(m.nu, nu, rtol=1e-4, err_msg=tag)
+    if b is not None:
+        np.testing.assert_allclose(m.b, b, rtol=1e-4, err_msg=tag)
+    np.testing.assert_array_equal(m.W_inv_jitter, jit, err_msg=tag)
+    np.testing.assert_allclose(m.W_shared_triu, wst, rtol=0, atol=1e-5 * np.abs(wst).max(),
+                               err_msg=tag)
+
+
+# Hierarchical fixtures: the sweeps through which the FP32 trajectory is
+# pinned to the reference (state after sweep i checked for i < pinned; -1 =
+# initialisation only), the tolerance of the state after initialisation, and
+# the bar of the held-out mean.  The hierarchical update amplifies float32
+# rounding: on hier_normal4_nojit (no jitter anywhere) the FP64 oracle itself
+# lands 4e-2 nats from the reference's normalisers after two sweeps, and on
+# the other three fixtures FP32 jitter decisions (or, for the two with
+# integer columns, ill-conditioned posteriors at initialisation) separate the
+# trajectories from sweep 0 or from initialisation on.
+HIER_PINS = {
+    # name: (sweeps pinned, init rtol, held-out mean bar)
+    "hier_circle8_nojit": (4, 1e-4, 1e-4),
+    "hier_normal4_nojit": (0, 1e-4, 1e-4),
+    "hier_smooth_small": (0, 1e-4, 2e-3),
+    "hier_latent_small": (-1, 5e-3, 2e-3),
+    "hier_mix6_small": (-1, 5e-3, 2e-3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(HIER_PINS))
+def test_hierarchical_fit_matches_reference(golden, name):
+    """The FP32 oracle's hierarchical fit against the reference sweep by
+    sweep (lakde/local_hierarchical_kde.py:198-311, the shared-prior update
+    via lakde/linalg_util.py:129-170): on the jitter-free, well-conditioned
+    hier_circle8_nojit every sweep's normalisers, nu, b and W_shared_triu;
+    elsewhere the sweeps listed in HIER_PINS, and the held-out mean."""
+    pinned, init_rtol, bar = HIER_PINS[name]
     d = golden(name)
     m = O.LocalHierarchicalKDE(k_or_knn_graph=d["graph"], block_size=int(d["block_size"]))
     X, t = d["X"], float(d["threshold"])
     m.init_parameters_(X, t)
     m.init_summaries_(X, t)
-    np.testing.assert_allclose(m.rnm_log_consts, d["init_log_consts"], rtol=init_rtol)
+    np.testing.assert_allclose(m.rnm_log_consts, d["init_log_consts"], rtol=init_rtol,
+                               atol=1e-4 if init_rtol <= 1e-4 else 0)
     np.testing.assert_allclose(m.nu, d["init_nu"], rtol=1e-5)
-    m.fit(X, iterations=int(d["iterations"]), sparsity_threshold=t)
+    np.testing.assert_allclose(m.W_shared_triu, d["init_W_shared_triu"], rtol=0,
+                               atol=1e-5 * np.abs(d["init_W_shared_triu"]).max())
+    nb = (X.shape[0] - 1) // m.block_size + 1
+    for it in range(int(d["iterations"])):
+        for b in range(nb):
+            m.partial_step_(X, b, sparsity_threshold=t)
+        if it < pinned:
+            _check_hier_state(m, d, it, f"{name} sweep {it}")
+    if pinned >= int(d["iterations"]):
+        assert not np.any(d["traj_W_inv_jitter"]) and not np.any(d["traj_W_shared_inv_jitter"])
     lp = m.log_pred_density(X, d["Y"])
-    assert abs(lp.mean() - d["log_pred"].mean()) <= 2e-3 * abs(d["log_pred"].mean())
+    rel = abs(lp.mean() - d["log_pred"].mean()) / abs(d["log_pred"].mean())
+    assert rel <= bar, rel
