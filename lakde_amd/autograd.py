@@ -10,11 +10,13 @@ log-sum-exp, nothing of size N x M is kept).
 
 ``log_pred_density(model, X, Y)`` builds the factors from a fitted model so
 that gradients flow to ``Y`` and, for LocalHierarchicalKDE, to the shared
-prior factor: the factors are the HIP posterior kernel's, and the chain
-W_inv = prior (nu0 E[tau]) + S_n -> Cholesky -> inverse is differentiated
-with the analytic Cholesky VJP in float64 (S_n, the responsibility scatter,
-comes from the HIP accumulation kernel and is held fixed, as are the
-persistent jitters).
+prior factor: the D x D chain W_inv = prior (nu0 E[tau]) + S_n -> Cholesky
+-> inverse runs in float64 torch (``hierarchical_factors``; S_n, the
+responsibility scatter, comes from the HIP accumulation kernel and is held
+fixed, as are the persistent jitters), and its factors feed the HIP forward
+and backward.  ``prior_scale_objective`` (LocalKNearestKDE, full scale)
+keeps the HIP posterior kernel's factors and differentiates the chain with
+the analytic Cholesky VJP in float64 (``FactorInverse``).
 """
 
 import torch
@@ -51,9 +53,15 @@ def _backward(rec, Y, out, g):
 
 
 class MixtureLogDensity(torch.autograd.Function):
+    """Forward and backward run in float32 on the HIP kernels; inputs of
+    another floating dtype (float64 factors of the differentiable
+    hierarchical chain) are rounded to float32 on the way in and their
+    gradients returned in their own dtype."""
+
     @staticmethod
     def forward(ctx, Y, Linv, Xc, nu):
         require_gpu(Y, Linv, Xc, nu)
+        ctx.dtypes = (Y.dtype, Linv.dtype, Xc.dtype, nu.dtype)
         Y = Y.detach().float().contiguous()
         Linv = Linv.detach().float().contiguous()
         Xc = Xc.detach().float().contiguous()
@@ -77,7 +85,8 @@ class MixtureLogDensity(torch.autograd.Function):
         if ctx.needs_input_grad[3]:
             h = (nu + 1) / 2
             dnu = 0.5 * (beta * (torch.digamma(h) - torch.digamma(h - D / 2)) - tau)
-        return (dY if ctx.needs_input_grad[0] else None), dLinv, dXc, dnu
+        grads = [(dY if ctx.needs_input_grad[0] else None), dLinv, dXc, dnu]
+        return tuple(None if gr is None else gr.to(dt) for gr, dt in zip(grads, ctx.dtypes))
 
 
 def mixture_log_density(Y, Linv, Xc, nu):
@@ -130,40 +139,98 @@ def _scatter(model, X, b):
     return S, nu
 
 
-def log_pred_density(model, X, Y, prior_factor=None):
+def log_pred_density(model, X, Y, prior_factor=None, keep=None):
     """Differentiable held-out log density of a fitted model.
 
     For LocalHierarchicalKDE, ``prior_factor`` (D x D, lower triangular,
     float64) is the shared prior factor W_shared_triu^T; pass a tensor with
     ``requires_grad=True`` to differentiate through it (default: the model's
-    current factor, no gradient).  For LocalKNearestKDE the kNN prior is data
-    and only Y (and the factors returned by ``posterior_factors``) carry
-    gradients.
+    current factor, no gradient).  ``keep`` restricts the mixture to a
+    subset of the kernels (see ``hierarchical_factors``; the normalisation
+    is then -log of the kept count).  For LocalKNearestKDE the kNN prior is
+    data and only Y (and the factors returned by ``posterior_factors``)
+    carry gradients.
     """
     require_gpu(X, Y)
-    N, D = X.shape
     from .local_hierarchical_kde import LocalHierarchicalKDE
     if isinstance(model, LocalHierarchicalKDE):
         if prior_factor is None:
             prior_factor = model.W_shared_triu.T
-        T = torch.sqrt(model.nu_shared) * prior_factor.T       # W_shared_triu scaled
-        sigma_bar = T @ T.T                                    # lakde/local_hierarchical_kde.py:269-271
-        Linv_val, nu = posterior_factors(model, X)
-        Linvs = []
+        Linv, nu = hierarchical_factors(model, X, prior_factor, keep)
+        Xk = X if keep is None else X[keep].contiguous()
+        return mixture_log_density(Y, Linv, Xk, nu)
+    if keep is not None:
+        raise ValueError("keep applies to LocalHierarchicalKDE only")
+    Linv, nu = posterior_factors(model, X)
+    return mixture_log_density(Y, Linv, X, nu)
+
+
+def hierarchical_factors(model, X, prior_factor, keep=None):
+    """(Linv, nu) of every kernel of a fitted LocalHierarchicalKDE as a
+    differentiable function of the shared prior factor P = W_shared_triu^T
+    (D x D lower, float64): the chain of lakde/local_hierarchical_kde.py:269-271
+    (Sigma_bar = nu_shared W_shared_triu W_shared_triu^T) and
+    lakde/local_adaptive_kde.py:83-96 (W_inv_n = Sigma_bar nu0 E[tau_n] + S_n
+    + j_n I, L_n = chol(W_inv_n), Linv_n = L_n^{-1}) evaluated in float64
+    torch on D x D matrices, so that its gradient w.r.t. P is that of the
+    float64 chain itself.  S_n and nu_n come from the HIP accumulation kernel
+    and are held fixed with E[tau_n] and the persistent jitters j_n (the
+    fitted values).  Linv is float64; the predictive kernels take it rounded
+    to float32.
+
+    ``keep`` (optional bool mask of N): only those kernels' factors are
+    formed and returned (with their nu), e.g. to leave out posteriors whose
+    float64 chain does not factor (``unfactorable``); without it, such a
+    posterior raises."""
+    N, D = X.shape
+    T = torch.sqrt(model.nu_shared.double()) * prior_factor.T
+    sigma_bar = T @ T.T
+    eye = torch.eye(D, dtype=torch.float64, device=X.device)
+    Linvs, nus = [], []
+    for b in model._blocks(N):
+        s, e, nb = model._span(b, N)
+        S, nu = _scatter(model, X, b)
+        W_inv = _hier_w_inv(model, sigma_bar, S, s, e, eye)
+        if keep is not None:
+            k = keep[s:e]
+            W_inv, nu = W_inv[k], nu[k]
+        L, info = torch.linalg.cholesky_ex(W_inv)
+        bad = int(torch.count_nonzero(info))
+        if bad:
+            raise RuntimeError(f"{bad} posterior scale matrices of block {b} do not factor "
+                               "in float64")
+        Linvs.append(torch.linalg.solve_triangular(L, eye.expand(W_inv.size(0), D, D),
+                                                   upper=False))
+        nus.append(nu)
+    return torch.cat(Linvs), torch.cat(nus)
+
+
+def _hier_w_inv(model, sigma_bar, S, s, e, eye):
+    scale = model._nu0 * (model.a[s:e] / model.b[s:e])  # float32, as the posterior kernel
+    W_inv = sigma_bar[None] * scale.double()[:, None, None] + S.double()
+    return W_inv + model.W_inv_jitter[s:e].double()[:, None, None] * eye
+
+
+def unfactorable(model, X, prior_factor=None):
+    """Bool mask of the kernels of a fitted LocalHierarchicalKDE whose float64
+    posterior chain (``hierarchical_factors``) does not factor: posteriors
+    that the float32 kernel factored only with a persistent jitter of the
+    size of float32 rounding can be indefinite when W_inv is formed in
+    float64."""
+    N, D = X.shape
+    if prior_factor is None:
+        prior_factor = model.W_shared_triu.T
+    with torch.no_grad():
+        T = torch.sqrt(model.nu_shared.double()) * prior_factor.T
+        sigma_bar = T @ T.T
+        eye = torch.eye(D, dtype=torch.float64, device=X.device)
+        out = []
         for b in model._blocks(N):
             s, e, nb = model._span(b, N)
             S, _ = _scatter(model, X, b)
-            scale = model._nu0 * (model.a[s:e] / model.b[s:e])
-            # W_inv as the posterior kernel forms it (plus the persistent
-            # jitter); its factor is the kernel's, its gradient analytic
-            W_inv = sigma_bar[None] * scale.double()[:, None, None] + S.double()
-            jit = model.W_inv_jitter[s:e].double()
-            W_inv = W_inv + torch.diag_embed(jit[:, None].expand(nb, D))
-            Linvs.append(FactorInverse.apply(W_inv, Linv_val[s:e]))
-        Linv = torch.cat(Linvs)
-    else:
-        Linv, nu = posterior_factors(model, X)
-    return mixture_log_density(Y, Linv, X, nu)
+            _, info = torch.linalg.cholesky_ex(_hier_w_inv(model, sigma_bar, S, s, e, eye))
+            out.append(info != 0)
+    return torch.cat(out)
 
 
 def prior_scale_objective(model, X, Y, log_scale=0.0, grad=True):
@@ -244,4 +311,5 @@ def posterior_factors(model, X):
 
 
 __all__ = ["MixtureLogDensity", "FactorInverse", "mixture_log_density", "log_pred_density",
-           "posterior_factors", "prior_scale_objective"]
+           "hierarchical_factors", "unfactorable", "posterior_factors",
+           "prior_scale_objective"]

@@ -146,6 +146,16 @@ class LocalHierarchicalKDE(LocallyAdaptiveKDE):
                 W_shared_inv, self.W_shared_inv_jitter)
             self.W_shared_triu = tril_inverse(W_shared_inv_tril).T
 
+    def log_pred_density(self, X, Y):
+        """lakde/local_adaptive_kde.py:200-223 with calc_prior_cov() of the
+        current shared prior factor (lakde/local_hierarchical_kde.py:269-271):
+        the posteriors are recomputed from the W_shared_triu of the last
+        shared-prior update, not from the prior the last partial step read."""
+        if self.W_shared_triu is None:
+            raise RuntimeError("fit the model first")
+        self._set_sigma_bar()
+        return super().log_pred_density(X, Y)
+
     def state_dict(self):
         d = super().state_dict()
         d.update({"k": self.k, "nu_shared_0": self.nu_shared_0, "lambda_0_triu": self.lambda_0_triu,

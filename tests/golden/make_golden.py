@@ -286,6 +286,14 @@ def probe_hier_configs():
         X, _ = synthetic.make(name, n, 10)
         jit, sj = _hier_jitter_counts(X, k, t, 512, 4)
         print(f"  {name} N={n} k={k} t={t:.4g}: jitters {jit}, shared {sj}", flush=True)
+    for n, k in [(3000, 256), (3000, 64), (3000, 32), (2000, 32), (6000, 32), (1200, 32),
+                 (1200, 64), (2000, 128)]:
+        X, _ = synthetic.make("latent_d43", n, 10)
+        try:
+            jit, sj = _hier_jitter_counts(X, k, 1 / k, 512, 2)
+            print(f"  latent_d43 N={n} k={k}: jitters {jit}, shared {sj}", flush=True)
+        except RuntimeError as e:
+            print(f"  latent_d43 N={n} k={k}: {e}", flush=True)
     rng = np.random.default_rng(12)
     for dims, n, k in [(4, 1000, 24), (10, 1200, 48), (6, 1200, 32), (6, 1200, 96),
                        (8, 1200, 64)]:
