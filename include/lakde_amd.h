@@ -390,6 +390,21 @@ int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, i
                        float *out, void *workspace, size_t workspace_bytes,
                        lakde_stream_t stream);
 
+/* lakde_pred_density without a workspace argument and without any device
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The per-split partial sums live in a library-owned
+ * file-scope device array of LAKDE_PRED_NOSCRATCH_MAX_BYTES bytes, one per
+ * GPU for the life of the process (its address is looked up once per device
+ * on the first call there).  Calls whose partial sums need more than that
+ * (lakde_pred_workspace(N, M) bytes: about 8 M bytes for M >= 4096, at most
+ * 2 MiB below) fail with LAKDE_EINVAL.  Same kernels and results as
+ * lakde_pred_density.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other threads) race. */
+#define LAKDE_PRED_NOSCRATCH_MAX_BYTES ((size_t)16 << 20)
+int lakde_pred_density_noscratch(const float *rec, int64_t N, const float *Y, int64_t M,
+                                 int64_t D, float *out, lakde_stream_t stream);
+
 /* k-nearest-neighbour graph for initialisation (lakde/utils.py:77-116,
  * _compute_knn_graph_generic with metric "l2"; called at init from
  * lakde/local_knn_kde.py:95 and lakde/local_hierarchical_kde.py:149).

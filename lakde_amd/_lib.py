@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LAKDE_AMD_LIB", os.path.join(_HERE, "liblakde_amd.so"))
 
 LAKDE_I32, LAKDE_I64, LAKDE_F32, LAKDE_F64 = 0, 1, 2, 3
-LAKDE_OK, LAKDE_ECAPACITY, LAKDE_ESTABILIZE = 0, 4, 5
+LAKDE_OK, LAKDE_EINVAL, LAKDE_ECAPACITY, LAKDE_ESTABILIZE = 0, 1, 4, 5
 ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE = 0, 1, 2
 PRIOR_KNN, PRIOR_SHARED = 0, 1
 
@@ -94,6 +94,7 @@ _SIGNATURES = {
                                    ctypes.POINTER(_i64), _vp]),
     "lakde_pred_workspace": (_sz, [_i64, _i64]),
     "lakde_pred_density": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "lakde_pred_density_noscratch": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp]),
     "lakde_fill_f32": (_int, [_vp, _i64, _f32, _vp]),
     "lakde_knn_graph_workspace": (_sz, [_i64, _i64]),
     "lakde_knn_graph": (_int, [_vp, _i64, _i64, _i64, _int, _vp, _int, _vp, _vp, _sz, _vp]),

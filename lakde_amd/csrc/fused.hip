@@ -3,6 +3,8 @@
 // split combine).  The D-templated kernels live in fused_impl.hpp.
 #include <cfloat>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "fused_impl.hpp"
 
@@ -668,6 +670,67 @@ int lakde_pred_density(const float *rec, int64_t N, const float *Y, int64_t M, i
   p.Y = Y;
   rc = dispatch_pred_main(D, p, as_stream(stream));
   if (rc) { set_error("lakde_pred_density: launch failed"); return rc; }
+  return pred_combine(p, stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------ predictive density, no allocation
+// The per-split partial (max, sum-exp) arrays of lakde_pred_density live in
+// this file-scope device array instead of a caller's workspace.  Each GPU
+// holds its own copy for the life of the process; host code never uses the
+// array's name as an address (on the host it is only a stand-in symbol) but
+// the per-device address that hipGetSymbolAddress returns.
+namespace lakde {
+
+constexpr size_t kPredScratchBytes = LAKDE_PRED_NOSCRATCH_MAX_BYTES;
+static __device__ __align__(256) unsigned char scratch[kPredScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// the current device's copy of `scratch` (looked up once per device)
+static int pred_scratch(void **out) {
+  int dev = 0;
+  LAKDE_HIP(hipGetDevice(&dev));
+  LAKDE_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    LAKDE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    LAKDE_CHECK_ARG(p != nullptr, "no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LAKDE_OK;
+}
+
+}  // namespace lakde
+
+extern "C" {
+
+int lakde_pred_density_noscratch(const float *rec, int64_t N, const float *Y, int64_t M,
+                                 int64_t D, float *out, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 1 && D <= 63, "D must be in [1, 63]");
+  LAKDE_CHECK_ARG(rec && Y && out && N >= 1 && M >= 0, "bad args");
+  const size_t need = lakde_pred_workspace(N, M);
+  if (need > kPredScratchBytes) {
+    set_error("lakde_pred_density_noscratch: " + std::to_string(need) +
+              " bytes of partial sums needed, the static scratch holds " +
+              std::to_string(kPredScratchBytes) + " (use lakde_pred_density with a workspace)");
+    return LAKDE_EINVAL;
+  }
+  if (M == 0) return LAKDE_OK;
+  void *ws = nullptr;
+  int rc = pred_scratch(&ws);
+  if (rc) return rc;
+  PParams p;
+  rc = pred_fill_params(rec, N, M, out, ws, kPredScratchBytes, &p);
+  if (rc) return rc;
+  p.Y = Y;
+  rc = dispatch_pred_main(D, p, as_stream(stream));
+  if (rc) { set_error("lakde_pred_density_noscratch: launch failed"); return rc; }
   return pred_combine(p, stream);
 }
 
