@@ -357,6 +357,13 @@ typedef struct {
    * does nothing when the word is 1, and is enqueued again after the host
    * has handled the step.  NULL: no posterior is enqueued ahead. */
   int32_t *abort_flag;
+  /* optional (with bf16_frag of lakde_screen_frag_bytes(D) per kernel):
+   * the screening E-step (lakde_pack_screen + lakde_estep_main_screen) with
+   * the points of lakde_screen_points and its origin; NULL: bf16_frag holds
+   * lakde_pack_frag_bf16 blocks for lakde_estep_main_bf16 */
+  const void *screen_xb;
+  const float *screen_xnorm;
+  const float *screen_origin;
 } lakde_step_args;
 
 /* A 256-byte host-coherent, device-mapped buffer (hipHostMalloc,
@@ -438,6 +445,29 @@ int lakde_pack_frag_bf16(const float *rec, int64_t nb, int64_t D, void *frag,
  * exactly in FP32. */
 int lakde_estep_main_bf16(const lakde_estep_args *args, const void *frag,
                           lakde_stream_t stream);
+
+/* Screening E-step (the default precision form; 8 <= D <= 63).  Every pair
+ * is screened on the BF16 matrix cores in the centre form
+ * bf16(Linv_n) bf16(x_m - o) - Linv_n (x_n - o) (FP32 accumulate, the centre
+ * term in FP32, o a fixed origin such as the data mean) with a rigorous
+ * bound on its error; the pairs that can matter to the normaliser or the
+ * keep test (about 0.5% at D = 63) are recomputed exactly in FP32, and the
+ * log-sum-exp, normaliser update and keep test run on those exact values.
+ * Outputs as lakde_estep_main (follow with lakde_estep_finish).
+ *   lakde_screen_points: xb (lakde_screen_points_bytes(N, D) bytes) =
+ *     bf16(X - o) zero padded per point, xnorm (N floats) = |x - o| rounded
+ *     up; once per fit (X and o fixed).
+ *   lakde_pack_screen: per kernel of a block (lakde_screen_frag_bytes(D)
+ *     bytes each) from its records and o.
+ * (lakde/local_adaptive_kde.py:36-44,98-153 as lakde_estep_main.) */
+int64_t lakde_screen_frag_bytes(int64_t D);
+size_t lakde_screen_points_bytes(int64_t N, int64_t D);
+int lakde_screen_points(const float *X, int64_t N, int64_t D, const float *origin, void *xb,
+                        float *xnorm, lakde_stream_t stream);
+int lakde_pack_screen(const float *rec, int64_t nb, int64_t D, const float *origin, void *frag,
+                      lakde_stream_t stream);
+int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, const void *xb,
+                            const float *xnorm, lakde_stream_t stream);
 
 /* lakde_pred_density with the quadratic forms on the matrix cores
  * (frag: N fragment blocks; Y: M x D float32 test points). */

@@ -166,10 +166,12 @@ def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=
 
 
 def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
-          arena=None, b=None, compact_only=False, bf16=None, order=None):
-    """One E-step launch sequence.  ``bf16=frag`` (the block's fragment
-    blocks) runs the main (quadratic form / LSE / ballot) kernel on the
-    matrix cores."""
+          arena=None, b=None, compact_only=False, bf16=None, order=None, screen=None):
+    """One E-step launch sequence.  ``bf16=frag`` (the block's
+    lakde_pack_frag_bf16 blocks) runs the main (quadratic form / LSE /
+    ballot) kernel on the matrix cores in bf16x3; ``screen=(frag, points)``
+    (the block's lakde_pack_screen blocks and the fit's ScreenPoints) runs
+    the screening E-step."""
     args = estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old, old_colsum, arena, b,
                       order)
     lib = _lib.load()
@@ -177,11 +179,41 @@ def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
     if compact_only:
         check(lib.lakde_estep_compact(args, st), "estep_compact")
         return
-    if bf16 is not None:
+    if screen is not None:
+        frag, pts = screen
+        check(lib.lakde_estep_main_screen(args, ptr(frag), ptr(pts.xb), ptr(pts.xnorm), st),
+              "estep_main_screen")
+    elif bf16 is not None:
         check(lib.lakde_estep_main_bf16(args, ptr(bf16), st), "estep_main_bf16")
     else:
         check(lib.lakde_estep_main(args, st), "estep_main")
     check(lib.lakde_estep_finish(args, st), "estep_finish")
+
+
+def screen_frag_bytes(D):
+    return int(_lib.load().lakde_screen_frag_bytes(D))
+
+
+class ScreenPoints:
+    """The screening E-step's per-fit operands: the origin o (the mean of
+    X), bf16(x_m - o) zero padded per point and |x_m - o| rounded up."""
+
+    def __init__(self, X):
+        N, D = X.shape
+        lib = _lib.load()
+        self.origin = X.mean(0).contiguous()
+        self.xb = torch.empty(max(int(lib.lakde_screen_points_bytes(N, D)), 1), dtype=torch.uint8,
+                              device=X.device)
+        self.xnorm = torch.empty(N, device=X.device)
+        check(lib.lakde_screen_points(ptr(X), N, D, ptr(self.origin), ptr(self.xb),
+                                      ptr(self.xnorm), stream_of(X)), "screen_points")
+
+
+def pack_screen(rec, D, origin, out):
+    nb = rec.size(0)
+    check(_lib.load().lakde_pack_screen(ptr(rec), nb, D, ptr(origin), ptr(out), stream_of(rec)),
+          "pack_screen")
+    return out
 
 
 def rebalance(arena, exclude_block, c_prev, c, ws):

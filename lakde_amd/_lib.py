@@ -56,6 +56,7 @@ class StepArgs(ctypes.Structure):
         ("G", _vp), ("U", _vp), ("U_old", _vp), ("lazy_ws", _vp), ("rebal_ws", _vp),
         ("fold_first", _int), ("bf16_frag", _vp), ("host_flags", _vp), ("ev_main", _vp * 2),
         ("ev_flags", _vp), ("flag_words", _vp), ("abort_flag", _vp),
+        ("screen_xb", _vp), ("screen_xnorm", _vp), ("screen_origin", _vp),
     ]
 
 
@@ -102,6 +103,11 @@ _SIGNATURES = {
     "lakde_pack_frag_bf16": (_int, [_vp, _i64, _i64, _vp, _vp]),
     "lakde_estep_main_bf16": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp]),
     "lakde_pred_density_bf16": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "lakde_screen_frag_bytes": (_i64, [_i64]),
+    "lakde_screen_points_bytes": (_sz, [_i64, _i64]),
+    "lakde_screen_points": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "lakde_pack_screen": (_int, [_vp, _i64, _i64, _vp, _vp, _vp]),
+    "lakde_estep_main_screen": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp, _vp, _vp]),
     "lakde_pack_records": (_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp]),
     "lakde_pred_backward_workspace": (_sz, [_i64, _i64, _i64]),
     "lakde_pred_backward": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -41,7 +41,8 @@ D_GROUPS = [(1, 6), (7, 12), (13, 18), (19, 24), (25, 32), (33, 40), (41, 48), (
 
 def _gen_inst(lo, hi):
     path = os.path.join(BUILD, f"fused_d{lo}_{hi}.hip")
-    lines = ['#include "%s"' % os.path.join(CSRC, "fused_impl.hpp"), "namespace lakde {"]
+    lines = ['#include "%s"' % os.path.join(CSRC, "fused_impl.hpp"),
+             '#include "%s"' % os.path.join(CSRC, "screen.hpp"), "namespace lakde {"]
     for d in range(lo, hi + 1):
         lines += [
             f"template int launch_estep_main<{d}>(const EParams &, int, hipStream_t);",
@@ -49,6 +50,12 @@ def _gen_inst(lo, hi):
             f"template int launch_pred_main<{d}>(const PParams &, hipStream_t);",
             f"template int launch_bwd_y<{d}>(const PParams &, const float *, const float *, "
             "float *, hipStream_t);",
+            f"template int launch_estep_screen<{d}>(const EParams &, const SParams &, int, "
+            "hipStream_t);",
+            f"template int launch_pack_screen<{d}>(const float *, int64_t, const float *, "
+            "uint8_t *, hipStream_t);",
+            f"template int launch_screen_points<{d}>(const float *, int64_t, const float *, "
+            "__bf16 *, float *, hipStream_t);",
         ]
     lines.append("}")
     text = "\n".join(lines) + "\n"
@@ -59,13 +66,17 @@ def _gen_inst(lo, hi):
 
 
 # per-source flags: the matrix-core kernels keep their MFMA accumulators in
-# ordinary VGPRs (the epilogue's squares read them without v_accvgpr_read)
-EXTRA_FLAGS = {"bf16.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+# ordinary VGPRs (the epilogue's squares read them without v_accvgpr_read);
+# the dimension-instantiated units hold the screening E-step's MFMA kernels
+MFMA_VGPR = ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
+EXTRA_FLAGS = {"bf16.hip": MFMA_VGPR}
 
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC] + CFLAGS + EXTRA_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
+    name = os.path.basename(src)
+    extra = EXTRA_FLAGS.get(name, MFMA_VGPR if name.startswith("fused_d") else [])
+    cmd = [HIPCC] + CFLAGS + extra + ["-c", src, "-o", obj]
     t = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

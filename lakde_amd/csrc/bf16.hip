@@ -17,8 +17,11 @@
 // C holds 16 floats per lane, column = lane % 32, row = 8 * (r / 4) +
 // 4 * (lane / 32) + r % 4 for register r.
 #include "fused_impl.hpp"
+#include "screen.hpp"
 
 namespace lakde {
+
+LAKDE_FOR_EACH_D(LAKDE_DECL_SCREEN_D)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -409,6 +412,66 @@ int lakde_estep_main_bf16(const lakde_estep_args *args, const void *frag,
 #undef LAUNCH
   LAKDE_LAUNCHED();
   return LAKDE_OK;
+}
+
+// ------------------------------------------------ screening E-step (screen.hpp)
+int64_t lakde_screen_frag_bytes(int64_t D) {
+  return (D >= 8 && D <= 63) ? sc_frag_bytes((int)D) : -1;
+}
+
+size_t lakde_screen_points_bytes(int64_t N, int64_t D) {
+  return (D >= 8 && D <= 63 && N >= 0) ? (size_t)N * sc_kp((int)D) * 2 : 0;
+}
+
+int lakde_screen_points(const float *X, int64_t N, int64_t D, const float *origin, void *xb,
+                        float *xnorm, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 8 && D <= 63, "the screening form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(X && origin && xb && xnorm && N >= 0, "bad args");
+  hipStream_t s = as_stream(stream);
+  int rc = LAKDE_EINVAL;
+  switch (D) {
+#define CASE(d) case d: rc = launch_screen_points<d>(X, N, origin, (__bf16 *)xb, xnorm, s); break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) set_error("lakde_screen_points: launch failed");
+  return rc;
+}
+
+int lakde_pack_screen(const float *rec, int64_t nb, int64_t D, const float *origin, void *frag,
+                      lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 8 && D <= 63, "the screening form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(rec && origin && frag && nb >= 0, "bad args");
+  hipStream_t s = as_stream(stream);
+  int rc = LAKDE_EINVAL;
+  switch (D) {
+#define CASE(d) case d: rc = launch_pack_screen<d>(rec, nb, origin, (uint8_t *)frag, s); break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) set_error("lakde_pack_screen: launch failed");
+  return rc;
+}
+
+int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, const void *xb,
+                            const float *xnorm, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr && frag && xb && xnorm, "null args");
+  LAKDE_CHECK_ARG(args->D >= 8 && args->D <= 63, "the screening form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(args->nb <= 32767, "the screening E-step takes blocks of at most 32767 kernels");
+  LAKDE_CHECK_ARG(args->col_perm == nullptr, "the screening E-step takes columns in index order");
+  EParams p;
+  int rc = estep_fill_params(*args, &p);
+  if (rc) return rc;
+  SParams sp{(const uint8_t *)frag, (const __bf16 *)xb, xnorm, p.skip_margin};
+  hipStream_t s = as_stream(stream);
+  rc = LAKDE_EINVAL;
+  switch (args->D) {
+#define CASE(d) case d: rc = launch_estep_screen<d>(p, sp, args->mode, s); break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) set_error("lakde_estep_main_screen: launch failed");
+  return rc;
 }
 
 int lakde_pred_density_bf16(const float *rec, const void *frag, int64_t N, const float *Y,

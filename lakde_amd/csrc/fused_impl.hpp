@@ -434,8 +434,10 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
     uint64_t v[MB];
 #pragma unroll
     for (int u = 0; u < MB; ++u) v[u] = (cb + u < c1) ? p.masks[(cb + u) * p.nb + n] : 0ull;
-    if (p.col_perm != nullptr) {
-      // the next skipping E-step ORs into the masks: leave them zero
+    {
+      // the skipping and screening E-steps OR into the masks: leave them
+      // zero (every nonzero mask lies in a segment with kept entries, so
+      // the masks are all zero after every compaction)
 #pragma unroll
       for (int u = 0; u < MB; ++u)
         if (v[u] != 0ull) p.masks[(cb + u) * p.nb + n] = 0ull;

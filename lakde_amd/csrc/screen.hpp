@@ -1,0 +1,481 @@
+// Precision form of the E-step (BF16 matrix cores + exact FP32 refinement).
+//
+// Every (kernel n, point m) pair of a block is first screened on the BF16
+// matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulate) in the centre
+// form
+//     y~ = bf16(U_n) bf16(x_m - o) - c_n,   c_n = U_n (x_n - o) in FP32,
+// with U_n = Linv_n, o a fixed origin (the data mean), c_n loaded as the
+// accumulators' initial value and bf16(x_m - o) prepared once per fit.  With
+// u = 2^-8 (bf16 round to nearest), F_n = |U_n|_F, a_m = |x_m - o| and
+// b_n = |x_n - o| (all rounded up), the error of every row of y~ is bounded
+// by (2u + u^2 + FP32 accumulation) sum_j |U_ij| |x_mj - o_j| plus the FP32
+// rounding of c_n, so by Cauchy-Schwarz
+//     | y~ - U_n (x_m - x_n) |_2 <= E_nm = ka F_n a_m + kb F_n b_n
+// (ka = 0.00785 >= 2u + u^2 + 65 * 2^-24 (1 + u)^2, kb = 1.2e-5).  Hence
+// the exact log rho = const_n - nu_n/2 Q lies between
+//     lo = const_n - nu_n/2 (sqrt Q~ + E)^2   and   hi = const_n - nu_n/2 max(sqrt Q~ - E, 0)^2
+// (each widened by the FP32 rounding of Q~ and of the bound itself).
+//
+// A pair whose upper bound lies more than `margin` nats below B, a running
+// lower bound of its column's new normaliser (max of the old-mass bound lb
+// and of the pairs' lower bounds lo), contributes less than nb e^-margin of
+// the normaliser and cannot be kept (margin = the pair-skipping E-step's,
+// >= 30 nats, >= 8 - log threshold): it is dropped.  The others (about 0.5%
+// of the pairs at D = 63, a few per column and block) are queued per lane in
+// LDS; after the kernel loop each wave compacts its lanes' queues and
+// recomputes the queued pairs exactly in FP32, one pair per lane, with
+// log_rho<D> -- the function the compaction uses, so kept entries carry the
+// same bits there.  The log-sum-exp, the normaliser update and the keep
+// test then run on exact values only: the result differs from the FP32
+// E-step only by the dropped pairs (< 5e-11 relative) and summation order.
+//
+// A lane whose queue fills up resumes in another pass over the kernels
+// (block-uniform, rare); a lane with more keep candidates than its keep
+// queue holds is finished by a keep pass (the FIXED-mode pass: threshold
+// c_m + log t).  Kept pairs set their bits in the ballot masks with atomics
+// (the masks read zero on entry; the compaction clears what it reads).
+//
+// MFMA layouts (v_mfma_f32_32x32x16_bf16): A and B fragments hold 8 bf16
+// per lane, row/column = lane % 32, K = 16 s + 8 (lane / 32) + [0, 8); C
+// holds 16 floats per lane, column = lane % 32, row = 8 (r / 4) +
+// 4 (lane / 32) + r % 4 for register r.
+#pragma once
+
+#include "fused_impl.hpp"
+
+namespace lakde {
+
+typedef __bf16 sc_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float sc_f32x16 __attribute__((ext_vector_type(16)));
+typedef float sc_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t sc_u32x4 __attribute__((ext_vector_type(4)));
+
+// error-bound coefficients (see above)
+constexpr float kScreenKa = 0.00785f;
+constexpr float kScreenKb = 1.2e-5f;
+constexpr int kScreenQ = 16;   // survivor queue entries per lane
+constexpr int kKeepQ = 4;      // keep-candidate queue entries per lane
+
+__host__ __device__ constexpr int sc_kp(int D) { return (D + 15) / 16 * 16; }
+__host__ __device__ constexpr int sc_dp(int D) { return (D + 31) / 32 * 32; }
+// fragment block per kernel: U hi [Dp x Kp] bf16, -c_n [Dp] f32, scal[4]:
+// ka F_n, kb F_n b_n, E-step const, E-step exponent (-nu/2)
+__host__ __device__ constexpr int64_t sc_frag_bytes(int D) {
+  return (int64_t)sc_dp(D) * sc_kp(D) * 2 + (int64_t)sc_dp(D) * 4 + 16;
+}
+
+// Parameters of the screening launches beyond EParams.
+struct SParams {
+  const uint8_t *frag;   // nb fragment blocks
+  const __bf16 *xb;      // N x Kp: bf16(x_m - o), zero padded
+  const float *xnorm;    // N: |x_m - o| rounded up
+  float margin;          // screening margin (nats)
+};
+
+// ------------------------------------------------------------- preparation
+// bf16(x_m - o) (zero padded to Kp) and |x_m - o| rounded up, per point.
+template <int D>
+__global__ void __launch_bounds__(256) screen_points_kernel(const float *__restrict__ X, int64_t N,
+                                                           const float *__restrict__ origin,
+                                                           __bf16 *__restrict__ xb,
+                                                           float *__restrict__ xnorm) {
+  constexpr int Kp = sc_kp(D);
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < Kp; ++j) {
+    float v = 0.f;
+    if (j < D) {
+      v = X[m * D + j] - origin[j];
+      s += (double)v * (double)v;
+    }
+    xb[m * Kp + j] = (__bf16)v;
+  }
+  xnorm[m] = (float)(sqrt(s) * (1.0 + 1e-6));
+}
+
+// Fragment blocks of nb kernels from their FP32 records: one wave per kernel.
+template <int D>
+__global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict__ rec,
+                                                         const float *__restrict__ origin,
+                                                         uint8_t *__restrict__ frag) {
+  constexpr int Kp = sc_kp(D), Dp = sc_dp(D);
+  constexpr int64_t R = rec_floats(D);
+  const int64_t n = blockIdx.x;
+  const int lane = threadIdx.x;
+  const float *r = rec + n * R;
+  uint8_t *f = frag + n * sc_frag_bytes(D);
+  __bf16 *uh = reinterpret_cast<__bf16 *>(f);
+  float *cneg = reinterpret_cast<float *>(uh + Dp * Kp);
+  float *scal = cneg + Dp;
+  for (int t = lane; t < Dp * Kp; t += 64) {
+    const int i = t / Kp, j = t - i * Kp;
+    const float v = (i < D && j <= i) ? r[i * (i + 1) / 2 + j] : 0.f;
+    uh[t] = (__bf16)v;
+  }
+  // c_n row by row (lane = row), |U|_F^2 and |x_n - o|^2 in double
+  double f2 = 0.0, b2 = 0.0;
+  float c = 0.f;
+  if (lane < D) {
+    const float *row = r + lane * (lane + 1) / 2;
+    for (int j = 0; j <= lane; ++j) {
+      const float dj = r[rec_x(D) + j] - origin[j];
+      c = fmaf(row[j], dj, c);
+      f2 += (double)row[j] * (double)row[j];
+    }
+    const float dl = r[rec_x(D) + lane] - origin[lane];
+    b2 = (double)dl * (double)dl;
+  }
+  if (lane < Dp) cneg[lane] = lane < D ? -c : 0.f;
+  f2 = wave_sum(f2);
+  b2 = wave_sum(b2);
+  if (lane == 0) {
+    const double F = sqrt(f2) * (1.0 + 1e-6), b = sqrt(b2) * (1.0 + 1e-6);
+    scal[0] = (float)((double)kScreenKa * F * (1.0 + 1e-6));
+    scal[1] = (float)((double)kScreenKb * F * b * (1.0 + 1e-6));
+    scal[2] = r[rec_c(D) + 0];
+    scal[3] = r[rec_c(D) + 1];
+  }
+}
+
+// ---------------------------------------------------------------- kernel
+// LDS image of one fragment block: U hi rows padded to KpL bf16 (16 extra
+// bytes per row, so that the 32 rows a K-step reads start in different
+// banks), then -c_n and scal.
+template <int D>
+struct ScreenLds {
+  static constexpr int Kp = sc_kp(D), Dp = sc_dp(D), KpL = Kp + 8;
+  static constexpr int kUBytes = Dp * KpL * 2;
+  static constexpr int kBytes = kUBytes + Dp * 4 + 16;
+  static constexpr int kGranU = Dp * Kp / 8;           // 16-byte granules of U in global
+  static constexpr int kGran = (int)(sc_frag_bytes(D) / 16);
+  static constexpr int kPer = (kGran + 255) / 256;     // granules per thread
+};
+
+template <int D>
+struct ScreenStage {
+  using L = ScreenLds<D>;
+  sc_u32x4 g[L::kPer];
+  __device__ __forceinline__ void load(const uint8_t *__restrict__ src) {
+    const sc_u32x4 *s = reinterpret_cast<const sc_u32x4 *>(src);
+#pragma unroll
+    for (int i = 0; i < L::kPer; ++i) {
+      const int q = (int)threadIdx.x + 256 * i;
+      if (q < L::kGran) g[i] = s[q];
+    }
+  }
+  __device__ __forceinline__ void store(uint8_t *dst) const {
+#pragma unroll
+    for (int i = 0; i < L::kPer; ++i) {
+      const int q = (int)threadIdx.x + 256 * i;
+      if (q < L::kGran) {
+        int off;
+        if (q < L::kGranU) {
+          const int row = q / (L::Kp / 8), cg = q - row * (L::Kp / 8);
+          off = (row * L::KpL + cg * 8) * 2;
+        } else {
+          off = L::kUBytes + (q - L::kGranU) * 16;
+        }
+        *reinterpret_cast<sc_u32x4 *>(dst + off) = g[i];
+      }
+    }
+  }
+};
+
+// per-wave flat pair list of the exact phase: entry j holds (kernel << 8 |
+// lane) until lane j % 64 has read it, then the pair's exact log rho
+struct ScreenWaveList {
+  int32_t ent[kScreenQ * kWave];
+};
+
+template <int D, int MODE>
+__global__ void __launch_bounds__(256)
+estep_screen_kernel(EParams p, SParams sp) {
+  using L = ScreenLds<D>;
+  constexpr int Kp = L::Kp, KpL = L::KpL;
+  constexpr int RT = sc_dp(D) / 32, KS = Kp / 16;
+  constexpr int64_t R = rec_floats(D);
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2][L::kBytes];
+  __shared__ int16_t sq[kScreenQ][256];
+  __shared__ ScreenWaveList wl[4];
+  __shared__ int16_t kq_n[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
+  __shared__ float kq_x[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int64_t base = (int64_t)blockIdx.x * 256 + w * kWave;  // the wave's 64 columns = one chunk
+  const int64_t m = base + lane;
+  const bool valid = m < p.N;
+  const int64_t chunk = base / kWave;
+  const int64_t nb = p.nb;
+  const int64_t diag_n = m - p.row0;
+
+  // this lane's B fragments (both column tiles) and column norm
+  sc_bf16x8 xf[2][KS];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    const int64_t col = base + ct * 32 + (lane & 31);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      if (col < p.N)
+        xf[ct][s] = *reinterpret_cast<const sc_bf16x8 *>(sp.xb + col * Kp + s * 16 + 8 * (lane >> 5));
+      else
+        xf[ct][s] = __builtin_bit_cast(sc_bf16x8, (sc_u32x4){0u, 0u, 0u, 0u});
+    }
+  }
+  const float am = valid ? sp.xnorm[m] : 0.f;
+
+  float lb = neg_inf();   // UPDATE: bound of the normaliser from the old mass
+  float cm = 0.f;         // FIXED: the given normaliser; later the new one
+  if (MODE == LAKDE_ESTEP_UPDATE && valid) {
+    const float co = p.c_old[m];
+    lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
+  }
+  if (MODE == LAKDE_ESTEP_FIXED && valid) cm = p.c[m];
+  if (MODE == LAKDE_ESTEP_ACCUM && valid) cm = p.c[m];
+  // running lower bound of the new normaliser (ACCUM: the new normaliser
+  // logaddexp(c, LSE) is at least c)
+  float B = MODE == LAKDE_ESTEP_ACCUM ? cm : lb;
+  Lse lse;
+  lse.init();
+  int kcnt = 0;           // UPDATE: keep candidates queued (may exceed kKeepQ)
+
+  // One pass over the block's kernels: screen, queue survivors from kernel
+  // `resume` on (at most kScreenQ; the first kernel not queued is returned
+  // in next_resume, nb when all were), then compute the queued pairs
+  // exactly.  keep_pass: survivors are the pairs that can pass the keep
+  // test against the final normaliser cm.  Returns the number queued.
+  auto pass = [&](bool keep_pass, int64_t resume, int64_t &next_resume) -> int {
+    int cnt = 0;
+    next_resume = nb;
+    ScreenStage<D> st;
+    st.load(sp.frag);
+    st.store(lds[0]);
+    __syncthreads();
+    for (int64_t n = 0; n < nb; ++n) {
+      st.load(sp.frag + (n + 1 < nb ? n + 1 : n) * sc_frag_bytes(D));
+      const uint8_t *fb = lds[n & 1];
+      const __bf16 *U = reinterpret_cast<const __bf16 *>(fb);
+      const float *cneg = reinterpret_cast<const float *>(fb + L::kUBytes);
+      const float *scal = cneg + sc_dp(D);
+      sc_f32x16 acc[RT][2];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        sc_f32x16 ci;
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const sc_f32x4 v =
+              *reinterpret_cast<const sc_f32x4 *>(cneg + rt * 32 + 8 * q4 + 4 * (lane >> 5));
+          ci[4 * q4 + 0] = v[0];
+          ci[4 * q4 + 1] = v[1];
+          ci[4 * q4 + 2] = v[2];
+          ci[4 * q4 + 3] = v[3];
+        }
+        acc[rt][0] = ci;
+        acc[rt][1] = ci;
+      }
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          if (s * 16 > rt * 32 + 31) continue;  // U is lower triangular
+          const sc_bf16x8 a = *reinterpret_cast<const sc_bf16x8 *>(
+              U + (rt * 32 + (lane & 31)) * KpL + s * 16 + 8 * (lane >> 5));
+          acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[0][s], acc[rt][0], 0, 0, 0);
+          acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[1][s], acc[rt][1], 0, 0, 0);
+        }
+      }
+      float q0 = 0.f, q1 = 0.f;
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          q0 = fmaf(acc[rt][0][r], acc[rt][0][r], q0);
+          q1 = fmaf(acc[rt][1][r], acc[rt][1][r], q1);
+        }
+      const bool upper = lane >= 32;
+      const float recv = __shfl_xor(upper ? q0 : q1, 32, kWave);
+      const float qt = (upper ? q1 : q0) + recv;  // Q~ of this lane's column
+      // bounds of the exact log rho
+      const float E = fmaf(scal[0], am, scal[1]);
+      const float sqt = sqrtf(qt);
+      const float rl = fmaf(sqt, 1.f - 4e-6f, -E);
+      const float qlb = rl > 0.f ? rl * rl * (1.f - 1e-5f) : 0.f;
+      const float rh = fmaf(sqt, 1.f + 4e-6f, E);
+      const float cst = scal[2], expo = scal[3];
+      const float slack = fmaf(1e-4f, fabsf(cst), 0.01f);
+      float hi = fmaf(expo, qlb, cst) + slack;
+      float lo = fmaf(expo, rh * rh * (1.f + 1e-5f), cst) - slack;
+      if (!valid || n == diag_n) {
+        hi = neg_inf();
+        lo = neg_inf();
+      }
+      bool surv;
+      if (keep_pass) {
+        surv = hi > cm + p.log_t_lo;
+      } else {
+        B = fmaxf(B, lo);
+        surv = !(hi <= B - sp.margin);  // NaN bounds survive (and propagate)
+        if (!valid || n == diag_n) surv = false;
+      }
+      if (surv && n >= resume && next_resume == nb) {
+        if (cnt < kScreenQ) {
+          sq[cnt][tid] = (int16_t)n;
+          ++cnt;
+        } else {
+          next_resume = n;
+        }
+      }
+      st.store(lds[(n + 1) & 1]);
+      __syncthreads();
+    }
+    // exact phase (wave-local): compact the lanes' queues into one list,
+    // one pair per lane, in kernel order per lane
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int t = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += t;
+    }
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, kWave - 1, kWave);
+    ScreenWaveList &L2 = wl[w];
+    for (int k = 0; k < cnt; ++k) L2.ent[excl + k] = ((int32_t)sq[k][tid] << 8) | lane;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (int j0 = 0; j0 < total; j0 += kWave) {
+      const int j = j0 + lane;
+      if (j < total) {
+        const int32_t e = L2.ent[j];
+        const int64_t pn = e >> 8;
+        const int64_t pm = base + (e & 63);
+        float xv[D];
+#pragma unroll
+        for (int t = 0; t < D; ++t) xv[t] = p.X[pm * D + t];
+        L2.ent[j] = __float_as_int(log_rho<D>(p.rec + pn * R, xv));
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return excl | (cnt << 16);
+  };
+
+  // ------------------------------------------------ log-sum-exp passes
+  if (MODE != LAKDE_ESTEP_FIXED) {
+    int64_t resume = 0;
+    while (true) {
+      int64_t next = nb;
+      const int r = pass(false, resume, next);
+      const int excl = r & 0xffff, cnt = r >> 16;
+      const ScreenWaveList &L2 = wl[w];
+      for (int k = 0; k < cnt; ++k) lse.add1(__int_as_float(L2.ent[excl + k]));
+      if (MODE == LAKDE_ESTEP_UPDATE) {
+        // keep candidates: exact values that can pass the keep test against
+        // any normaliser >= max(B, the largest exact value)
+        const float Bk = fmaxf(B, lse.m);
+        for (int k = 0; k < cnt; ++k) {
+          const float x = __int_as_float(L2.ent[excl + k]);
+          if (__fsub_rn(x, Bk) > p.log_t_lo) {
+            if (kcnt < kKeepQ) {
+              kq_n[kcnt][tid] = sq[k][tid];
+              kq_x[kcnt][tid] = x;
+            }
+            ++kcnt;
+          }
+        }
+      }
+      resume = next;
+      if (!__syncthreads_or(next < nb)) break;
+    }
+    const float l = lse.value();
+    if (valid) {
+      cm = MODE == LAKDE_ESTEP_ACCUM ? logaddexp_f(cm, l) : logaddexp_f(lb, l);
+      p.c[m] = cm;
+    }
+    if (MODE == LAKDE_ESTEP_UPDATE && p.lazy_unsafe != nullptr)
+      lazy_unsafe_test(p, valid, m, valid ? p.c_old[m] : 0.f, cm);
+  }
+  if (MODE == LAKDE_ESTEP_ACCUM) return;
+  const int64_t cseg = chunk / p.seg_len;
+
+  auto keep = [&](int64_t n, float x) {
+    const float d = __fsub_rn(x, cm);
+    float r;
+    if (keep_resp(d, p.threshold, p.log_t_lo, &r)) {
+      atomicOr((unsigned long long *)&p.masks[chunk * nb + n], 1ull << lane);
+      atomicAdd(&p.seg_tot[cseg * nb + n], 1);
+      atomicAdd(&p.row_tot[n], 1);
+    }
+  };
+
+  if (MODE == LAKDE_ESTEP_UPDATE && !__syncthreads_or(kcnt > kKeepQ)) {
+    if (valid)
+      for (int k = 0; k < kcnt; ++k) keep(kq_n[k][tid], kq_x[k][tid]);
+    return;
+  }
+  // keep passes (FIXED mode, or an UPDATE block with a full keep queue)
+  int64_t resume = 0;
+  while (true) {
+    int64_t next = nb;
+    const int r = pass(true, resume, next);
+    const int excl = r & 0xffff, cnt = r >> 16;
+    const ScreenWaveList &L2 = wl[w];
+    if (valid)
+      for (int k = 0; k < cnt; ++k) keep(sq[k][tid], __int_as_float(L2.ent[excl + k]));
+    resume = next;
+    if (!__syncthreads_or(next < nb)) break;
+  }
+}
+
+template <int D>
+int launch_estep_screen(const EParams &p, const SParams &sp, int mode, hipStream_t s) {
+  if constexpr (D < 8) {
+    return LAKDE_EINVAL;
+  } else {
+    const dim3 grid((unsigned)ceil_div(p.N, 256));
+    if (mode == LAKDE_ESTEP_ACCUM)
+      hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_ACCUM>), grid, dim3(256), 0, s, p, sp);
+    else if (mode == LAKDE_ESTEP_FIXED)
+      hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_FIXED>), grid, dim3(256), 0, s, p, sp);
+    else
+      hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_UPDATE>), grid, dim3(256), 0, s, p, sp);
+    return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+  }
+}
+
+template <int D>
+int launch_pack_screen(const float *rec, int64_t nb, const float *origin, uint8_t *frag,
+                       hipStream_t s) {
+  if constexpr (D < 8) {
+    return LAKDE_EINVAL;
+  } else {
+    if (nb == 0) return LAKDE_OK;
+    hipLaunchKernelGGL((pack_screen_kernel<D>), dim3((unsigned)nb), dim3(64), 0, s, rec, origin,
+                       frag);
+    return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+  }
+}
+
+template <int D>
+int launch_screen_points(const float *X, int64_t N, const float *origin, __bf16 *xb,
+                         float *xnorm, hipStream_t s) {
+  if constexpr (D < 8) {
+    return LAKDE_EINVAL;
+  } else {
+    if (N == 0) return LAKDE_OK;
+    hipLaunchKernelGGL((screen_points_kernel<D>), dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s,
+                       X, N, origin, xb, xnorm);
+    return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+  }
+}
+
+#define LAKDE_DECL_SCREEN_D(d)                                                                \
+  extern template int launch_estep_screen<d>(const EParams &, const SParams &, int, hipStream_t); \
+  extern template int launch_pack_screen<d>(const float *, int64_t, const float *, uint8_t *,  \
+                                            hipStream_t);                                       \
+  extern template int launch_screen_points<d>(const float *, int64_t, const float *, __bf16 *,  \
+                                              float *, hipStream_t);
+
+}  // namespace lakde

@@ -164,7 +164,9 @@ int step_begin(const lakde_step_args *a, hipStream_t s, const int32_t *abort) {
     if (rc) return rc;
   }
   if (a->bf16_frag != nullptr) {
-    rc = lakde_pack_frag_bf16(e.rec, nb, D, a->bf16_frag, s);
+    rc = a->screen_xb != nullptr
+             ? lakde_pack_screen(e.rec, nb, D, a->screen_origin, a->bf16_frag, s)
+             : lakde_pack_frag_bf16(e.rec, nb, D, a->bf16_frag, s);
     if (rc) return rc;
   }
   return LAKDE_OK;
@@ -184,8 +186,12 @@ int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
     em.lazy_unsafe = unsafe;
   }
   if (a->ev_main[0]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[0], s));
-  rc = a->bf16_frag != nullptr ? lakde_estep_main_bf16(&em, a->bf16_frag, s)
-                               : lakde_estep_main(&em, s);
+  if (a->bf16_frag != nullptr && a->screen_xb != nullptr)
+    rc = lakde_estep_main_screen(&em, a->bf16_frag, a->screen_xb, a->screen_xnorm, s);
+  else if (a->bf16_frag != nullptr)
+    rc = lakde_estep_main_bf16(&em, a->bf16_frag, s);
+  else
+    rc = lakde_estep_main(&em, s);
   if (rc) return rc;
   if (a->ev_main[1]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[1], s));
   // offsets first: the flags the host needs are final before the compaction;

@@ -38,9 +38,11 @@ from .utils import InterruptDelay, column_order, tensor_as
 
 
 class LocallyAdaptiveKDE(AbstractKDE):
-    """``precision``: "fp32" (VALU quadratic forms) or "bf16" (the precision
-    form: quadratic forms on the BF16 matrix cores with bf16x3 split
-    precision, kept responsibilities recomputed exactly in FP32; 8 <= D <= 63).
+    """``precision``: "fp32" (FP32 quadratic forms for every pair) or "bf16"
+    (the precision form, 8 <= D <= 63: the E-step screens every pair on the
+    BF16 matrix cores with a rigorous error bound and recomputes exactly in
+    FP32 the pairs that can matter to a normaliser or a kept responsibility;
+    the predictive density runs bf16x3 split precision on the matrix cores).
     """
 
     def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32",
@@ -137,10 +139,13 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._col_order = None
         if self.estep_skip and self.precision == "fp32" and D <= 16:
             self._col_order = column_order(X)
+        self._screen = None
         if self.precision == "bf16":
             if not 8 <= D <= 63:
                 raise ValueError("the bf16 precision form needs 8 <= D <= 63")
-            self._fb = F.bf16_frag_bytes(D)
+            # screening E-step: per-fit points, per-block fragments
+            self._screen = F.ScreenPoints(X)
+            self._fb = F.screen_frag_bytes(D)
             self._frag_blk = torch.empty(bs * self._fb, dtype=torch.uint8, device=dev)
 
     def _init_arena(self, X, graph, sparsity_threshold):
@@ -230,7 +235,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         s, e, nb = self._span(b, N)
         c = self.rnm_log_consts
         F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, arena=self.arena, b=b,
-                bf16=self._bf16_operands(b, N), order=self._col_order)
+                screen=self._screen_operands(b, N), order=self._col_order)
         nnz = self._sync(b)
         if nnz > self.arena.cap:
             self.arena.grow(nnz)
@@ -241,13 +246,15 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self.arena.nnz[b] = nnz
         return old, nnz
 
-    def _bf16_operands(self, b, N):
+    def _screen_operands(self, b, N):
+        """(fragments of block b, the fit's screening points) for the
+        screening E-step, or None for the FP32 path."""
         if self.precision != "bf16":
             return None
         s, e, nb = self._span(b, N)
         frag = self._frag_blk[:nb * self._fb]
-        F.pack_frag_bf16(self.rec[s:e], self.rec_D, frag)
-        return frag
+        F.pack_screen(self.rec[s:e], self.rec_D, self._screen.origin, frag)
+        return frag, self._screen
 
     # ------------------------------------------------------------- E-step
     def _init_lse_pass(self, X, b):
@@ -255,7 +262,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         N = X.size(0)
         s, e, nb = self._span(b, N)
         F.estep(X, self.rec[s:e], s, ESTEP_ACCUM, 0.0, self.rnm_log_consts, self._ews,
-                bf16=self._bf16_operands(b, N))
+                screen=self._screen_operands(b, N))
 
     def _init_fixed_pass(self, X, b, sparsity_threshold):
         """partial_expectation_step(update_normalization=False)."""
@@ -320,12 +327,17 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._count_steps = count_steps
         lazy = self._lazy
         t = float(sparsity_threshold or 0.0)
-        frag = self._frag_blk.data_ptr() if self.precision == "bf16" else None
+        bf16 = self.precision == "bf16"
+        frag = self._frag_blk.data_ptr() if bf16 else None
+        sc = self._screen if bf16 else None
         for b in range(first, first + count):
             a = self._step_args(X, b)
             a.estep.threshold = t
             a.post.prior = self._prior_ptr(b, a.post.prior)
             a.bf16_frag = frag
+            a.screen_xb = sc.xb.data_ptr() if sc is not None else None
+            a.screen_xnorm = sc.xnorm.data_ptr() if sc is not None else None
+            a.screen_origin = sc.origin.data_ptr() if sc is not None else None
             ev = self._estep_timer.pair() if self._estep_timer is not None else None
             if ev is not None:
                 a.ev_main[0], a.ev_main[1] = ev[0].cuda_event, ev[1].cuda_event
@@ -362,11 +374,13 @@ class LocallyAdaptiveKDE(AbstractKDE):
                 old = self._step_cache["arr"][b]
                 keep = (old.estep.threshold, old.post.prior, old.bf16_frag, old.ev_main[0],
                         old.ev_main[1], old.fold_first)
+                keep_sc = (old.screen_xb, old.screen_xnorm, old.screen_origin)
                 arena.grow(int(self._flags[b]))
                 self._step_cache = {}
                 for j in range(b, end):
                     a = self._step_args(X, j)
                     a.estep.threshold, a.post.prior, a.bf16_frag = keep[0], keep[1], keep[2]
+                    a.screen_xb, a.screen_xnorm, a.screen_origin = keep_sc
                     if j == b:
                         a.ev_main[0], a.ev_main[1], a.fold_first = keep[3], keep[4], keep[5]
                 a = self._step_cache["arr"][b]
@@ -476,7 +490,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._sync(None)
         out = torch.empty(Y.size(0), device=Y.device)
         if self.precision == "bf16":
-            frag = torch.empty(N * self._fb, dtype=torch.uint8, device=Y.device)
+            frag = torch.empty(N * F.bf16_frag_bytes(D), dtype=torch.uint8, device=Y.device)
             F.pack_frag_bf16(self.rec, D, frag)
             return F.pred_density_bf16(self.rec, frag, Y, out)
         return F.pred_density(self.rec, Y, out)

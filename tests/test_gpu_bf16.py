@@ -63,6 +63,74 @@ def test_bf16_estep_lse_matches_fp32(name, n, m, k, tol):
         assert d.mean() < 0.5
 
 
+def _screen_case(name, n, k):
+    from lakde_amd import synthetic
+    if name == "normal8":
+        rng = np.random.default_rng(8)
+        X = rng.standard_normal((n, 8)).astype(np.float32)
+        Y = rng.standard_normal((100, 8)).astype(np.float32)
+        return X, Y
+    return synthetic.make(name, n, 100)
+
+
+@pytest.mark.parametrize("name,n,k,t", [("mix64_d21", 3000, 32, 1 / 32),
+                                        ("mix128_d63", 3000, 96, 1 / 96),
+                                        ("latent_d43", 2000, 32, 1 / 32),
+                                        ("normal8", 2500, 64, 0.0)])
+def test_screen_estep_matches_fp32_kernels(name, n, k, t):
+    """The screening E-step (BF16 matrix-core screen with a rigorous error
+    bound, exact FP32 recompute of the surviving pairs) against the FP32
+    E-step kernel on the same records: the ACCUM normalisers within float32
+    summation-order differences (1e-6 relative), and the FIXED-mode CSR of
+    the block identical entry for entry (the keep decisions of both run on
+    the same exact FP32 values).  normal8 with threshold 0 keeps every
+    nonzero responsibility (110-nat margin): its columns queue more pairs
+    than a lane holds and more keep candidates than the keep queue, so the
+    resume passes and the keep pass run."""
+    from lakde_amd import _fused as F
+    from lakde_amd._lib import ESTEP_ACCUM, ESTEP_FIXED
+    X, Y = _screen_case(name, n, k)
+    graph = O.compute_knn_graph(X, k)
+    model, _ = _fit(X, Y, graph, "fp32", 512, t if t else 1 / k, 1)
+    Xd = _t(X)
+    N, D = X.shape
+    pts = F.ScreenPoints(Xd)
+    for b in (0, (N - 1) // 512):
+        s, e = b * 512, min(N, b * 512 + 512)
+        nb = e - s
+        print(f"[{name}] block {b}: rows {s}..{e} D={D} threshold={t}", flush=True)
+        rec = model.rec[s:e]
+        frag = torch.empty(nb * F.screen_frag_bytes(D), dtype=torch.uint8, device=DEV)
+        F.pack_screen(rec, D, pts.origin, frag)
+        ws = F.EstepWorkspace(N, nb, Xd.device)
+        c32 = torch.full((N,), float("-inf"), device=DEV)
+        csc = c32.clone()
+        # start from the model's normalisers minus a little, so that the
+        # ACCUM update is a real combination of both terms
+        base = model.rnm_log_consts - 3.0
+        c32.copy_(base)
+        csc.copy_(base)
+        F.estep(Xd, rec, s, ESTEP_ACCUM, 0.0, c32, ws)
+        F.estep(Xd, rec, s, ESTEP_ACCUM, 0.0, csc, ws, screen=(frag, pts))
+        np.testing.assert_allclose(csc.cpu().numpy(), c32.cpu().numpy(), rtol=1e-6, atol=1e-5)
+        # FIXED: same normaliser in, identical CSR out
+        outs = []
+        for use_screen in (False, True):
+            arena = F.Arena(1, nb, 1 << 21, Xd.device)
+            F.estep(Xd, rec, s, ESTEP_FIXED, t, c32, ws, arena=arena, b=0,
+                    screen=(frag, pts) if use_screen else None)
+            cnt = int(arena.nnz_dev[0])
+            assert cnt <= arena.cap
+            outs.append((arena.rowptr[0, :nb + 1].cpu().numpy(), arena.col[0, :cnt].cpu().numpy(),
+                         arena.val[0, :cnt].cpu().numpy(), arena.row[0, :cnt].cpu().numpy()))
+        (p0, c0, v0, r0), (p1, c1, v1, r1) = outs
+        print(f"[{name}] block {b}: {c0.size} kept entries", flush=True)
+        np.testing.assert_array_equal(p1, p0)
+        np.testing.assert_array_equal(r1, r0)
+        np.testing.assert_array_equal(c1, c0)
+        np.testing.assert_array_equal(v1, v0)
+
+
 @pytest.mark.parametrize("name,n,m,k,iters", [("mix64_d21", 4000, 400, 32, 2),
                                               ("mix128_d63", 2000, 200, 96, 2)])
 def test_bf16_fit_matches_fp32_and_fp64(name, n, m, k, iters):
