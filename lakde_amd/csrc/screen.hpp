@@ -140,42 +140,52 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
 }
 
 // ---------------------------------------------------------------- kernel
-// LDS image of one fragment block: U hi rows padded to KpL bf16 (16 extra
-// bytes per row, so that the 32 rows a K-step reads start in different
-// banks), then -c_n and scal.
+// LDS image of one kernel's fragment block: U hi rows padded to KpL bf16 (16
+// extra bytes per row, so that the 32 rows a K-step reads start in
+// different banks), then -c_n and scal.  A stage holds kScreenKps kernels.
+constexpr int kScreenKps = 2;
+
 template <int D>
 struct ScreenLds {
   static constexpr int Kp = sc_kp(D), Dp = sc_dp(D), KpL = Kp + 8;
   static constexpr int kUBytes = Dp * KpL * 2;
-  static constexpr int kBytes = kUBytes + Dp * 4 + 16;
-  static constexpr int kGranU = Dp * Kp / 8;           // 16-byte granules of U in global
-  static constexpr int kGran = (int)(sc_frag_bytes(D) / 16);
-  static constexpr int kPer = (kGran + 255) / 256;     // granules per thread
+  static constexpr int kOne = kUBytes + Dp * 4 + 16;               // one kernel in LDS
+  static constexpr int kBytes = kScreenKps * kOne;
+  static constexpr int kGranU = Dp * Kp / 8;                       // 16-byte granules of U
+  static constexpr int kGranOne = (int)(sc_frag_bytes(D) / 16);    // ... of a fragment block
+  static constexpr int kGran = kScreenKps * kGranOne;
+  static constexpr int kPer = (kGran + 255) / 256;                 // granules per thread
 };
 
+// Cooperative copy of kScreenKps consecutive fragment blocks (contiguous in
+// global memory) into one LDS stage: every load issued into registers
+// before the current stage's work, the stores after it.
 template <int D>
 struct ScreenStage {
   using L = ScreenLds<D>;
   sc_u32x4 g[L::kPer];
-  __device__ __forceinline__ void load(const uint8_t *__restrict__ src) {
+  __device__ __forceinline__ void load(const uint8_t *__restrict__ src, int kernels) {
     const sc_u32x4 *s = reinterpret_cast<const sc_u32x4 *>(src);
+    const int lim = kernels * L::kGranOne;
 #pragma unroll
     for (int i = 0; i < L::kPer; ++i) {
       const int q = (int)threadIdx.x + 256 * i;
-      if (q < L::kGran) g[i] = s[q];
+      if (q < lim) g[i] = s[q];
     }
   }
-  __device__ __forceinline__ void store(uint8_t *dst) const {
+  __device__ __forceinline__ void store(uint8_t *dst, int kernels) const {
+    const int lim = kernels * L::kGranOne;
 #pragma unroll
     for (int i = 0; i < L::kPer; ++i) {
       const int q = (int)threadIdx.x + 256 * i;
-      if (q < L::kGran) {
-        int off;
-        if (q < L::kGranU) {
-          const int row = q / (L::Kp / 8), cg = q - row * (L::Kp / 8);
-          off = (row * L::KpL + cg * 8) * 2;
+      if (q < lim) {
+        const int k = q / L::kGranOne, qq = q - k * L::kGranOne;
+        int off = k * L::kOne;
+        if (qq < L::kGranU) {
+          const int row = qq / (L::Kp / 8), cg = qq - row * (L::Kp / 8);
+          off += (row * L::KpL + cg * 8) * 2;
         } else {
-          off = L::kUBytes + (q - L::kGranU) * 16;
+          off += L::kUBytes + (qq - L::kGranU) * 16;
         }
         *reinterpret_cast<sc_u32x4 *>(dst + off) = g[i];
       }
@@ -189,13 +199,27 @@ struct ScreenWaveList {
   int32_t ent[kScreenQ * kWave];
 };
 
+// log_rho<D>(rec, xm) with a per-lane record: the same function, so the
+// same bits; the record pointer is declared 16-byte aligned (records are),
+// so that the fully unrolled loads of the packed triangle can be merged
+// into 16-byte loads.
+template <int D>
+__device__ __forceinline__ float log_rho_lane(const float *rec, const float *__restrict__ xrow) {
+  const float *r = reinterpret_cast<const float *>(__builtin_assume_aligned(rec, 16));
+  float xv[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) xv[t] = xrow[t];
+  return log_rho<D>(r, xv);
+}
+
 template <int D, int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 estep_screen_kernel(EParams p, SParams sp) {
   using L = ScreenLds<D>;
   constexpr int Kp = L::Kp, KpL = L::KpL;
   constexpr int RT = sc_dp(D) / 32, KS = Kp / 16;
   constexpr int64_t R = rec_floats(D);
+  constexpr int64_t FB = sc_frag_bytes(D);
   __shared__ __attribute__((aligned(16))) uint8_t lds[2][L::kBytes];
   __shared__ int16_t sq[kScreenQ][256];
   __shared__ ScreenWaveList wl[4];
@@ -228,13 +252,12 @@ estep_screen_kernel(EParams p, SParams sp) {
   const float am = valid ? sp.xnorm[m] : 0.f;
 
   float lb = neg_inf();   // UPDATE: bound of the normaliser from the old mass
-  float cm = 0.f;         // FIXED: the given normaliser; later the new one
+  float cm = 0.f;         // ACCUM, FIXED: the given normaliser; later the new one
   if (MODE == LAKDE_ESTEP_UPDATE && valid) {
     const float co = p.c_old[m];
     lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
   }
-  if (MODE == LAKDE_ESTEP_FIXED && valid) cm = p.c[m];
-  if (MODE == LAKDE_ESTEP_ACCUM && valid) cm = p.c[m];
+  if (MODE != LAKDE_ESTEP_UPDATE && valid) cm = p.c[m];
   // running lower bound of the new normaliser (ACCUM: the new normaliser
   // logaddexp(c, LSE) is at least c)
   float B = MODE == LAKDE_ESTEP_ACCUM ? cm : lb;
@@ -242,83 +265,82 @@ estep_screen_kernel(EParams p, SParams sp) {
   lse.init();
   int kcnt = 0;           // UPDATE: keep candidates queued (may exceed kKeepQ)
 
+  // bounds [lo, hi] of the exact log rho of kernel n (LDS image fb) at this
+  // lane's column; -inf for the excluded diagonal and invalid columns
+  auto bounds = [&](const uint8_t *fb, int64_t n, float &hi, float &lo) {
+    const __bf16 *U = reinterpret_cast<const __bf16 *>(fb);
+    const float *cneg = reinterpret_cast<const float *>(fb + L::kUBytes);
+    const float *scal = cneg + sc_dp(D);
+    sc_f32x16 acc[RT][2];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      sc_f32x16 ci;
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const sc_f32x4 v =
+            *reinterpret_cast<const sc_f32x4 *>(cneg + rt * 32 + 8 * q4 + 4 * (lane >> 5));
+        ci[4 * q4 + 0] = v[0];
+        ci[4 * q4 + 1] = v[1];
+        ci[4 * q4 + 2] = v[2];
+        ci[4 * q4 + 3] = v[3];
+      }
+      acc[rt][0] = ci;
+      acc[rt][1] = ci;
+    }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        if (s * 16 > rt * 32 + 31) continue;  // U is lower triangular
+        const sc_bf16x8 a = *reinterpret_cast<const sc_bf16x8 *>(
+            U + (rt * 32 + (lane & 31)) * KpL + s * 16 + 8 * (lane >> 5));
+        acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[0][s], acc[rt][0], 0, 0, 0);
+        acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[1][s], acc[rt][1], 0, 0, 0);
+      }
+    }
+    float q0 = 0.f, q1 = 0.f;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        q0 = fmaf(acc[rt][0][r], acc[rt][0][r], q0);
+        q1 = fmaf(acc[rt][1][r], acc[rt][1][r], q1);
+      }
+    const bool upper = lane >= 32;
+    const float recv = __shfl_xor(upper ? q0 : q1, 32, kWave);
+    const float qt = (upper ? q1 : q0) + recv;  // Q~ of this lane's column
+    const float E = fmaf(scal[0], am, scal[1]);
+    const float sqt = sqrtf(qt);
+    const float rl = fmaf(sqt, 1.f - 4e-6f, -E);
+    const float qlb = rl > 0.f ? rl * rl * (1.f - 1e-5f) : 0.f;
+    const float rh = fmaf(sqt, 1.f + 4e-6f, E);
+    const float cst = scal[2], expo = scal[3];
+    const float slack = fmaf(1e-4f, fabsf(cst), 0.01f);
+    hi = fmaf(expo, qlb, cst) + slack;
+    lo = fmaf(expo, rh * rh * (1.f + 1e-5f), cst) - slack;
+    if (!valid || n == diag_n) {
+      hi = neg_inf();
+      lo = neg_inf();
+    }
+  };
+
   // One pass over the block's kernels: screen, queue survivors from kernel
-  // `resume` on (at most kScreenQ; the first kernel not queued is returned
-  // in next_resume, nb when all were), then compute the queued pairs
-  // exactly.  keep_pass: survivors are the pairs that can pass the keep
-  // test against the final normaliser cm.  Returns the number queued.
+  // `resume` on (at most kScreenQ per lane; the first kernel not queued is
+  // returned in next_resume, nb when all were), then compute the queued
+  // pairs exactly.  keep_pass: survivors are the pairs that can pass the
+  // keep test against the final normaliser cm.  Returns the lane's first
+  // entry in its wave's list | (entries << 16).
   auto pass = [&](bool keep_pass, int64_t resume, int64_t &next_resume) -> int {
     int cnt = 0;
     next_resume = nb;
-    ScreenStage<D> st;
-    st.load(sp.frag);
-    st.store(lds[0]);
-    __syncthreads();
-    for (int64_t n = 0; n < nb; ++n) {
-      st.load(sp.frag + (n + 1 < nb ? n + 1 : n) * sc_frag_bytes(D));
-      const uint8_t *fb = lds[n & 1];
-      const __bf16 *U = reinterpret_cast<const __bf16 *>(fb);
-      const float *cneg = reinterpret_cast<const float *>(fb + L::kUBytes);
-      const float *scal = cneg + sc_dp(D);
-      sc_f32x16 acc[RT][2];
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt) {
-        sc_f32x16 ci;
-#pragma unroll
-        for (int q4 = 0; q4 < 4; ++q4) {
-          const sc_f32x4 v =
-              *reinterpret_cast<const sc_f32x4 *>(cneg + rt * 32 + 8 * q4 + 4 * (lane >> 5));
-          ci[4 * q4 + 0] = v[0];
-          ci[4 * q4 + 1] = v[1];
-          ci[4 * q4 + 2] = v[2];
-          ci[4 * q4 + 3] = v[3];
-        }
-        acc[rt][0] = ci;
-        acc[rt][1] = ci;
-      }
-#pragma unroll
-      for (int s = 0; s < KS; ++s) {
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          if (s * 16 > rt * 32 + 31) continue;  // U is lower triangular
-          const sc_bf16x8 a = *reinterpret_cast<const sc_bf16x8 *>(
-              U + (rt * 32 + (lane & 31)) * KpL + s * 16 + 8 * (lane >> 5));
-          acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[0][s], acc[rt][0], 0, 0, 0);
-          acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[1][s], acc[rt][1], 0, 0, 0);
-        }
-      }
-      float q0 = 0.f, q1 = 0.f;
-#pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          q0 = fmaf(acc[rt][0][r], acc[rt][0][r], q0);
-          q1 = fmaf(acc[rt][1][r], acc[rt][1][r], q1);
-        }
-      const bool upper = lane >= 32;
-      const float recv = __shfl_xor(upper ? q0 : q1, 32, kWave);
-      const float qt = (upper ? q1 : q0) + recv;  // Q~ of this lane's column
-      // bounds of the exact log rho
-      const float E = fmaf(scal[0], am, scal[1]);
-      const float sqt = sqrtf(qt);
-      const float rl = fmaf(sqt, 1.f - 4e-6f, -E);
-      const float qlb = rl > 0.f ? rl * rl * (1.f - 1e-5f) : 0.f;
-      const float rh = fmaf(sqt, 1.f + 4e-6f, E);
-      const float cst = scal[2], expo = scal[3];
-      const float slack = fmaf(1e-4f, fabsf(cst), 0.01f);
-      float hi = fmaf(expo, qlb, cst) + slack;
-      float lo = fmaf(expo, rh * rh * (1.f + 1e-5f), cst) - slack;
-      if (!valid || n == diag_n) {
-        hi = neg_inf();
-        lo = neg_inf();
-      }
+    auto consider = [&](int64_t n, float hi, float lo) {
       bool surv;
       if (keep_pass) {
         surv = hi > cm + p.log_t_lo;
       } else {
         B = fmaxf(B, lo);
-        surv = !(hi <= B - sp.margin);  // NaN bounds survive (and propagate)
-        if (!valid || n == diag_n) surv = false;
+        // NaN bounds survive (and propagate through the exact values)
+        surv = valid && n != diag_n && !(hi <= B - sp.margin);
       }
       if (surv && n >= resume && next_resume == nb) {
         if (cnt < kScreenQ) {
@@ -328,8 +350,25 @@ estep_screen_kernel(EParams p, SParams sp) {
           next_resume = n;
         }
       }
-      st.store(lds[(n + 1) & 1]);
+    };
+    ScreenStage<D> st;
+    st.load(sp.frag, nb < kScreenKps ? (int)nb : kScreenKps);
+    st.store(lds[0], nb < kScreenKps ? (int)nb : kScreenKps);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t n0 = 0; n0 < nb; n0 += kScreenKps) {
+      const int64_t nx = n0 + kScreenKps;
+      const int kn = nx >= nb ? 0 : (nb - nx < kScreenKps ? (int)(nb - nx) : kScreenKps);
+      st.load(sp.frag + nx * FB, kn);
+      const uint8_t *fb = lds[buf];
+      float hi0, lo0, hi1, lo1;
+      bounds(fb, n0, hi0, lo0);
+      bounds(fb + L::kOne, n0 + 1, hi1, lo1);
+      consider(n0, hi0, lo0);
+      if (n0 + 1 < nb) consider(n0 + 1, hi1, lo1);
+      st.store(lds[buf ^ 1], kn);
       __syncthreads();
+      buf ^= 1;
     }
     // exact phase (wave-local): compact the lanes' queues into one list,
     // one pair per lane, in kernel order per lane
@@ -341,20 +380,17 @@ estep_screen_kernel(EParams p, SParams sp) {
     }
     const int excl = incl - cnt;
     const int total = __shfl(incl, kWave - 1, kWave);
-    ScreenWaveList &L2 = wl[w];
-    for (int k = 0; k < cnt; ++k) L2.ent[excl + k] = ((int32_t)sq[k][tid] << 8) | lane;
+    ScreenWaveList &W = wl[w];
+    for (int k = 0; k < cnt; ++k) W.ent[excl + k] = ((int32_t)sq[k][tid] << 8) | lane;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     for (int j0 = 0; j0 < total; j0 += kWave) {
       const int j = j0 + lane;
       if (j < total) {
-        const int32_t e = L2.ent[j];
+        const int32_t e = W.ent[j];
         const int64_t pn = e >> 8;
         const int64_t pm = base + (e & 63);
-        float xv[D];
-#pragma unroll
-        for (int t = 0; t < D; ++t) xv[t] = p.X[pm * D + t];
-        L2.ent[j] = __float_as_int(log_rho<D>(p.rec + pn * R, xv));
+        W.ent[j] = __float_as_int(log_rho_lane<D>(p.rec + pn * R, p.X + pm * D));
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -362,44 +398,7 @@ estep_screen_kernel(EParams p, SParams sp) {
     return excl | (cnt << 16);
   };
 
-  // ------------------------------------------------ log-sum-exp passes
-  if (MODE != LAKDE_ESTEP_FIXED) {
-    int64_t resume = 0;
-    while (true) {
-      int64_t next = nb;
-      const int r = pass(false, resume, next);
-      const int excl = r & 0xffff, cnt = r >> 16;
-      const ScreenWaveList &L2 = wl[w];
-      for (int k = 0; k < cnt; ++k) lse.add1(__int_as_float(L2.ent[excl + k]));
-      if (MODE == LAKDE_ESTEP_UPDATE) {
-        // keep candidates: exact values that can pass the keep test against
-        // any normaliser >= max(B, the largest exact value)
-        const float Bk = fmaxf(B, lse.m);
-        for (int k = 0; k < cnt; ++k) {
-          const float x = __int_as_float(L2.ent[excl + k]);
-          if (__fsub_rn(x, Bk) > p.log_t_lo) {
-            if (kcnt < kKeepQ) {
-              kq_n[kcnt][tid] = sq[k][tid];
-              kq_x[kcnt][tid] = x;
-            }
-            ++kcnt;
-          }
-        }
-      }
-      resume = next;
-      if (!__syncthreads_or(next < nb)) break;
-    }
-    const float l = lse.value();
-    if (valid) {
-      cm = MODE == LAKDE_ESTEP_ACCUM ? logaddexp_f(cm, l) : logaddexp_f(lb, l);
-      p.c[m] = cm;
-    }
-    if (MODE == LAKDE_ESTEP_UPDATE && p.lazy_unsafe != nullptr)
-      lazy_unsafe_test(p, valid, m, valid ? p.c_old[m] : 0.f, cm);
-  }
-  if (MODE == LAKDE_ESTEP_ACCUM) return;
   const int64_t cseg = chunk / p.seg_len;
-
   auto keep = [&](int64_t n, float x) {
     const float d = __fsub_rn(x, cm);
     float r;
@@ -410,22 +409,59 @@ estep_screen_kernel(EParams p, SParams sp) {
     }
   };
 
-  if (MODE == LAKDE_ESTEP_UPDATE && !__syncthreads_or(kcnt > kKeepQ)) {
-    if (valid)
-      for (int k = 0; k < kcnt; ++k) keep(kq_n[k][tid], kq_x[k][tid]);
-    return;
-  }
-  // keep passes (FIXED mode, or an UPDATE block with a full keep queue)
+  // log-sum-exp passes (ACCUM, UPDATE), then keep passes (FIXED, or an
+  // UPDATE block with a full keep queue); one pass call site
+  bool keep_phase = MODE == LAKDE_ESTEP_FIXED;
   int64_t resume = 0;
   while (true) {
     int64_t next = nb;
-    const int r = pass(true, resume, next);
+    const int r = pass(keep_phase, resume, next);
     const int excl = r & 0xffff, cnt = r >> 16;
-    const ScreenWaveList &L2 = wl[w];
-    if (valid)
-      for (int k = 0; k < cnt; ++k) keep(sq[k][tid], __int_as_float(L2.ent[excl + k]));
-    resume = next;
-    if (!__syncthreads_or(next < nb)) break;
+    const ScreenWaveList &W = wl[w];
+    if (keep_phase) {
+      if (valid)
+        for (int k = 0; k < cnt; ++k) keep(sq[k][tid], __int_as_float(W.ent[excl + k]));
+    } else {
+      for (int k = 0; k < cnt; ++k) lse.add1(__int_as_float(W.ent[excl + k]));
+      if (MODE == LAKDE_ESTEP_UPDATE) {
+        // keep candidates: exact values that can pass the keep test against
+        // any normaliser >= max(B, the largest exact value)
+        const float Bk = fmaxf(B, lse.m);
+        for (int k = 0; k < cnt; ++k) {
+          const float x = __int_as_float(W.ent[excl + k]);
+          if (__fsub_rn(x, Bk) > p.log_t_lo) {
+            if (kcnt < kKeepQ) {
+              kq_n[kcnt][tid] = sq[k][tid];
+              kq_x[kcnt][tid] = x;
+            }
+            ++kcnt;
+          }
+        }
+      }
+      // the largest exact value bounds the normaliser too (later passes)
+      B = fmaxf(B, lse.m);
+    }
+    if (__syncthreads_or(next < nb)) {
+      resume = next;
+      continue;
+    }
+    if (keep_phase || MODE == LAKDE_ESTEP_FIXED) return;
+    // the log-sum-exp is complete: the normaliser update
+    const float l = lse.value();
+    if (valid) {
+      cm = MODE == LAKDE_ESTEP_ACCUM ? logaddexp_f(cm, l) : logaddexp_f(lb, l);
+      p.c[m] = cm;
+    }
+    if (MODE == LAKDE_ESTEP_ACCUM) return;
+    if (MODE == LAKDE_ESTEP_UPDATE && p.lazy_unsafe != nullptr)
+      lazy_unsafe_test(p, valid, m, valid ? p.c_old[m] : 0.f, cm);
+    if (!__syncthreads_or(kcnt > kKeepQ)) {
+      if (valid)
+        for (int k = 0; k < kcnt; ++k) keep(kq_n[k][tid], kq_x[k][tid]);
+      return;
+    }
+    keep_phase = true;
+    resume = 0;
   }
 }
 
