@@ -1,49 +1,67 @@
-"""Benchmark: VI iterations/s (and test points/s) of LocalKNearestKDE on one
-MI355X, with the E-step kernel's roofline and the CPU oracle beside it.
+"""Benchmark: VI iterations/s and test points/s of LocalKNearestKDE on one
+MI355X, on the full-scale d=63 workload (mix128_d63: N=1,000,000 training and
+50,000 test points, D=63, k=128, threshold 1/128, block size 512), the
+precision form (BF16 screening E-step with exact FP32 refinement) against the
+plain FP32 path, with the dominant kernel's roofline and the reference's CPU
+path beside it.
 
-    python bench.py [--steps K] [--warmup W] [--workload mix32_d6] [--block-size 512]
+    python bench.py [--steps K] [--warmup W] [--budget-s S] [--no-fp32]
+                    [--no-cpu-baseline] [--no-secondary] [--no-tuning]
 
-A step is one VI iteration: one sweep of partial steps over all kernel blocks
-of the workload (lakde/kdes.py:64-67), with the data already resident in HBM.
-Prints one JSON line.
+A step is one VI iteration: one sweep of partial steps over all 1954 kernel
+blocks (lakde/kdes.py:64-67), with the data resident in HBM.  The K timed
+iterations run in one fit() call of the reference API.  A whole sweep of this
+workload takes seconds, so the number of iterations actually timed (and of
+warm-up iterations) is the largest that keeps the run within --budget-s of
+wall time, at most K (and W); the line reports the numbers run, with the
+requested ones beside them.  Prints one JSON line.
 """
 
 import argparse
+import hashlib
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+HEADLINE = "mix128_d63"
+SECONDARY = "mix32_d6"
 
-# HBM bytes per launch of the pair-skipping estep_main_kernel<6,UPDATE> on
-# mix32_d6 with block size 512: rocprofv3 FETCH_SIZE and WRITE_SIZE (separate
-# runs, KiB per dispatch) scaled by what they read for a plain 512 MiB
-# elementwise copy (factors 1.9999 and 1.0000); summary in
-# profiles/estep_mix32_d6_hbm_traffic.md
-MEASURED_TRAFFIC = {("mix32_d6", 512): {"fetch": 47249802, "write": 8159539}}
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc runs
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the same
+# counters read for a plain copy of known size; valid for the kernel source
+# whose sha256 is recorded (the line reports null for any other source).
+MEASURED_TRAFFIC = None
 
 
 def _parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1,
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5,
                     help="untimed VI iterations after initialisation (at least 1)")
-    ap.add_argument("--workload", default="mix32_d6")
-    ap.add_argument("--block-size", type=int, default=512)
+    ap.add_argument("--budget-s", type=float, default=480.0,
+                    help="wall-time budget of the whole run, which caps the iterations run")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-precision-form", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-secondary", action="store_true", help="skip the mix32_d6 FP32 block")
+    ap.add_argument("--no-tuning", action="store_true", help="skip the prior-scale gradient leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="finish with a plain device copy of this many MiB (a known-size "
                          "reference for hardware byte counters)")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"# [{time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def copy_probe(mib):
@@ -56,8 +74,7 @@ def copy_probe(mib):
     for _ in range(2):
         torch.neg(a, out=b)
         torch.cuda.synchronize()
-    print(f"# copy probe: {mib} MiB read + {mib} MiB written (neg kernel)", file=sys.stderr,
-          flush=True)
+    log(f"copy probe: {mib} MiB read + {mib} MiB written (neg kernel)")
 
 
 class KernelTimer:
@@ -100,35 +117,30 @@ class KernelTimer:
         return sum(ts) / len(ts), len(ts)
 
 
-def cpu_baseline(X, graph, block_size, threshold, seconds):
-    """Time the oracle's partial step on the full workload (bounded: as many
-    partial steps as fit in ``seconds``) and extrapolate to a sweep."""
-    import numpy as np
-    from threadpoolctl import threadpool_limits
+def host_cpu():
+    """(model name, cores usable by this process; capped at 16, the host
+    share of one GPU)."""
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return model, min(len(os.sched_getaffinity(0)), 16)
 
-    from oracle import lakde_np as O
 
-    with threadpool_limits(limits=1):
-        m = O.LocalKNearestKDE(k_or_knn_graph=graph, block_size=block_size)
-        m.init_parameters_(X, threshold)
-        # state as after initialisation, without the O(N^2) init passes:
-        # normalisers from one LSE pass of block 0 only (values do not change
-        # the work of a partial step)
-        N = X.shape[0]
-        m.rnm_log_consts = np.zeros(N, np.float32)
-        m.nu[:] = m.nu_0 + 1
-        t0 = time.perf_counter()
-        steps = 0
-        nb = (N - 1) // m.block_size + 1
-        while True:
-            m.partial_step_(X, steps % nb, sparsity_threshold=threshold)
-            steps += 1
-            if time.perf_counter() - t0 > seconds:
-                break
-        dt = (time.perf_counter() - t0) / steps
-    return {"value": 1.0 / (dt * nb), "unit": "VI iters/sec", "cores": 1, "kind": "port",
-            "sample": f"{steps} partial steps of the full workload (block size {block_size}), "
-                      f"{dt:.3f} s each, extrapolated to {nb} steps per VI iteration"}
+def _sweep(model, X, t, label=None):
+    """One VI iteration through the reference's per-step API (the body of
+    lakde/kdes.py:64-67), for the warm-up and comparison sweeps; with a
+    label, a progress line every 256 partial steps."""
+    nb = (X.size(0) - 1) // model.block_size + 1
+    for b in range(nb):
+        model.partial_step_(X, b, sparsity_threshold=t)
+        if label and b % 256 == 255:
+            log(f"{label}: {b + 1}/{nb} partial steps")
+    model.iter_steps += nb
 
 
 def pred_fp64(rec, D, Y, kchunk=2048, pchunk=2000):
@@ -195,120 +207,159 @@ def prior_scale_tuning(model, X, Y, rounds=2):
         if best[0] == ls:
             break
         ls, J = best
-    print(f"# prior scale: J {J0:.5f}, dJ/dlog s {g0:.4e} (central difference {fd:.4e}, "
-          f"{t_grad:.1f}s); tuned log s {ls:+.2f}: J {J:.5f}", file=sys.stderr, flush=True)
+    log(f"prior scale: J {J0:.5f}, dJ/dlog s {g0:.4e} (central difference {fd:.4e}, "
+        f"{t_grad:.1f}s); tuned log s {ls:+.2f}: J {J:.5f}")
     return {"points": int(Y.size(0)), "J_at_s1": J0, "dJ_dlog_s": g0,
             "central_difference": fd, "rel_diff": abs(g0 - fd) / max(abs(fd), 1e-30),
             "gradient_s": round(t_grad, 2), "path": path, "tuned_log_s": ls, "J_tuned": J}
 
 
-def precision_form(seconds_budget=None):
-    """The precision comparison on the full-scale d=63 workload
-    (mix128_d63: N=1,000,000, D=63, k=128, threshold 1/128, block size 512):
-    initialise once (BF16 form), then time partial steps of the FP32 and the
-    BF16 E-step from the same state and extrapolate to a VI iteration
-    (num_blocks partial steps); predictive density on 10,000 held-out points
-    with both forms on the same posterior; both against float64 arithmetic
-    on the same records (10,000 held-out points)."""
+def calibrate_c1(cores):
+    """The CPU port on the circle workload (N=5,000, D=2, k=20, no threshold):
+    initialisation plus 2 VI iterations, projected to the 50-iteration fit
+    the reference ran in 41.8 s on 8 threads (the fit time recorded with the
+    reference's outputs for this workload)."""
+    from lakde_amd import synthetic
+    from oracle import cpu_path as C
+    from oracle import lakde_np as O
+    from oracle.lakde_np import compute_knn_graph
+    X, _ = synthetic.make("circle8")
+    graph = compute_knn_graph(X, 20, include_self=False)
+    with C.reference_cpu_ops(cores):
+        m = O.LocalKNearestKDE(k_or_knn_graph=graph, block_size=512)
+        t0 = time.perf_counter()
+        m.init_parameters_(X, 0)
+        m.init_summaries_(X, 0)
+        t_init = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        nb = (X.shape[0] - 1) // 512 + 1
+        for _ in range(2):
+            for b in range(nb):
+                m.partial_step_(X, b, 0)
+        t_it = (time.perf_counter() - t0) / 2
+    proj = t_init + 50 * t_it
+    return {"workload": "circle8: N=5000, D=2, k=20, threshold 0, 50 VI iterations",
+            "port_init_s": round(t_init, 3), "port_s_per_iteration": round(t_it, 3),
+            "port_fit_s_projected": round(proj, 2), "reference_fit_s": 41.81,
+            "reference_threads": 8, "port_over_reference": round(proj / 41.81, 3)}
+
+
+def cpu_baseline(model, Xh, graph_h, t, n_cols=131072):
+    """The reference's CPU path timed on the host cores: the oracle's model
+    code (the reference's algorithm) with the reference's torch CPU ops
+    (oracle/cpu_path.py), on a bounded sample of one partial step of the
+    workload, from the GPU model's state at the end of the timed iterations:
+    the posterior of block 0 (kNN prior, W_inv contributions, stabilised
+    Cholesky), its E-step over the first ``n_cols`` columns (scaled to all
+    N), and the rebalance over all blocks; x num_blocks per VI iteration."""
+    import numpy as np
+
+    from oracle import cpu_path as C
+    from oracle import lakde_np as O
+    cpu_model, cores = host_cpu()
+    N, D = Xh.shape
+    bs = model.block_size
+    nblocks = (N - 1) // bs + 1
+    n_cols = min(n_cols, N)
+    blocks = [(ind.cpu().numpy(), val.cpu().numpy()) for ind, val in model.rnm_blocks]
+    c = model.rnm_log_consts.cpu().numpy()
+    with C.reference_cpu_ops(cores):
+        m = O.LocalKNearestKDE(k_or_knn_graph=graph_h, block_size=bs)
+        m._init_common(Xh, t)
+        m.knn_graph = np.asarray(graph_h, np.int32)
+        m.k = m.knn_graph.shape[1]
+        for k in ("a", "b", "nu", "W_inv_jitter", "sigma_0_jitter"):
+            setattr(m, k, getattr(model, k).cpu().numpy().copy())
+        m.logdet_W = np.empty(N, np.float32)
+        m.logdet_sigma_0 = np.empty(N, np.float32)
+        m.sigma_0_lambda_mm_trace = np.empty(N, np.float32)
+        m.rnm_blocks = blocks
+        t0 = time.perf_counter()
+        sigma_0, _ = m.calc_prior_cov(Xh, 0)
+        contrib, nu = m.calc_posterior_contribs(Xh, 0)
+        W_triu = m.calc_posterior(0, sigma_0, contrib)
+        t_post = time.perf_counter() - t0
+        ind, val = blocks[0]
+        keep = ind[1] < n_cols
+        m.rnm_blocks = [(ind[:, keep], val[keep])]
+        m.rnm_log_consts = c[:n_cols].copy()
+        t0 = time.perf_counter()
+        m.partial_expectation_step(np.ascontiguousarray(Xh[:n_cols]), 0, W_triu, nu,
+                                   sparsity_threshold=t)
+        t_e = time.perf_counter() - t0
+        m.rnm_blocks = blocks
+        m.rnm_log_consts = c
+        t0 = time.perf_counter()
+        m.rebalance_responsibilities_(Xh, 0, c.copy())
+        t_r = time.perf_counter() - t0
+    t_step = t_post + t_e * N / n_cols + t_r
+    log(f"cpu baseline ({cores} threads, {cpu_model}): posterior {t_post:.2f}s, E-step "
+        f"{t_e:.2f}s over {n_cols} columns, rebalance {t_r:.2f}s -> {t_step:.1f}s per partial step")
+    return {"value": 1.0 / (t_step * nblocks), "unit": "VI iters/sec", "cores": cores,
+            "kind": "port", "cpu_model": cpu_model,
+            "sample": f"one partial step of {HEADLINE} (block 0, 512 kernels), from the GPU "
+                      f"model's state after the timed iterations: posterior {t_post:.2f}s, "
+                      f"E-step over the first {n_cols} of {N} columns {t_e:.2f}s (scaled "
+                      f"x{N / n_cols:.3f}), rebalance over all {nblocks} blocks {t_r:.2f}s; "
+                      f"{t_step:.1f}s per partial step x {nblocks} partial steps per VI "
+                      f"iteration; the oracle's model code with the reference's torch CPU ops "
+                      f"on {cores} threads",
+            "partial_step_s": round(t_step, 3)}
+
+
+def screen_source_sha():
+    with open(os.path.join(ROOT, "lakde_amd", "csrc", "screen.hpp"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def secondary(steps=20, warmup=3):
+    """The plain FP32 path on mix32_d6 (N=200,000, D=6, k=64, threshold
+    1/64): VI iterations/s through fit() and test points/s."""
     import torch
 
     from lakde_amd import LocalKNearestKDE, synthetic
     from lakde_amd.utils import compute_knn_graph
-
-    w = synthetic.WORKLOADS["mix128_d63"]
-    Xh, Yh = synthetic.make("mix128_d63", n_test=10000)
+    w = synthetic.WORKLOADS[SECONDARY]
+    Xh, Yh = synthetic.make(SECONDARY)
     N, D = Xh.shape
-    dev = torch.device("cuda:0")
-    X = torch.from_numpy(Xh).to(dev)
-    Y = torch.from_numpy(Yh).to(dev)
+    X = torch.from_numpy(Xh).cuda()
+    Y = torch.from_numpy(Yh).cuda()
     t = w["threshold"]
-    t0 = time.perf_counter()
     graph = compute_knn_graph(X, w["k"], 1024, include_self=False)
-    torch.cuda.synchronize()
-    t_graph = time.perf_counter() - t0
-    print(f"# precision form: mix128_d63 N={N} D={D}, knn graph {t_graph:.1f}s", file=sys.stderr,
-          flush=True)
-    model = LocalKNearestKDE(k_or_knn_graph=graph, block_size=512, precision="bf16")
-    t0 = time.perf_counter()
+    model = LocalKNearestKDE(k_or_knn_graph=graph, block_size=512)
     model.init_parameters_(X, t)
     model.init_summaries_(X, t)
+    _sweep(model, X, t)
+    model.fit(X, iterations=warmup - 1, sparsity_threshold=t)
+    nblocks = (N - 1) // 512 + 1
+    timer = KernelTimer(every=8)
+    timer.reserve(steps * nblocks)
+    model._estep_timer = timer
     torch.cuda.synchronize()
-    t_init = time.perf_counter() - t0
-    nblocks = (N - 1) // model.block_size + 1
-    print(f"# precision form: init (bf16) {t_init:.1f}s, {nblocks} blocks", file=sys.stderr,
-          flush=True)
-    res = {"workload": f"mix128_d63: LocalKNearestKDE, N={N}, D={D}, k={w['k']}, "
-                       f"threshold=1/128, block_size=512",
-           "init_s_bf16": round(t_init, 2), "num_blocks": nblocks}
-    steps = 6
-    for prec in ("fp32", "bf16"):
-        model.precision = prec
-        timer = KernelTimer()
-        timer.reserve(steps + 1)
-        model._estep_timer = timer
-        model.partial_step_(X, 0, sparsity_threshold=t)  # warm-up
-        torch.cuda.synchronize()
-        timer.pairs.clear()
-        t0 = time.perf_counter()
-        for i in range(steps):
-            model.partial_step_(X, 1 + i * (nblocks // steps), sparsity_threshold=t)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / steps
-        k_ms, _ = timer.mean_ms()
-        model._estep_timer = None
-        pairs = 512 * N
-        res[prec] = {"partial_step_ms": round(dt * 1e3, 2),
-                     "vi_iters_per_s_extrapolated": round(1.0 / (dt * nblocks), 5),
-                     "estep_main_ms": round(k_ms, 3),
-                     "estep_algorithmic_tflops": round(pairs * D * (D + 1) / (k_ms * 1e-3) / 1e12, 2)}
-        print(f"# precision form {prec}: {dt * 1e3:.1f} ms/partial step, E-step main {k_ms:.2f} ms",
-              file=sys.stderr, flush=True)
-    # executed matrix-core work of the bf16 E-step main kernel: one pass of
-    # 3 (bf16x3) x 2 x 32 x 16 flop per pair for every 32-row x 16-column
-    # tile of the padded Linv that is not all zeros (the UPDATE step tests
-    # its queued candidates without a second pass)
-    rt_n, ks_n = (D + 31) // 32, (D + 15) // 16
-    tiles = sum(1 for rt in range(rt_n) for s_ in range(ks_n) if s_ * 16 <= rt * 32 + 31)
-    mfma_flop = 3 * 2 * 32 * 16 * tiles * 512 * N
-    res["bf16"]["mfma_executed_tflops"] = round(mfma_flop / (res["bf16"]["estep_main_ms"] * 1e-3) / 1e12, 1)
-    res["bf16"]["mfma_frac_of_2500_tflops"] = round(res["bf16"]["mfma_executed_tflops"] / 2500.0, 4)
-    res["speedup_bf16_over_fp32"] = round(res["fp32"]["partial_step_ms"] / res["bf16"]["partial_step_ms"], 2)
-    # predictive density, both forms, same posterior
-    lps = {}
-    for prec in ("fp32", "bf16"):
-        model.precision = prec
-        model.log_pred_density(X, Y[:256])
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        lps[prec] = model.log_pred_density(X, Y)
-        torch.cuda.synchronize()
-        res[prec]["test_pts_per_s"] = round(Y.size(0) / (time.perf_counter() - t0), 1)
-    d = (lps["bf16"] - lps["fp32"]).abs()
-    res["pred_abs_err_bf16_vs_fp32_nats"] = {"mean": float(d.mean()), "max": float(d.max())}
-    res["mean_log_density_fp32"] = float(lps["fp32"].mean())
-    # both forms against float64 arithmetic on the same records, on the
-    # 10,000 held-out points
     t0 = time.perf_counter()
-    lp64 = pred_fp64(model.rec, D, Y)
+    model.fit(X, iterations=steps, sparsity_threshold=t)
     torch.cuda.synchronize()
-    for prec in ("fp32", "bf16"):
-        e = (lps[prec].double() - lp64).abs()
-        res[prec]["pred_err_vs_fp64_nats"] = {"mean": float(e.mean()), "max": float(e.max()),
-                                              "points": int(Y.size(0))}
-    print(f"# precision form: float64 predictive on {Y.size(0)} points in "
-          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    # the differentiation form at full scale: tune a multiplicative scale s
-    # of the kNN prior covariances by the held-out gradient dJ/dlog s
-    # (HIP backward of the predictive density + float64 Cholesky VJPs over
-    # all 1M posteriors), checked by central differences of J
-    try:
-        res["prior_scale_tuning"] = prior_scale_tuning(model, X, Y[:2000])
-    except Exception as e:  # reported, never fatal to the headline line
-        print(f"# prior scale tuning failed: {e!r}", file=sys.stderr, flush=True)
-        res["prior_scale_tuning"] = {"error": repr(e)}
-    del model, X, Y, graph
-    torch.cuda.empty_cache()
-    return res
+    el = time.perf_counter() - t0
+    model._estep_timer = None
+    k_ms, k_n = timer.mean_ms()
+    model.log_pred_density(X, Y[:256])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lp = model.log_pred_density(X, Y)
+    torch.cuda.synchronize()
+    t_pred = time.perf_counter() - t0
+    flop = N * N * D * (D + 1) / nblocks
+    log(f"{SECONDARY}: {el / steps * 1e3:.1f} ms per VI iteration, E-step {k_ms:.4f} ms")
+    return {"workload": f"{SECONDARY}: LocalKNearestKDE fit, N={N}, D={D}, k={w['k']}, "
+                        f"threshold=1/64, block_size=512, FP32",
+            "steps": steps, "warmup": warmup, "vi_iters_per_s": round(steps / el, 4),
+            "ms_per_step": round(el / steps * 1e3, 3), "test_pts_per_s": round(Y.size(0) / t_pred, 1),
+            "mean_log_density": float(lp.mean()),
+            "estep_kernel": {"kernel": "estep_main_kernel<6,UPDATE,SKIP>", "ms": round(k_ms, 5),
+                             "launches_timed": k_n,
+                             "effective_tflops": round(flop / (k_ms * 1e-3) / 1e12, 2),
+                             "note": "effective: the reference's nb*N pairs; the kernel skips "
+                                     "the pairs its box bound excludes"}}
 
 
 def main():
@@ -317,127 +368,243 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(no multi-GPU path is built)", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    t_start = time.perf_counter()
     import torch
 
     from lakde_amd import LocalKNearestKDE, synthetic
-    from lakde_amd import _fused as F
-    from lakde_amd import _lib
     from lakde_amd.utils import compute_knn_graph
 
-    w = synthetic.WORKLOADS[args.workload]
-    Xh, Yh = synthetic.make(args.workload)
+    w = synthetic.WORKLOADS[HEADLINE]
+    Xh, Yh = synthetic.make(HEADLINE)
     N, D = Xh.shape
     M = Yh.shape[0]
+    k = w["k"]
+    t = w["threshold"]
+    bs = 512
+    nblocks = (N - 1) // bs + 1
     dev = torch.device("cuda:0")
     X = torch.from_numpy(Xh).to(dev)
     Y = torch.from_numpy(Yh).to(dev)
-    t = w["threshold"]
-    print(f"# workload {args.workload}: N={N} M={M} D={D} k={w['k']} threshold={t:.6g} "
-          f"bs={args.block_size}", file=sys.stderr, flush=True)
-
+    Ys = Y[:10000]
+    log(f"workload {HEADLINE}: N={N} M={M} D={D} k={k} threshold=1/128 bs={bs} "
+        f"({nblocks} blocks)")
     t0 = time.perf_counter()
-    graph = compute_knn_graph(X, w["k"], 1024, include_self=False)
+    graph = compute_knn_graph(X, k, 1024, include_self=False)
     torch.cuda.synchronize()
     t_graph = time.perf_counter() - t0
+    log(f"knn graph {t_graph:.1f}s")
 
-    model = LocalKNearestKDE(k_or_knn_graph=graph, block_size=args.block_size)
-    nblocks = (N - 1) // min(args.block_size, N) + 1
-    # initialisation and the warm-up iterations, untimed
-    warm = max(args.warmup, 1)
+    def new_model():
+        m = LocalKNearestKDE(k_or_knn_graph=graph, block_size=bs, precision="bf16")
+        t0 = time.perf_counter()
+        m.init_parameters_(X, t)
+        m.init_summaries_(X, t)
+        torch.cuda.synchronize()
+        return m, time.perf_counter() - t0
+
+    model, t_init = new_model()
+    log(f"init (bf16) {t_init:.1f}s, {model.arena.total_nnz()} kept responsibilities")
+    # first warm-up iteration; its state is the comparison point
     t0 = time.perf_counter()
-    model.init_parameters_(X, t)
-    model.init_summaries_(X, t)
+    _sweep(model, X, t)
     torch.cuda.synchronize()
-    t_init = time.perf_counter() - t0
-    print(f"# knn graph {t_graph:.2f}s, init {t_init:.2f}s, nnz {model.arena.total_nnz()}",
-          file=sys.stderr, flush=True)
-    for i in range(warm):
-        for b in range(nblocks):
-            model.partial_step_(X, b, sparsity_threshold=t)
-        model.iter_steps += nblocks
-    print(f"# {warm} warm-up iteration(s) done", file=sys.stderr, flush=True)
+    t_sweep = time.perf_counter() - t0
+    log(f"warm-up iteration 1 (bf16): {t_sweep:.2f}s")
+    lp_bf = model.log_pred_density(X, Ys)
+    model.precision = "fp32"
+    lp_bf_fit_32pred = model.log_pred_density(X, Ys)
+    model.precision = "bf16"
+    torch.cuda.synchronize()
 
-    # timed: fit() of the reference API, K VI iterations (one native sweep
-    # loop each), the responsibilities folded at the end
+    # the FP32 path: a second model from the same initialisation, one whole
+    # iteration with the FP32 E-step
+    fp32 = None
+    if not args.no_fp32:
+        log("fp32 comparison model: initialising")
+        twin, t_init2 = new_model()
+        log(f"fp32 comparison model: init {t_init2:.1f}s")
+        twin.precision = "fp32"
+        timer32 = KernelTimer(every=16)
+        timer32.reserve(nblocks)
+        twin._estep_timer = timer32
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _sweep(twin, X, t, label="fp32 iteration")
+        torch.cuda.synchronize()
+        t_fp32 = time.perf_counter() - t0
+        twin._estep_timer = None
+        k32, n32 = timer32.mean_ms()
+        lp32 = twin.log_pred_density(X, Ys)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        twin.log_pred_density(X, Ys)
+        torch.cuda.synchronize()
+        t_pred32 = time.perf_counter() - t0
+        d_fit = (lp_bf_fit_32pred - lp32).abs()
+        d_all = (lp_bf - lp32).abs()
+        fp32 = {"vi_iters_per_s": round(1.0 / t_fp32, 6), "sweep_s": round(t_fp32, 2),
+                "steps": 1, "init_s": round(t_init2, 2),
+                "estep_kernel": {"kernel": f"estep_main_kernel<{D},UPDATE>",
+                                 "ms": round(k32, 3), "launches_timed": n32,
+                                 "algorithmic_tflops": round(bs * N * D * (D + 1) / (k32 * 1e-3)
+                                                             / 1e12, 2),
+                                 "peak_tflops": PEAK_FP32_TFLOPS},
+                "test_pts_per_s": round(Ys.size(0) / t_pred32, 1),
+                "bf16_vs_fp32_after_one_iteration_nats": {
+                    "fit_only": {"mean": float(d_fit.mean()), "max": float(d_fit.max())},
+                    "fit_and_predictive": {"mean": float(d_all.mean()), "max": float(d_all.max())},
+                    "points": int(Ys.size(0)),
+                    "note": "both models from the same initialisation, one VI iteration each "
+                            "(BF16 screening vs FP32 E-step); fit_only: both predicted with the "
+                            "FP32 predictive kernel"}}
+        log(f"fp32 iteration {t_fp32:.1f}s (E-step {k32:.2f} ms); |bf16 - fp32| after one "
+            f"iteration: fit {float(d_fit.mean()):.2e} mean, fit+pred {float(d_all.mean()):.2e}")
+        del twin
+        torch.cuda.empty_cache()
+    # both predictive kernels against float64 arithmetic on the same records
+    t0 = time.perf_counter()
+    lp64 = pred_fp64(model.rec, D, Ys)
+    err = {}
+    for name, v in (("bf16", lp_bf), ("fp32", lp_bf_fit_32pred)):
+        e = (v.double() - lp64).abs()
+        err[name] = {"mean": float(e.mean()), "max": float(e.max())}
+    log(f"float64 predictive on {Ys.size(0)} points in {time.perf_counter() - t0:.1f}s: "
+        f"bf16 {err['bf16']['mean']:.2e}, fp32 {err['fp32']['mean']:.2e} nats mean")
+
+    # the iterations that fit the budget (a reserve for the legs after the
+    # timed region: prediction, tuning, CPU baseline, secondary block)
+    reserve = 25.0 + (0 if args.no_tuning else 25.0) + (0 if args.no_cpu_baseline else 45.0) \
+        + (0 if args.no_secondary else 15.0)
+    avail = int((args.budget_s - (time.perf_counter() - t_start) - reserve) // t_sweep)
+    steps = max(1, min(args.steps, avail))
+    warm = 1 + max(0, min(args.warmup - 1, avail - steps))
+    log(f"{avail} iterations fit the budget: {warm} warm-up (1 done), {steps} timed "
+        f"(requested {args.warmup} + {args.steps})")
+    for i in range(warm - 1):
+        model.fit(X, iterations=1, sparsity_threshold=t)
+        torch.cuda.synchronize()
+        log(f"warm-up iteration {i + 2} done")
+
+    # timed: fit() of the reference API, `steps` VI iterations, one call per
+    # iteration (each call ends with the host seeing the iteration done, so
+    # the run reports progress as it goes)
     timer = KernelTimer(every=8)
-    timer.reserve(args.steps * nblocks)
+    timer.reserve(steps * nblocks)
     model._estep_timer = timer
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    model.fit(X, iterations=args.steps, sparsity_threshold=t)
+    for i in range(steps):
+        model.fit(X, iterations=1, sparsity_threshold=t)
+        log(f"timed iteration {i + 1}/{steps} done")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     model._estep_timer = None
-    ms_step = elapsed / args.steps * 1e3
+    ms_step = elapsed / steps * 1e3
     k_ms, k_n = timer.mean_ms()
-    print(f"# timed {args.steps} VI iterations: {ms_step:.1f} ms each; E-step main kernel "
-          f"{k_ms:.4f} ms avg over {k_n} launches", file=sys.stderr, flush=True)
+    log(f"timed {steps} VI iterations: {ms_step / 1e3:.3f} s each; E-step main kernel "
+        f"{k_ms:.4f} ms avg over {k_n} launches")
 
-    # predictive density
-    lp = model.log_pred_density(X, Y)
+    # test points per second: the BF16 predictive on all test points
+    model.log_pred_density(X, Y[:256])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    reps = 3
-    for _ in range(reps):
-        lp = model.log_pred_density(X, Y)
+    lp = model.log_pred_density(X, Y)
     torch.cuda.synchronize()
-    t_pred = (time.perf_counter() - t0) / reps
+    t_pred = time.perf_counter() - t0
     mean_lp = float(lp.mean())
+    log(f"predictive: {M} points in {t_pred:.2f}s, mean log density {mean_lp:.4f}")
 
-    # roofline of the E-step main kernel (UPDATE mode): algorithmic work is
-    # D(D+1) FLOP per (kernel, point) pair, nb * N pairs per launch -- the
-    # reference's pairs; with pair skipping (the default for D <= 16) the
-    # kernel evaluates only the pairs its bounds cannot exclude
-    bs = model.block_size
-    pairs_per_launch = (N * N) / nblocks
-    flop = pairs_per_launch * D * (D + 1)
+    # roofline of the dominant kernel, the screening E-step (UPDATE mode):
+    # algorithmic work D(D+1) flop per (kernel, point) pair, nb*N pairs per
+    # launch; executed matrix-core work 2*32*32*16 flop per MFMA, one MFMA
+    # per (32-row, 16-column) tile of the padded triangular factor that is
+    # not all zeros, per 32 columns
+    rt_n, ks_n = (D + 31) // 32, (D + 15) // 16
+    tiles = sum(1 for rt in range(rt_n) for s_ in range(ks_n) if s_ * 16 <= rt * 32 + 31)
+    pairs = bs * N
+    flop = pairs * D * (D + 1)
+    mfma_flop = tiles * 2 * 32 * 32 * 16 * bs * (N // 32 + (N % 32 > 0))
     achieved = flop / (k_ms * 1e-3) / 1e12
-    tr = MEASURED_TRAFFIC.get((args.workload, bs))
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": (tr["fetch"] + tr["write"]) if tr else None,
-                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)",
-                "kernel": f"estep_main_kernel<{D},UPDATE{',SKIP' if model._col_order is not None else ''}>",
-                "kernel_ms": round(k_ms, 5), "pairs_per_launch": int(pairs_per_launch)}
+    executed = mfma_flop / (k_ms * 1e-3) / 1e12
+    tr = MEASURED_TRAFFIC
+    traffic, tnote = None, "not measured for this kernel source"
+    if tr is not None and tr["source_sha256"] == screen_source_sha():
+        traffic = tr["fetch_bytes"] + tr["write_bytes"]
+        tnote = tr["profile"]
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
+                "traffic": traffic, "traffic_note": tnote,
+                "kernel": f"estep_screen_kernel<{D},UPDATE>", "kernel_ms": round(k_ms, 4),
+                "launches_timed": k_n, "pairs_per_launch": pairs,
+                "achieved_executed_mfma": round(executed, 2),
+                "frac_executed_mfma": round(executed / PEAK_BF16_TFLOPS, 4),
+                "note": "achieved: the reference's D(D+1) flop per pair over all nb*N pairs; "
+                        "executed: the BF16 MFMA flop the screen issues (FP32 exact refinement "
+                        "of the surviving pairs not counted)"}
 
-    prec = None
-    if not args.no_precision_form:
+    tuning = None
+    if not args.no_tuning:
         try:
-            prec = precision_form()
+            tuning = prior_scale_tuning(model, X, Y[:2000])
         except Exception as e:  # reported, never fatal to the headline line
-            print(f"# precision form failed: {e!r}", file=sys.stderr, flush=True)
-            prec = {"error": repr(e)}
+            log(f"prior scale tuning failed: {e!r}")
+            tuning = {"error": repr(e)}
 
     cpu = None
     if not args.no_cpu_baseline:
-        g = graph.cpu().numpy()
-        cpu = cpu_baseline(Xh, g, bs, t, args.cpu_seconds)
+        try:
+            cpu = cpu_baseline(model, Xh, graph.cpu().numpy(), t)
+            cpu["calibration"] = calibrate_c1(cpu["cores"])
+            log(f"cpu calibration: {cpu['calibration']}")
+        except Exception as e:
+            log(f"cpu baseline failed: {e!r}")
+            cpu = {"error": repr(e)}
+    del model
+    torch.cuda.empty_cache()
 
+    sec = None
+    if not args.no_secondary:
+        try:
+            sec = secondary()
+        except Exception as e:
+            log(f"secondary block failed: {e!r}")
+            sec = {"error": repr(e)}
+
+    value = steps / elapsed
     out = {
         "metric": "VI iters/sec & test pts/sec, BF16+refine vs FP32 at <=1e-3 nats/pt vs FP64",
-        "value": round(1e3 / ms_step, 4),
+        "value": round(value, 5),
         "unit": "VI iters/sec",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "steps": steps,
+        "warmup": warm,
+        "steps_requested": args.steps,
+        "warmup_requested": args.warmup,
+        "ms_per_step": round(ms_step, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16 screen + f32 refine",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: LocalKNearestKDE fit, N={N}, D={D}, "
-                               f"k={w['k']}, threshold=1/{round(1 / t) if t else 0}, "
-                               f"block_size={bs}, FP32",
-                   "N": N, "M": M, "D": D, "k": w["k"], "block_size": bs},
+        "config": {"workload": f"{HEADLINE}: LocalKNearestKDE fit, N={N}, D={D}, k={k}, "
+                               f"threshold=1/128, block_size={bs}, precision form (BF16 "
+                               f"screening E-step, exact FP32 refinement)",
+                   "N": N, "M": M, "D": D, "k": k, "block_size": bs},
         "test_pts_per_sec": round(M / t_pred, 1),
-        "predict_ms": round(t_pred * 1e3, 3),
+        "predict_s": round(t_pred, 3),
         "mean_log_density": round(mean_lp, 6),
-        "init_s": round(t_init, 3),
+        "init_s": round(t_init, 2),
+        "knn_graph_s": round(t_graph, 2),
+        "fp32": fp32,
+        "speedup_bf16_over_fp32": round(value / fp32["vi_iters_per_s"], 3) if fp32 else None,
+        "pred_err_vs_fp64_nats": dict(err, points=int(Ys.size(0)),
+                                      note="the predictive kernels against float64 arithmetic "
+                                           "on the same records, after one VI iteration"),
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "precision_form": prec,
+        "prior_scale_tuning": tuning,
+        "secondary": sec,
+        "wall_s": round(time.perf_counter() - t_start, 1),
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib:

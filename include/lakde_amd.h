@@ -459,6 +459,13 @@ int lakde_estep_main_bf16(const lakde_estep_args *args, const void *frag,
  *     up; once per fit (X and o fixed).
  *   lakde_pack_screen: per kernel of a block (lakde_screen_frag_bytes(D)
  *     bytes each) from its records and o.
+ *   lakde_screen_knn_bound: out[m] = the largest exact log rho of point m
+ *     under the kernels of its first J neighbours of graph (N x k, int32 or
+ *     int64; J a power of two <= 64), a lower bound of the normaliser the
+ *     first log-sum-exp passes of a fit accumulate.
+ *   lbound (ACCUM only, may be NULL): such a bound; the passes drop the
+ *     pairs far below it from the start instead of below the normaliser
+ *     accumulated so far.
  * (lakde/local_adaptive_kde.py:36-44,98-153 as lakde_estep_main.) */
 int64_t lakde_screen_frag_bytes(int64_t D);
 size_t lakde_screen_points_bytes(int64_t N, int64_t D);
@@ -466,8 +473,11 @@ int lakde_screen_points(const float *X, int64_t N, int64_t D, const float *origi
                         float *xnorm, lakde_stream_t stream);
 int lakde_pack_screen(const float *rec, int64_t nb, int64_t D, const float *origin, void *frag,
                       lakde_stream_t stream);
+int lakde_screen_knn_bound(const float *rec, const float *X, int64_t N, int64_t D,
+                           const void *graph, int graph_dtype, int64_t k, int64_t J, float *out,
+                           lakde_stream_t stream);
 int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, const void *xb,
-                            const float *xnorm, lakde_stream_t stream);
+                            const float *xnorm, const float *lbound, lakde_stream_t stream);
 
 /* lakde_pred_density with the quadratic forms on the matrix cores
  * (frag: N fragment blocks; Y: M x D float32 test points). */

@@ -7,7 +7,7 @@ import math
 import torch
 
 from . import _lib
-from ._lib import EstepArgs, PosteriorArgs, check, idx_code, ptr, stream_of
+from ._lib import ESTEP_ACCUM, EstepArgs, PosteriorArgs, check, idx_code, ptr, stream_of
 
 
 def record_floats(D):
@@ -181,8 +181,9 @@ def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
         return
     if screen is not None:
         frag, pts = screen
-        check(lib.lakde_estep_main_screen(args, ptr(frag), ptr(pts.xb), ptr(pts.xnorm), st),
-              "estep_main_screen")
+        lb = pts.lbound if mode == ESTEP_ACCUM else None
+        check(lib.lakde_estep_main_screen(args, ptr(frag), ptr(pts.xb), ptr(pts.xnorm), ptr(lb),
+                                          st), "estep_main_screen")
     elif bf16 is not None:
         check(lib.lakde_estep_main_bf16(args, ptr(bf16), st), "estep_main_bf16")
     else:
@@ -207,6 +208,18 @@ class ScreenPoints:
         self.xnorm = torch.empty(N, device=X.device)
         check(lib.lakde_screen_points(ptr(X), N, D, ptr(self.origin), ptr(self.xb),
                                       ptr(self.xnorm), stream_of(X)), "screen_points")
+        self.lbound = None  # lower bounds of the normalisers for ACCUM passes
+
+    def knn_bound(self, rec, X, graph, J=16):
+        """lbound[m] = the largest exact log rho of x_m under the kernels of
+        its first J graph neighbours (a lower bound of its normaliser)."""
+        N, D = X.shape
+        self.lbound = torch.empty(N, device=X.device)
+        check(_lib.load().lakde_screen_knn_bound(ptr(rec), ptr(X), N, D, ptr(graph),
+                                                 idx_code(graph.dtype), graph.size(1), J,
+                                                 ptr(self.lbound), stream_of(X)),
+              "screen_knn_bound")
+        return self.lbound
 
 
 def pack_screen(rec, D, origin, out):

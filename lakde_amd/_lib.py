@@ -107,7 +107,8 @@ _SIGNATURES = {
     "lakde_screen_points_bytes": (_sz, [_i64, _i64]),
     "lakde_screen_points": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp]),
     "lakde_pack_screen": (_int, [_vp, _i64, _i64, _vp, _vp, _vp]),
-    "lakde_estep_main_screen": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp, _vp, _vp]),
+    "lakde_screen_knn_bound": (_int, [_vp, _vp, _i64, _i64, _vp, _int, _i64, _i64, _vp, _vp]),
+    "lakde_estep_main_screen": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp, _vp, _vp, _vp]),
     "lakde_pack_records": (_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp]),
     "lakde_pred_backward_workspace": (_sz, [_i64, _i64, _i64]),
     "lakde_pred_backward": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -56,6 +56,8 @@ def _gen_inst(lo, hi):
             "uint8_t *, hipStream_t);",
             f"template int launch_screen_points<{d}>(const float *, int64_t, const float *, "
             "__bf16 *, float *, hipStream_t);",
+            f"template int launch_screen_knn_bound<{d}>(const float *, const float *, int64_t, "
+            "const void *, int, int64_t, int, float *, hipStream_t);",
         ]
     lines.append("}")
     text = "\n".join(lines) + "\n"

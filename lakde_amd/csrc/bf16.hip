@@ -453,16 +453,37 @@ int lakde_pack_screen(const float *rec, int64_t nb, int64_t D, const float *orig
   return rc;
 }
 
+int lakde_screen_knn_bound(const float *rec, const float *X, int64_t N, int64_t D,
+                           const void *graph, int graph_dtype, int64_t k, int64_t J, float *out,
+                           lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 8 && D <= 63, "the screening form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(rec && X && graph && out && N >= 0 && k >= 0, "bad args");
+  LAKDE_CHECK_ARG(graph_dtype == LAKDE_I32 || graph_dtype == LAKDE_I64, "graph must be int32/int64");
+  LAKDE_CHECK_ARG(J >= 1 && J <= 64 && (J & (J - 1)) == 0, "J must be a power of two <= 64");
+  hipStream_t s = as_stream(stream);
+  int rc = LAKDE_EINVAL;
+  switch (D) {
+#define CASE(d)                                                                               \
+  case d: rc = launch_screen_knn_bound<d>(rec, X, N, graph, graph_dtype, k, (int)J, out, s); break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) set_error("lakde_screen_knn_bound: launch failed");
+  return rc;
+}
+
 int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, const void *xb,
-                            const float *xnorm, lakde_stream_t stream) {
+                            const float *xnorm, const float *lbound, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(args != nullptr && frag && xb && xnorm, "null args");
   LAKDE_CHECK_ARG(args->D >= 8 && args->D <= 63, "the screening form needs 8 <= D <= 63");
   LAKDE_CHECK_ARG(args->nb <= 32767, "the screening E-step takes blocks of at most 32767 kernels");
   LAKDE_CHECK_ARG(args->col_perm == nullptr, "the screening E-step takes columns in index order");
+  LAKDE_CHECK_ARG(lbound == nullptr || args->mode == LAKDE_ESTEP_ACCUM,
+                  "a normaliser bound is an input of ACCUM passes only");
   EParams p;
   int rc = estep_fill_params(*args, &p);
   if (rc) return rc;
-  SParams sp{(const uint8_t *)frag, (const __bf16 *)xb, xnorm, p.skip_margin};
+  SParams sp{(const uint8_t *)frag, (const __bf16 *)xb, xnorm, p.skip_margin, lbound};
   hipStream_t s = as_stream(stream);
   rc = LAKDE_EINVAL;
   switch (args->D) {

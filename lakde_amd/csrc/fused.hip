@@ -415,9 +415,12 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   // normaliser: all of a block's skipped pairs together change the
   // normaliser by less than nb e^-30 (5e-11 relative for nb = 512, FP32
   // resolves 6e-8), and none can pass a threshold t (margin >= 8 - log t;
-  // threshold 0 keeps r != 0, i.e. down to 2^-149 = e^-103.3)
-  p->skip_margin = a.threshold > 0.f ? fmaxf(30.f + logf((float)a.nb / 512.f), 8.f - logf(a.threshold))
-                                     : 110.f;
+  // threshold 0 keeps r != 0, i.e. down to 2^-149 = e^-103.3); an ACCUM
+  // pass keeps nothing, so only the normaliser bound applies
+  const float norm_margin = 30.f + logf((float)a.nb / 512.f);
+  p->skip_margin = a.mode == LAKDE_ESTEP_ACCUM ? norm_margin
+                   : a.threshold > 0.f         ? fmaxf(norm_margin, 8.f - logf(a.threshold))
+                                               : 110.f;
   if (a.col_perm != nullptr)
     LAKDE_CHECK_ARG(a.chunk_box != nullptr && a.D <= kSkipMaxD,
                     "col_perm needs chunk_box and D <= 16");

@@ -70,6 +70,7 @@ struct SParams {
   const __bf16 *xb;      // N x Kp: bf16(x_m - o), zero padded
   const float *xnorm;    // N: |x_m - o| rounded up
   float margin;          // screening margin (nats)
+  const float *lbound;   // ACCUM (optional): a lower bound of each column's final normaliser
 };
 
 // ------------------------------------------------------------- preparation
@@ -137,6 +138,34 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
     scal[2] = r[rec_c(D) + 0];
     scal[3] = r[rec_c(D) + 1];
   }
+}
+
+template <int D>
+__device__ __forceinline__ float log_rho_lane(const float *rec, const float *__restrict__ xrow);
+
+// A lower bound of every column's normaliser for the first log-sum-exp pass
+// of a fit (ACCUM from -inf): the largest exact log rho over the kernels of
+// the column's first J graph neighbours.  The log-sum-exp over all kernels
+// is at least any of its terms (the maximum is not fooled by a repeated
+// neighbour); the diagonal and out-of-range entries are skipped.  One thread
+// per (point, neighbour), J (a power of two <= 64) consecutive lanes per
+// point.
+template <int D>
+__global__ void __launch_bounds__(256) screen_knn_bound_kernel(
+    const float *__restrict__ rec, const float *__restrict__ X, int64_t N,
+    const void *__restrict__ graph, int gdtype, int64_t gk, int J, float *__restrict__ out) {
+  constexpr int64_t R = rec_floats(D);
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t m = t / J;
+  const int j = (int)(t - m * J);
+  float v = neg_inf();
+  if (m < N && j < gk) {
+    const int64_t n = gdtype == LAKDE_I64 ? ((const int64_t *)graph)[m * gk + j]
+                                          : (int64_t)((const int32_t *)graph)[m * gk + j];
+    if (n >= 0 && n < N && n != m) v = log_rho_lane<D>(rec + n * R, X + m * D);
+  }
+  for (int d = J / 2; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, kWave));
+  if (m < N && j == 0) out[m] = v;
 }
 
 // ---------------------------------------------------------------- kernel
@@ -259,8 +288,10 @@ estep_screen_kernel(EParams p, SParams sp) {
   }
   if (MODE != LAKDE_ESTEP_UPDATE && valid) cm = p.c[m];
   // running lower bound of the new normaliser (ACCUM: the new normaliser
-  // logaddexp(c, LSE) is at least c)
+  // logaddexp(c, LSE) is at least c, and the final one at least the given
+  // bound)
   float B = MODE == LAKDE_ESTEP_ACCUM ? cm : lb;
+  if (MODE == LAKDE_ESTEP_ACCUM && sp.lbound != nullptr && valid) B = fmaxf(B, sp.lbound[m]);
   Lse lse;
   lse.init();
   int kcnt = 0;           // UPDATE: keep candidates queued (may exceed kKeepQ)
@@ -507,7 +538,23 @@ int launch_screen_points(const float *X, int64_t N, const float *origin, __bf16 
   }
 }
 
+template <int D>
+int launch_screen_knn_bound(const float *rec, const float *X, int64_t N, const void *graph,
+                            int gdtype, int64_t gk, int J, float *out, hipStream_t s) {
+  if constexpr (D < 8) {
+    return LAKDE_EINVAL;
+  } else {
+    if (N == 0) return LAKDE_OK;
+    hipLaunchKernelGGL((screen_knn_bound_kernel<D>), dim3((unsigned)ceil_div(N * J, 256)),
+                       dim3(256), 0, s, rec, X, N, graph, gdtype, gk, J, out);
+    return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+  }
+}
+
 #define LAKDE_DECL_SCREEN_D(d)                                                                \
+  extern template int launch_screen_knn_bound<d>(const float *, const float *, int64_t,       \
+                                                 const void *, int, int64_t, int, float *,    \
+                                                 hipStream_t);                                 \
   extern template int launch_estep_screen<d>(const EParams &, const SParams &, int, hipStream_t); \
   extern template int launch_pack_screen<d>(const float *, int64_t, const float *, uint8_t *,  \
                                             hipStream_t);                                       \

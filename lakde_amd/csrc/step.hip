@@ -187,7 +187,7 @@ int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
   }
   if (a->ev_main[0]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[0], s));
   if (a->bf16_frag != nullptr && a->screen_xb != nullptr)
-    rc = lakde_estep_main_screen(&em, a->bf16_frag, a->screen_xb, a->screen_xnorm, s);
+    rc = lakde_estep_main_screen(&em, a->bf16_frag, a->screen_xb, a->screen_xnorm, nullptr, s);
   else if (a->bf16_frag != nullptr)
     rc = lakde_estep_main_bf16(&em, a->bf16_frag, s);
   else

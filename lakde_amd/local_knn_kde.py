@@ -74,18 +74,26 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
 
     # lakde/local_knn_kde.py:140-193
     def init_summaries_(self, X, sparsity_threshold=None):
+        """The posterior of a block reads only the block's own rows of the
+        initial responsibilities, which neither loop changes before it
+        reads them: the posteriors of all blocks are computed once, before
+        the log-sum-exp passes (the reference's second loop recomputes the
+        same values), so that the screening E-step can start its passes from
+        a lower bound of every normaliser."""
         N = X.size(0)
         for b in self._blocks(N):
             s, e, nb = self._span(b, N)
             self._block_posterior(X, b, self.nu[s:e], self.logdet_W[s:e],
                                   self.sigma_0_lambda_mm_trace[s:e], self.logdet_sigma_0[s:e],
                                   update_b=False)
+        if self._screen is not None:
+            self._screen.knn_bound(self.rec, X, self.knn_graph)
+        for b in self._blocks(N):
             self._init_lse_pass(X, b)
+        if self._screen is not None:
+            self._screen.lbound = None
         self._active = 0.0
         for b in self._blocks(N):
-            s, e, nb = self._span(b, N)
-            self._block_posterior(X, b, self._nu_s[:nb], self._logdet_s[:nb],
-                                  self._trace_s[:nb], self._logdet_prior_s[:nb], update_b=False)
             self._init_fixed_pass(X, b, sparsity_threshold)
         self._init_rebalance()
 

@@ -1,11 +1,17 @@
-"""Precision form (BF16 matrix cores, bf16x3 split precision) against the
-FP32 GPU path and the FP64 oracle.
+"""Precision form against the FP32 GPU path and the FP64 oracle.
 
-Bars, written per assertion: the E-step normalisers within 2e-2 nats of the
-FP32 kernels per column; the fitted model's held-out log density within
+The fit's E-step is the screening form (every pair screened on the BF16
+matrix cores with a rigorous error bound, the pairs that can matter
+recomputed exactly in FP32); the bf16x3 E-step and predictive kernels
+(split-precision products on the matrix cores) are tested beside it.
+
+Bars, written per assertion: screening normalisers within float32
+summation-order differences of the FP32 kernels (1e-6 relative) and its
+FIXED-mode CSR identical; the bf16x3 E-step normalisers within 2e-2 nats of
+the FP32 kernels per column; the fitted model's held-out log density within
 1e-3 nats/point (mean absolute) of the FP32 path and of the FP64 oracle;
-the predictive kernel within 1e-2 nats per point of the FP32 kernel on the
-same posterior.
+the bf16x3 predictive kernel within 1e-2 nats per point of the FP32 kernel
+on the same posterior.
 """
 
 import numpy as np
@@ -129,6 +135,60 @@ def test_screen_estep_matches_fp32_kernels(name, n, k, t):
         np.testing.assert_array_equal(r1, r0)
         np.testing.assert_array_equal(c1, c0)
         np.testing.assert_array_equal(v1, v0)
+
+
+@pytest.mark.parametrize("name,n,k", [("mix128_d63", 3000, 96), ("latent_d43", 2500, 32)])
+def test_screen_init_matches_fp32(name, n, k):
+    """The initialisation of the precision form (log-sum-exp passes started
+    from the graph-neighbour bound of every normaliser, then the keep pass)
+    against the FP32 path's: the normalisers within float32 summation-order
+    differences, the initial CSR identical entry for entry, and the bound
+    below every final normaliser."""
+    from lakde_amd import LocalKNearestKDE
+    from lakde_amd import _fused as F
+    X, _ = _screen_case(name, n, k)
+    graph = O.compute_knn_graph(X, k)
+    Xd = _t(X)
+    t = 1 / k
+    ms = {}
+    for prec in ("fp32", "bf16"):
+        m = LocalKNearestKDE(k_or_knn_graph=_t(graph), block_size=512, precision=prec)
+        m.init_parameters_(Xd, t)
+        m.init_summaries_(Xd, t)
+        ms[prec] = m
+    a, b = ms["fp32"], ms["bf16"]
+    np.testing.assert_allclose(b.rnm_log_consts.cpu().numpy(), a.rnm_log_consts.cpu().numpy(),
+                               rtol=1e-6, atol=1e-5)
+    # the normalisers differ in the last bits, so a value at the threshold
+    # may be kept by one form only (values after the column renormalisation)
+    flips, total = 0, 0
+    a.materialize()
+    b.materialize()
+    for blk in range(a.arena.num_blocks):
+        ent = []
+        for mm in (a, b):
+            nz = mm.arena.nnz[blk]
+            r = mm.arena.row[blk, :nz].cpu().numpy().astype(np.int64)
+            c = mm.arena.col[blk, :nz].cpu().numpy().astype(np.int64)
+            v = mm.arena.val[blk, :nz].cpu().numpy()
+            ent.append(dict(zip((r << 32 | c).tolist(), v.tolist())))
+        ea, eb = ent
+        for key in set(ea) ^ set(eb):
+            v = ea.get(key, eb.get(key))
+            assert v <= 2 * t, f"block {blk}: entry {key} value {v} kept by one form"
+            flips += 1
+        total += len(ea)
+        common = sorted(set(ea) & set(eb))
+        np.testing.assert_allclose([eb[q] for q in common], [ea[q] for q in common],
+                                   rtol=1e-5, atol=1e-9)
+    print(f"[{name}] entries kept by one form only: {flips} of {total}")
+    assert flips <= max(2, 1e-4 * total)
+    pts = F.ScreenPoints(Xd)
+    lb = pts.knn_bound(b.rec, Xd, b.knn_graph).cpu().numpy()
+    c = b.rnm_log_consts.cpu().numpy()
+    print(f"[{name}] normaliser - bound: min {np.min(c - lb):.3e} median "
+          f"{np.median(c - lb):.3f} nats")
+    assert np.all(lb <= c + 1e-5 * np.abs(c))
 
 
 @pytest.mark.parametrize("name,n,m,k,iters", [("mix64_d21", 4000, 400, 32, 2),
