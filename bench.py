@@ -39,7 +39,13 @@ SECONDARY = "mix32_d6"
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) scaled by what the same
 # counters read for a plain copy of known size; valid for the kernel source
 # whose sha256 is recorded (the line reports null for any other source).
-MEASURED_TRAFFIC = None
+MEASURED_TRAFFIC = {
+    "kernel": "estep_screen_kernel<63, 2>",
+    "source_sha256": "614dd0767a3e7df85ff520835ed030686a78d96b5d80550dbbe9de24ff38647c",
+    "fetch_bytes": 3918575000, "write_bytes": 4144000,
+    "profile": "profiles/estep_screen_mix128_d63.md: FETCH_SIZE 1,913,366.9 KiB x 2.0000 + "
+               "WRITE_SIZE 4,046.9 KiB x 1.0000 per dispatch (copy-probe scale factors)",
+}
 
 
 def _parse():
