@@ -34,6 +34,16 @@ extern "C" {
 
 typedef void *lakde_stream_t; /* hipStream_t */
 
+/* An order-independent column sum: a 128-bit fixed-point integer (value *
+ * 2^80 in hi:lo; the top bit of hi flags NaN).  The kernels add values to it
+ * with integer atomics, so the sum does not depend on the order the adds
+ * arrive in and a fit is reproducible run to run; a value added and later
+ * removed cancels exactly.  Arrays of them must read zero before first use
+ * (device memory cleared to zero). */
+typedef struct {
+  uint64_t lo, hi;
+} lakde_fx128;
+
 /* status codes */
 #define LAKDE_OK 0
 #define LAKDE_EINVAL 1      /* bad argument (size, stride, null pointer) */
@@ -56,6 +66,7 @@ typedef void *lakde_stream_t; /* hipStream_t */
 /* prior modes (lakde_posterior) */
 #define LAKDE_PRIOR_KNN 0    /* per-point kNN covariance, factored with jitter   */
 #define LAKDE_PRIOR_SHARED 1 /* one shared D x D prior covariance                */
+#define LAKDE_PRIOR_KNN_CACHED 2 /* kNN prior read from lakde_knn_prior_cache's cache */
 
 int lakde_version(void);
 const char *lakde_last_error(void);
@@ -135,7 +146,7 @@ int lakde_posterior_contribs(const void *rowptr, int rowptr_dtype, const void *c
 typedef struct {
   int64_t nb, D;
   int prior_mode;
-  const float *prior;     /* kNN: nb x D x D; shared: D x D */
+  const float *prior;     /* kNN: nb x D x D; shared: D x D; cached: nb cache entries */
   float *prior_jitter;    /* kNN: nb, in/out; shared: NULL */
   const float *S;         /* nb x D x D */
   const float *nu;        /* nb */
@@ -156,6 +167,24 @@ typedef struct {
 
 int lakde_posterior(const lakde_posterior_args *args, lakde_stream_t stream);
 
+/* The kNN prior's jittered factor, cached for a fit: X and the graph do not
+ * change during a fit, so the prior covariance of every point and, with its
+ * persistent jitter, its factor are those of the first computation
+ * (lakde/local_knn_kde.py:195-207 recomputes both every partial step and
+ * every prediction).  For rows [row0, row0+nb): lakde_knn_prior_cov into
+ * prior_scratch (nb x D x D), then each factored with the stabilised jitter
+ * schedule (prior_jitter in/out, logdet_prior out, *status = 1 when it does
+ * not converge), and cache entry n (lakde_prior_cache_floats(D) floats) =
+ * prior + j0 I then its Cholesky factor, both packed lower row-major.  A
+ * posterior with prior_mode LAKDE_PRIOR_KNN_CACHED reads prior = the block's
+ * entries and leaves prior_jitter, prior_tril and logdet_prior alone (D >= 7;
+ * the fused small-D posterior computes its prior in place). */
+int64_t lakde_prior_cache_floats(int64_t D);
+int lakde_knn_prior_cache(const float *X, int64_t N, int64_t D, int64_t row0, int64_t nb,
+                          const void *graph, int graph_dtype, int64_t K, float *prior_scratch,
+                          float *prior_jitter, float *logdet_prior, float *cache, int32_t *status,
+                          lakde_stream_t stream);
+
 /* One block's posterior from one call: for a kNN prior (knn_graph = the
  * block's nb x knn_k graph rows, int32/int64) lakde_knn_prior_cov into
  * post->prior, then lakde_posterior_contribs of the block's CSR (rowptr nb+1,
@@ -171,9 +200,10 @@ int lakde_block_posterior(const lakde_posterior_args *post, const float *X, int6
                           int knn_graph_dtype, int64_t knn_k, lakde_stream_t stream);
 
 /* Column sums of one block's responsibilities: sums[m] = sum_{e: col[e]=m} val[e]
- * (sums is cleared first).  (lakde/local_adaptive_kde.py:120) */
+ * (written, not accumulated), summed order-independently in ws (N
+ * lakde_fx128, zero on entry and left zero).  (lakde/local_adaptive_kde.py:120) */
 int lakde_colsum(const int32_t *col, const float *val, const int32_t *nnz_dev, int64_t N,
-                 float *sums, lakde_stream_t stream);
+                 float *sums, lakde_fx128 *ws, lakde_stream_t stream);
 
 /* Fused E-step of one kernel block over all N points
  * (lakde/local_adaptive_kde.py:36-44,98-153, lakde/local_knn_kde.py:157-166):
@@ -206,7 +236,7 @@ typedef struct {
   size_t workspace_bytes;
   /* optional (UPDATE): the unsafe-step test of lakde_rebalance_lazy_check
    * fused into the main kernel: *lazy_unsafe |= 1 for a column whose
-   * lazy_G[m] clamp(exp(c_old - c)) leaves [2^-60, 2^60] (the caller clears
+   * lazy_G[m] clamp(exp(c_old - c)) leaves [2^-40, 2^40] (the caller clears
    * *lazy_unsafe beforehand); NULL: no test */
   const float *lazy_G;
   int32_t *lazy_unsafe;
@@ -241,49 +271,54 @@ int lakde_estep_compact(const lakde_estep_args *args, lakde_stream_t stream);
  * block arena: block b's entries live at [b*cap, b*cap + nnz[b]).  Blocks other
  * than `exclude_block` are scaled by clamp(exp(c_prev - c), tiny, max)[col];
  * column sums s are clamped to eps; c += log s; every value is scaled by
- * clamp(1/s, tiny, max)[col].  c_prev may alias c.  ws: 2*N floats. */
+ * clamp(1/s, tiny, max)[col].  c_prev may alias c.  The column sums are
+ * order-independent: s = coeff (sum of the other blocks' old values) + (sum
+ * of the excluded block's values), each a lakde_fx128 sum.  ws:
+ * lakde_rebalance_workspace(N) bytes, zero on first use and left so. */
+size_t lakde_rebalance_workspace(int64_t N);
 int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_block,
                     const int32_t *nnz, const int32_t *cols, float *vals,
-                    const float *c_prev, float *c, float *ws, lakde_stream_t stream);
+                    const float *c_prev, float *c, void *ws, lakde_stream_t stream);
 
 /* The same rebalance with the responsibilities stored unscaled (lazy
  * rebalance): the value of entry e is val[e] * G[col[e]] with per-column
- * factors G (float) and stored column totals U (double).  A partial step on
- * block i then touches only block i's entries and O(N) column state instead
- * of every block:
+ * factors G (float) and stored column totals U.  A partial step on block i
+ * then touches only block i's entries and O(N) column state instead of
+ * every block:
  *   lakde_rebalance_lazy_begin (before lakde_estep, ESTEP_UPDATE):
- *     old_colsum[m] += sum over block i of val*G (the true old column sums
- *     the E-step reads), U_old[m] += sum of block i's stored values,
- *     *unsafe = 0;
+ *     old_colsum[m] = sum over block i of val*G (the true old column sums
+ *     the E-step reads; written), U_old[m] += sum of block i's stored
+ *     values, *unsafe = 0;
  *   lakde_rebalance_lazy_check (after the E-step wrote c):
  *     *unsafe = 1 if any G[m] clamp(exp(c_prev - c)) lies outside
- *     [2^-60, 2^60] or is NaN; the stored form cannot represent that step
- *     (float under/overflow) and the caller must instead fold
- *     (lakde_rebalance_fold), run lakde_rebalance, fold again and clear
- *     old_colsum and U_old;
+ *     [2^-40, 2^40] or is NaN; the stored form cannot represent that step
+ *     and the caller must instead fold (lakde_rebalance_fold), run
+ *     lakde_rebalance, fold again and clear U_old;
  *   lakde_rebalance_lazy (otherwise; block i now holds true values r):
  *     coeff = clamp(exp(c_prev - c), tiny, max), val = r / (G coeff),
  *     s = max(coeff G (U - U_old) + sum r, eps), c += log s,
  *     G *= coeff clamp(1/s, tiny, max), U += sum val - U_old,
- *     and old_colsum, U_old are cleared again.
- * old_colsum (N floats) and U_old (N doubles) are zero between steps; ws is
- * 2*N doubles, zero on entry and left zero.  With col, val, nnz_dev NULL no
- * block was updated (the initial normalisation, c_prev == c).  c_prev may
- * alias c. */
+ *     and U_old is cleared again.
+ * U, U_old: N lakde_fx128 each (order-independent sums: U - U_old is exact,
+ * so a column whose entries all lie in block i reads exactly zero); U_old is
+ * zero between steps; ws is 3*N lakde_fx128, zero on entry and left zero.
+ * With col, val, nnz_dev NULL no block was updated (the initial
+ * normalisation, c_prev == c).  c_prev may alias c. */
 int lakde_rebalance_lazy_begin(int64_t N, const int32_t *col, const float *val,
                                const int32_t *nnz_dev, const float *G, float *old_colsum,
-                               double *U_old, int32_t *unsafe, lakde_stream_t stream);
+                               lakde_fx128 *U_old, lakde_fx128 *ws, int32_t *unsafe,
+                               lakde_stream_t stream);
 int lakde_rebalance_lazy_check(int64_t N, const float *c_prev, const float *c, const float *G,
                                int32_t *unsafe, lakde_stream_t stream);
 int lakde_rebalance_lazy(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
-                         const float *c_prev, float *c, float *G, double *U, double *U_old,
-                         float *old_colsum, double *ws, lakde_stream_t stream);
+                         const float *c_prev, float *c, float *G, lakde_fx128 *U,
+                         lakde_fx128 *U_old, lakde_fx128 *ws, lakde_stream_t stream);
 
 /* Fold the column factors into the stored values of every block of the
  * arena (val *= G[col]), then G = 1 and U = the exact stored column sums.
  * Bounds the drift of U; afterwards the arena holds true values. */
 int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32_t *nnz,
-                         const int32_t *cols, float *vals, float *G, double *U,
+                         const int32_t *cols, float *vals, float *G, lakde_fx128 *U,
                          lakde_stream_t stream);
 
 /* ------------------------------------------------------------------ *
@@ -335,9 +370,10 @@ typedef struct {
   int32_t *cols_all;       /* num_blocks x cap */
   float *vals_all;         /* num_blocks x cap */
   float *G;                /* N, lazy rebalance factors, or NULL (exact) */
-  double *U, *U_old;       /* N each (lazy) */
-  double *lazy_ws;         /* 2 N (lazy) */
-  float *rebal_ws;         /* 2 N: exact rebalance and the lazy fallback */
+  lakde_fx128 *U, *U_old;  /* N each (lazy) */
+  lakde_fx128 *lazy_ws;    /* 3 N (lazy) */
+  void *rebal_ws;          /* lakde_rebalance_workspace(N) bytes: exact rebalance, the
+                              exact path's old column sums and the lazy fallback */
   int fold_first;          /* lazy: fold G into the values before the step */
   void *bf16_frag;         /* nb * lakde_bf16_frag_bytes(D) bytes, or NULL */
   int32_t *host_flags;     /* pinned host, num_blocks + 2 */

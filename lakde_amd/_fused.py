@@ -100,6 +100,22 @@ def knn_prior_cov(X, row0, graph_blk, out):
     return out
 
 
+def prior_cache_floats(D):
+    return int(_lib.load().lakde_prior_cache_floats(D))
+
+
+def knn_prior_cache(X, row0, graph_blk, scratch, prior_jitter, logdet_prior, cache, status):
+    """The kNN prior of rows row0.. factored once into cache (one entry per
+    row: prior + j0 I and its factor, packed)."""
+    N, D = X.shape
+    nb, K = graph_blk.shape
+    check(_lib.load().lakde_knn_prior_cache(ptr(X), N, D, row0, nb, ptr(graph_blk),
+                                            idx_code(graph_blk.dtype), K, ptr(scratch),
+                                            ptr(prior_jitter), ptr(logdet_prior), ptr(cache),
+                                            ptr(status), stream_of(X)),
+          "knn_prior_cache")
+
+
 def posterior_contribs(rowptr, col, val, Xn, X, nu0, S, nu, colscale=None):
     nb, D = Xn.shape
     check(_lib.load().lakde_posterior_contribs(ptr(rowptr), idx_code(rowptr.dtype), ptr(col),
@@ -136,9 +152,27 @@ def block_posterior(X, row0, rowptr, col, val, colscale, graph_blk, post_args):
           "block_posterior")
 
 
-def colsum(col, val, nnz_dev, N, out):
-    check(_lib.load().lakde_colsum(ptr(col), ptr(val), ptr(nnz_dev), N, ptr(out), stream_of(val)),
+def colsum(col, val, nnz_dev, N, out, ws=None):
+    """out[m] = sum of the block's values in column m (an order-independent
+    sum; ws: N zeroed 16-byte accumulators, left zero)."""
+    if ws is None:
+        ws = fx_zeros(N, out.device)
+    check(_lib.load().lakde_colsum(ptr(col), ptr(val), ptr(nnz_dev), N, ptr(out), ptr(ws),
+                                   stream_of(val)),
           "colsum")
+    return out
+
+
+def fx_zeros(n, device):
+    """n zeroed lakde_fx128 accumulators (order-independent column sums)."""
+    return torch.zeros(n, 2, dtype=torch.int64, device=device)
+
+
+def rebalance_workspace(N, device):
+    """The exact rebalance's workspace: zeroed (its column sums must read
+    zero on first use) and left so."""
+    nbytes = int(_lib.load().lakde_rebalance_workspace(N))
+    return torch.zeros((nbytes + 15) // 16, 2, dtype=torch.int64, device=device)
 
 
 class EstepWorkspace:
@@ -239,20 +273,21 @@ def rebalance(arena, exclude_block, c_prev, c, ws):
 
 class LazyRebalance:
     """Column state of the lazy rebalance: the arena holds stored values u
-    whose true values are u * G[col]; U (float64) are the stored column
-    totals.  ``old_colsum`` (the true old column sums of the updated block,
-    read by the E-step), ``U_old`` and the workspace stay zero between
-    steps; ``exact_ws`` is the exact rebalance's workspace for the steps the
-    stored form cannot represent."""
+    whose true values are u * G[col]; U are the stored column totals
+    (order-independent fixed-point sums, N x 2 int64 = lakde_fx128).
+    ``old_colsum`` holds the true old column sums of the updated block (read
+    by the E-step); ``U_old`` and the workspace (3 N accumulators) stay zero
+    between steps; ``exact_ws`` is the exact rebalance's workspace for the
+    steps the stored form cannot represent."""
 
     def __init__(self, N, device):
         self.N = N
         self.G = torch.ones(N, device=device)
-        self.U = torch.zeros(N, dtype=torch.float64, device=device)
-        self.U_old = torch.zeros(N, dtype=torch.float64, device=device)
+        self.U = fx_zeros(N, device)
+        self.U_old = fx_zeros(N, device)
         self.old_colsum = torch.zeros(N, device=device)
-        self.ws = torch.zeros(2 * N, dtype=torch.float64, device=device)
-        self.exact_ws = torch.empty(2 * N, device=device)
+        self.ws = fx_zeros(3 * N, device)
+        self.exact_ws = rebalance_workspace(N, device)
         self.dirty = False  # G != 1 somewhere
 
     def end(self, arena, b, c_prev, c):
@@ -262,7 +297,7 @@ class LazyRebalance:
             blk = (ptr(arena.col[b]), ptr(arena.val[b]), ptr(arena.nnz_dev[b:b + 1]))
         check(_lib.load().lakde_rebalance_lazy(
             self.N, blk[0], blk[1], blk[2], ptr(c_prev), ptr(c), ptr(self.G), ptr(self.U),
-            ptr(self.U_old), ptr(self.old_colsum), ptr(self.ws), stream_of(c)),
+            ptr(self.U_old), ptr(self.ws), stream_of(c)),
             "rebalance_lazy")
         self.dirty = True
 

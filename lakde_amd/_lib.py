@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("LAKDE_AMD_LIB", os.path.join(_HERE, "liblakde_amd.so"
 LAKDE_I32, LAKDE_I64, LAKDE_F32, LAKDE_F64 = 0, 1, 2, 3
 LAKDE_OK, LAKDE_EINVAL, LAKDE_ECAPACITY, LAKDE_ESTABILIZE = 0, 1, 4, 5
 ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE = 0, 1, 2
-PRIOR_KNN, PRIOR_SHARED = 0, 1
+PRIOR_KNN, PRIOR_SHARED, PRIOR_KNN_CACHED = 0, 1, 2
 
 _i64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -72,19 +72,22 @@ _SIGNATURES = {
     "lakde_posterior_contribs": (_int, [_vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _i64, _i64,
                                         _i64, _f32, _vp, _vp, _vp]),
     "lakde_posterior": (_int, [ctypes.POINTER(PosteriorArgs), _vp]),
+    "lakde_prior_cache_floats": (_i64, [_i64]),
+    "lakde_knn_prior_cache": (_int, [_vp, _i64, _i64, _i64, _i64, _vp, _int, _i64, _vp, _vp, _vp,
+                                     _vp, _vp, _vp]),
     "lakde_block_posterior": (_int, [ctypes.POINTER(PosteriorArgs), _vp, _i64, _i64, _vp, _vp, _vp,
                                      _vp, _vp, _int, _i64, _vp]),
-    "lakde_colsum": (_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "lakde_colsum": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "lakde_estep_workspace": (_sz, [_i64, _i64]),
     "lakde_estep": (_int, [ctypes.POINTER(EstepArgs), _vp]),
     "lakde_estep_compact": (_int, [ctypes.POINTER(EstepArgs), _vp]),
     "lakde_estep_main": (_int, [ctypes.POINTER(EstepArgs), _vp]),
     "lakde_estep_finish": (_int, [ctypes.POINTER(EstepArgs), _vp]),
+    "lakde_rebalance_workspace": (_sz, [_i64]),
     "lakde_rebalance": (_int, [_i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "lakde_rebalance_lazy_begin": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lakde_rebalance_lazy_begin": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lakde_rebalance_lazy_check": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
-    "lakde_rebalance_lazy": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                    _vp]),
+    "lakde_rebalance_lazy": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lakde_rebalance_fold": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lakde_flag_words_alloc": (_int, [ctypes.POINTER(_vp)]),
     "lakde_flag_words_free": (_int, [_vp]),

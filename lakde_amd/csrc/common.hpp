@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/lakde_amd.h"
+#include "fixed.hpp"
 
 namespace lakde {
 
@@ -78,23 +79,28 @@ __device__ __forceinline__ float wave_max(float v) {
 
 // column sums of a block's old entries accumulated by the contributions
 // kernel for the lazy rebalance (all null: not accumulated)
+// sums[m] = acc[m] as a float, acc cleared (the end of an order-independent
+// column sum; fused.hip)
+int colsum_finish(int64_t N, Fx *acc, float *sums, hipStream_t s);
+
+// (order-independent fixed-point sums)
 struct OldSums {
-  float *colsum;         // true values (stored value x column factor)
-  double *stored;        // stored values
+  Fx *colsum;            // true values (stored value x column factor)
+  Fx *stored;            // stored values
   int32_t *unsafe_reset; // cleared by the launch
 };
 
 int posterior_contribs_lazy(const int32_t *rowptr, const int32_t *col, const float *value,
                             const float *colscale, const float *Xn, const float *Xm, int64_t nb,
-                            int64_t D, float nu0, float *S, float *nu, float *old_colsum,
-                            double *U_old, int32_t *unsafe, hipStream_t s);
+                            int64_t D, float nu0, float *S, float *nu, Fx *old_fx, Fx *U_old,
+                            int32_t *unsafe, hipStream_t s);
 
 // one block's posterior (post_small.hip): [kNN prior covariance] +
 // contributions of the block's CSR [+ lazy old column sums] + factorisation
 int block_posterior(const lakde_posterior_args &post, const float *X, int64_t N, int64_t row0,
                     const int32_t *rowptr, const int32_t *col, const float *val,
                     const float *colscale, const void *graph, int graph_dtype, int64_t K,
-                    float nu0, float *old_colsum, double *U_old, int32_t *unsafe, hipStream_t s,
+                    float nu0, Fx *old_fx, Fx *U_old, int32_t *unsafe, hipStream_t s,
                     const int32_t *abort = nullptr);
 
 // Packed record layout (floats): [ Linv packed lower row-major: T = D(D+1)/2 ]

@@ -122,24 +122,25 @@ __global__ void __launch_bounds__(1024) estep_offsets_kernel(EParams p) {
 
 // ------------------------------------------------------------ rebalance
 
-// coeff[m] = clamp(exp(c_prev[m] - c[m]), tiny, max); sums[m] = 0
+// coeff[m] = clamp(exp(c_prev[m] - c[m]), tiny, max)
 __global__ void __launch_bounds__(256) rebal_coeff_kernel(int64_t N, const float *c_prev,
-                                                         const float *c, float *coeff,
-                                                         float *sums) {
+                                                         const float *c, float *coeff) {
   int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (m >= N) return;
   coeff[m] = clamp_nan(expf(c_prev[m] - c[m]), FLT_MIN, FLT_MAX);
-  sums[m] = 0.f;
 }
 
-// other blocks: v *= coeff[col]; every entry: sums[col] += v.
+// other blocks: v *= coeff[col], their old values summed in A[col]; the
+// excluded block's values summed in B[col] (order-independent sums, so the
+// column factor is applied to the sum, per column, afterwards).
 // grid.y = block, grid.x strides over the block's nnz entries.
 __global__ void __launch_bounds__(256) rebal_scale_sum_kernel(int64_t cap, int64_t exclude,
                                                              const int32_t *__restrict__ nnz,
                                                              const int32_t *__restrict__ cols,
                                                              float *__restrict__ vals,
                                                              const float *__restrict__ coeff,
-                                                             float *__restrict__ sums) {
+                                                             Fx *__restrict__ A,
+                                                             Fx *__restrict__ B) {
   const int64_t b = blockIdx.y;
   const int32_t n = nnz[b];
   const int32_t *cb = cols + b * cap;
@@ -147,21 +148,25 @@ __global__ void __launch_bounds__(256) rebal_scale_sum_kernel(int64_t cap, int64
   const bool scale = b != exclude;
   for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const int32_t col = cb[e];
-    float v = vb[e];
+    const float v = vb[e];
     if (scale) {
-      v = v * coeff[col];
-      vb[e] = v;
+      vb[e] = v * coeff[col];
+      fx_atomic_add(&A[col], v);
+    } else {
+      fx_atomic_add(&B[col], v);
     }
-    atomicAdd(&sums[col], v);
   }
 }
 
-// s = clamp_min(sums, eps); c += log s; coeff = clamp(1/s, tiny, max)
-__global__ void __launch_bounds__(256) rebal_norm_kernel(int64_t N, float *c, const float *sums,
+// s = clamp_min(coeff A + B, eps); c += log s; coeff = clamp(1/s, tiny, max);
+// A, B cleared
+__global__ void __launch_bounds__(256) rebal_norm_kernel(int64_t N, float *c, Fx *A, Fx *B,
                                                         float *coeff) {
   int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (m >= N) return;
-  float s = sums[m];
+  float s = (float)((double)coeff[m] * fx_value(A[m]) + fx_value(B[m]));
+  A[m] = fx_zero();
+  B[m] = fx_zero();
   s = (s != s) ? s : fmaxf(s, FLT_EPSILON);
   c[m] = c[m] + logf(s);
   coeff[m] = clamp_nan(1.0f / s, FLT_MIN, FLT_MAX);
@@ -182,31 +187,42 @@ __global__ void __launch_bounds__(256) rebal_rescale_kernel(int64_t cap,
 
 // ------------------------------------------------------- lazy rebalance
 // Stored value u_e, true value u_e * G[col_e]; U[m] = sum of stored values
-// of column m (double, so that a column that empties reads exactly zero up
-// to 1e-16 and is clamped to eps like the exact rebalance).
+// of column m, an order-independent fixed-point sum: U - U_old is exact, so
+// a column that empties reads exactly zero and is clamped to eps like the
+// exact rebalance.
 
 // before the E-step: true column sums of the block (for the normaliser
-// update) and its stored column sums
+// update; finished into floats by colsum_finish_kernel) and its stored
+// column sums
 __global__ void __launch_bounds__(256) lazy_begin_kernel(const int32_t *__restrict__ cols,
                                                         const float *__restrict__ vals,
                                                         const int32_t *__restrict__ nnz_dev,
                                                         const float *__restrict__ G,
-                                                        float *__restrict__ old_colsum,
-                                                        double *__restrict__ Uold,
+                                                        Fx *__restrict__ old_fx,
+                                                        Fx *__restrict__ Uold,
                                                         int32_t *__restrict__ unsafe) {
   const int32_t n = *nnz_dev;
   if (blockIdx.x == 0 && threadIdx.x == 0) *unsafe = 0;
   for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const int32_t col = cols[e];
     const float u = vals[e];
-    atomicAdd(&old_colsum[col], u * G[col]);
-    atomicAdd(&Uold[col], (double)u);
+    fx_atomic_add(&old_fx[col], u * G[col]);
+    fx_atomic_add(&Uold[col], u);
   }
 }
 
+// sums[m] = float(acc[m]), acc cleared
+__global__ void __launch_bounds__(256) colsum_finish_kernel(int64_t N, Fx *__restrict__ acc,
+                                                           float *__restrict__ sums) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  sums[m] = (float)fx_value(acc[m]);
+  acc[m] = fx_zero();
+}
+
 // after the E-step, before the host reads the entry count: flag columns
-// where G coeff leaves [2^-60, 2^60] (or is NaN), for which the stored
-// representation would under/overflow; the caller then rebalances exactly
+// where G coeff leaves [kLazyGcLo, kLazyGcHi] (or is NaN), which the stored
+// representation does not cover; the caller then rebalances exactly
 __global__ void __launch_bounds__(256) lazy_check_kernel(int64_t N, const float *__restrict__ c_prev,
                                                         const float *__restrict__ c,
                                                         const float *__restrict__ G,
@@ -215,7 +231,7 @@ __global__ void __launch_bounds__(256) lazy_check_kernel(int64_t N, const float 
   bool bad = false;
   if (m < N) {
     const float gc = G[m] * clamp_nan(expf(c_prev[m] - c[m]), FLT_MIN, FLT_MAX);
-    bad = !(gc >= 0x1p-60f && gc <= 0x1p60f);
+    bad = !(gc >= kLazyGcLo && gc <= kLazyGcHi);
   }
   if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(unsafe, 1);
 }
@@ -235,8 +251,8 @@ __global__ void __launch_bounds__(256) lazy_block_kernel(const int32_t *__restri
                                                         const float *__restrict__ c_prev,
                                                         const float *__restrict__ c,
                                                         const float *__restrict__ G,
-                                                        double *__restrict__ Rnew,
-                                                        double *__restrict__ Unew,
+                                                        Fx *__restrict__ Rnew,
+                                                        Fx *__restrict__ Unew,
                                                         LazyGuard guard) {
   if (lazy_skip(guard)) return;
   const int32_t n = *nnz_dev;
@@ -246,38 +262,37 @@ __global__ void __launch_bounds__(256) lazy_block_kernel(const int32_t *__restri
     const float coeff = clamp_nan(expf(c_prev[col] - c[col]), FLT_MIN, FLT_MAX);
     const float u = r / (G[col] * coeff);
     vals[e] = u;
-    atomicAdd(&Rnew[col], (double)r);
-    atomicAdd(&Unew[col], (double)u);
+    fx_atomic_add(&Rnew[col], r);
+    fx_atomic_add(&Unew[col], u);
   }
 }
 
 // per column: s = max(coeff G (U - Uold) + Rnew, eps), c += log s,
-// G *= coeff clamp(1/s), U += Unew - Uold; clears the per-step sums
+// G *= coeff clamp(1/s), U += Unew - Uold (exactly); clears the per-step sums
 __global__ void __launch_bounds__(256) lazy_norm_kernel(int64_t N, const float *__restrict__ c_prev,
                                                        float *__restrict__ c,
                                                        float *__restrict__ G,
-                                                       double *__restrict__ U,
-                                                       double *__restrict__ Uold,
-                                                       float *__restrict__ old_colsum,
-                                                       double *__restrict__ Rnew,
-                                                       double *__restrict__ Unew,
+                                                       Fx *__restrict__ U,
+                                                       Fx *__restrict__ Uold,
+                                                       Fx *__restrict__ Rnew,
+                                                       Fx *__restrict__ Unew,
                                                        LazyGuard guard) {
   const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (m >= N || lazy_skip(guard)) return;
   const float cp = c_prev[m], cn = c[m];
   const float coeff = clamp_nan(expf(cp - cn), FLT_MIN, FLT_MAX);
   const float g = G[m];
-  const double um = U[m] - Uold[m];
-  float s = (float)((double)(coeff * g) * um + Rnew[m]);
+  const Fx u = U[m], uo = Uold[m], un = Unew[m];
+  const double um = fx_diff(u, uo);
+  float s = (float)((double)(coeff * g) * um + fx_value(Rnew[m]));
   s = (s != s) ? s : fmaxf(s, FLT_EPSILON);
   c[m] = cn + logf(s);
   const float coeff2 = clamp_nan(1.0f / s, FLT_MIN, FLT_MAX);
   G[m] = g * coeff * coeff2;
-  U[m] = um + Unew[m];
-  Uold[m] = 0.0;
-  old_colsum[m] = 0.f;
-  Rnew[m] = 0.0;
-  Unew[m] = 0.0;
+  U[m] = fx_sub_add(u, uo, un);
+  Uold[m] = fx_zero();
+  Rnew[m] = fx_zero();
+  Unew[m] = fx_zero();
 }
 
 // u *= G[col] over every block, U = exact stored column sums
@@ -286,7 +301,7 @@ __global__ void __launch_bounds__(256) fold_values_kernel(int64_t cap,
                                                          const int32_t *__restrict__ cols,
                                                          float *__restrict__ vals,
                                                          const float *__restrict__ G,
-                                                         double *__restrict__ U) {
+                                                         Fx *__restrict__ U) {
   const int64_t b = blockIdx.y;
   const int32_t n = nnz[b];
   const int32_t *cb = cols + b * cap;
@@ -295,7 +310,7 @@ __global__ void __launch_bounds__(256) fold_values_kernel(int64_t cap,
     const int32_t col = cb[e];
     const float v = vb[e] * G[col];
     vb[e] = v;
-    atomicAdd(&U[col], (double)v);
+    fx_atomic_add(&U[col], v);
   }
 }
 
@@ -305,9 +320,9 @@ __global__ void __launch_bounds__(256) fold_reset_kernel(int64_t N, float *__res
   G[m] = 1.f;
 }
 
-__global__ void __launch_bounds__(256) zero_f64_kernel(double *p, int64_t n) {
+__global__ void __launch_bounds__(256) zero_fx_kernel(Fx *p, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+  if (i < n) p[i] = fx_zero();
 }
 
 // ----------------------------------------------------------- predictive
@@ -457,19 +472,26 @@ int pred_combine(const PParams &p, lakde_stream_t stream) {
 }
 
 int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
-                           const float *c_prev, float *c, float *G, double *U, double *U_old,
-                           float *old_colsum, double *ws, LazyGuard guard, hipStream_t s) {
-  LAKDE_CHECK_ARG(N >= 1 && c_prev && c && G && U && U_old && old_colsum && ws, "bad args");
+                           const float *c_prev, float *c, float *G, Fx *U, Fx *U_old, Fx *ws,
+                           LazyGuard guard, hipStream_t s) {
+  LAKDE_CHECK_ARG(N >= 1 && c_prev && c && G && U && U_old && ws, "bad args");
   LAKDE_CHECK_ARG((col && val && nnz_dev) || (!col && !val && !nnz_dev),
                   "col, val and nnz_dev must all be given or all be null");
-  double *Rnew = ws, *Unew = ws + N;
+  Fx *Rnew = ws, *Unew = ws + N;
   if (col) {
     hipLaunchKernelGGL(lazy_block_kernel, dim3(256), dim3(256), 0, s, col, val, nnz_dev, c_prev,
                        c, G, Rnew, Unew, guard);
     LAKDE_LAUNCHED();
   }
   hipLaunchKernelGGL(lazy_norm_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, N,
-                     c_prev, c, G, U, U_old, old_colsum, Rnew, Unew, guard);
+                     c_prev, c, G, U, U_old, Rnew, Unew, guard);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int colsum_finish(int64_t N, Fx *acc, float *sums, hipStream_t s) {
+  hipLaunchKernelGGL(colsum_finish_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, N,
+                     acc, sums);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }
@@ -477,8 +499,8 @@ int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int3
 // the compaction of a lazy-rebalance step, storing u = r / (G coeff) and
 // accumulating the column sums of r and u unless the step is unsafe
 // (lakde_rebalance_lazy's first half, fused)
-int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsafe, double *Rnew,
-                       double *Unew, hipStream_t s) {
+int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsafe, Fx *Rnew,
+                       Fx *Unew, hipStream_t s) {
   LAKDE_CHECK_ARG(a.mode == LAKDE_ESTEP_UPDATE && a.c_old && G && unsafe && Rnew && Unew,
                   "bad lazy compaction args");
   EParams p;
@@ -562,25 +584,31 @@ int lakde_estep_compact(const lakde_estep_args *args, lakde_stream_t stream) {
   return rc;
 }
 
+size_t lakde_rebalance_workspace(int64_t N) {
+  return N < 0 ? 0 : ((size_t)N * 4 + 15) / 16 * 16 + 2 * (size_t)N * sizeof(Fx);
+}
+
 int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_block,
                     const int32_t *nnz, const int32_t *cols, float *vals, const float *c_prev,
-                    float *c, float *ws, lakde_stream_t stream) {
+                    float *c, void *ws, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(N >= 1 && num_blocks >= 1 && cap >= 1, "bad size");
   LAKDE_CHECK_ARG(nnz && cols && vals && c_prev && c && ws, "null pointer");
+  LAKDE_CHECK_ARG(((uintptr_t)ws & 15) == 0, "ws must be 16-byte aligned");
   hipStream_t s = as_stream(stream);
   LAKDE_CHECK_ARG(num_blocks <= 65535 && cap < (1LL << 31), "too many blocks or entries");
-  float *coeff = ws, *sums = ws + N;
+  float *coeff = (float *)ws;
+  Fx *A = (Fx *)((char *)ws + ((size_t)N * 4 + 15) / 16 * 16), *B = A + N;
   const unsigned gN = (unsigned)ceil_div(N, 256);
   // x-extent: enough workgroups per block to cover ~all entries in one pass
   int64_t gx = ceil_div(cap, 256 * 4);
   if (gx > 64) gx = 64;
   const dim3 ge((unsigned)gx, (unsigned)num_blocks);
-  hipLaunchKernelGGL(rebal_coeff_kernel, dim3(gN), dim3(256), 0, s, N, c_prev, c, coeff, sums);
+  hipLaunchKernelGGL(rebal_coeff_kernel, dim3(gN), dim3(256), 0, s, N, c_prev, c, coeff);
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(rebal_scale_sum_kernel, ge, dim3(256), 0, s, cap, exclude_block, nnz, cols,
-                     vals, coeff, sums);
+                     vals, coeff, A, B);
   LAKDE_LAUNCHED();
-  hipLaunchKernelGGL(rebal_norm_kernel, dim3(gN), dim3(256), 0, s, N, c, sums, coeff);
+  hipLaunchKernelGGL(rebal_norm_kernel, dim3(gN), dim3(256), 0, s, N, c, A, B, coeff);
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(rebal_rescale_kernel, ge, dim3(256), 0, s, cap, nnz, cols, vals, coeff);
   LAKDE_LAUNCHED();
@@ -589,13 +617,16 @@ int lakde_rebalance(int64_t N, int64_t num_blocks, int64_t cap, int64_t exclude_
 
 int lakde_rebalance_lazy_begin(int64_t N, const int32_t *col, const float *val,
                                const int32_t *nnz_dev, const float *G, float *old_colsum,
-                               double *U_old, int32_t *unsafe, lakde_stream_t stream) {
-  LAKDE_CHECK_ARG(N >= 1 && col && val && nnz_dev && G && old_colsum && U_old && unsafe,
+                               lakde_fx128 *U_old, lakde_fx128 *ws, int32_t *unsafe,
+                               lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(N >= 1 && col && val && nnz_dev && G && old_colsum && U_old && ws && unsafe,
                   "bad args");
-  hipLaunchKernelGGL(lazy_begin_kernel, dim3(256), dim3(256), 0, as_stream(stream), col, val,
-                     nnz_dev, G, old_colsum, U_old, unsafe);
+  hipStream_t s = as_stream(stream);
+  Fx *old_fx = ws + 2 * N;
+  hipLaunchKernelGGL(lazy_begin_kernel, dim3(256), dim3(256), 0, s, col, val, nnz_dev, G, old_fx,
+                     U_old, unsafe);
   LAKDE_LAUNCHED();
-  return LAKDE_OK;
+  return colsum_finish(N, old_fx, old_colsum, s);
 }
 
 int lakde_rebalance_lazy_check(int64_t N, const float *c_prev, const float *c, const float *G,
@@ -608,20 +639,20 @@ int lakde_rebalance_lazy_check(int64_t N, const float *c_prev, const float *c, c
 }
 
 int lakde_rebalance_lazy(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
-                         const float *c_prev, float *c, float *G, double *U, double *U_old,
-                         float *old_colsum, double *ws, lakde_stream_t stream) {
-  return rebalance_lazy_guarded(N, col, val, nnz_dev, c_prev, c, G, U, U_old, old_colsum, ws,
+                         const float *c_prev, float *c, float *G, lakde_fx128 *U,
+                         lakde_fx128 *U_old, lakde_fx128 *ws, lakde_stream_t stream) {
+  return rebalance_lazy_guarded(N, col, val, nnz_dev, c_prev, c, G, U, U_old, ws,
                                 LazyGuard{nullptr, 0, nullptr}, as_stream(stream));
 }
 
 int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32_t *nnz,
-                         const int32_t *cols, float *vals, float *G, double *U,
+                         const int32_t *cols, float *vals, float *G, lakde_fx128 *U,
                          lakde_stream_t stream) {
   LAKDE_CHECK_ARG(N >= 1 && num_blocks >= 1 && num_blocks <= 65535 && cap >= 1, "bad size");
   LAKDE_CHECK_ARG(nnz && cols && vals && G && U, "null pointer");
   hipStream_t s = as_stream(stream);
   const unsigned gN = (unsigned)ceil_div(N, 256);
-  hipLaunchKernelGGL(zero_f64_kernel, dim3(gN), dim3(256), 0, s, U, N);
+  hipLaunchKernelGGL(zero_fx_kernel, dim3(gN), dim3(256), 0, s, U, N);
   LAKDE_LAUNCHED();
   int64_t gx = ceil_div(cap, 256 * 4);
   if (gx > 64) gx = 64;

@@ -35,7 +35,7 @@ struct EParams {
   // optional (compaction of a lazy-rebalance step): unless the step is
   // unsafe, store u = r / (G coeff) instead of r and accumulate the column
   // sums of r and u (the work of lazy_block_kernel)
-  double *lazy_Rnew, *lazy_Unew;
+  Fx *lazy_Rnew, *lazy_Unew;
   // optional (step driver): the row-pointer kernel also writes the entry
   // count and the status / unsafe flags into host-coherent memory as 64-bit
   // words (seq << 32 | value), each valid on its own when its tag is seq
@@ -59,19 +59,24 @@ struct EParams {
 
 constexpr int kSkipMaxD = 16;
 
-// the lazy rebalance's unsafe-step test for column m of an UPDATE step
-// (lakde_rebalance_lazy_check): G[m] clamp(exp(c_old - c)) outside
-// [2^-60, 2^60] or NaN; one atomic per wave with an unsafe column
 __device__ __forceinline__ float clamp_nan(float x, float lo, float hi) {
   return (x != x) ? x : fminf(fmaxf(x, lo), hi);
 }
 
+// the window of G clamp(exp(c_old - c)) the stored form represents: stored
+// values u = r / (G coeff) stay below 2^40 (r <= 1), inside the range of the
+// fixed-point column sums, and far from float under/overflow
+constexpr float kLazyGcLo = 0x1p-40f, kLazyGcHi = 0x1p40f;
+
+// the lazy rebalance's unsafe-step test for column m of an UPDATE step
+// (lakde_rebalance_lazy_check): G[m] clamp(exp(c_old - c)) outside the
+// window or NaN; one atomic per wave with an unsafe column
 __device__ __forceinline__ void lazy_unsafe_test(const EParams &p, bool valid, int64_t m,
                                                  float co, float cm) {
   bool bad = false;
   if (valid) {
     const float gc = p.lazy_G[m] * clamp_nan(expf(co - cm), FLT_MIN, FLT_MAX);
-    bad = !(gc >= 0x1p-60f && gc <= 0x1p60f);
+    bad = !(gc >= kLazyGcLo && gc <= kLazyGcHi);
   }
   if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(p.lazy_unsafe, 1);
 }
@@ -416,8 +421,8 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
         if (to_stored) {
           const float coeff = clamp_nan(expf(p.c_old[ms[i]] - cm), FLT_MIN, FLT_MAX);
           v = r / (p.lazy_G[ms[i]] * coeff);
-          atomicAdd(&p.lazy_Rnew[ms[i]], (double)r);
-          atomicAdd(&p.lazy_Unew[ms[i]], (double)v);
+          fx_atomic_add(&p.lazy_Rnew[ms[i]], r);
+          fx_atomic_add(&p.lazy_Unew[ms[i]], v);
         }
         if (pos >= 0 && pos < p.cap) {  // never write outside the arena slots
           p.out_row[pos] = (int32_t)n;
@@ -637,15 +642,15 @@ struct LazyGuard {
 };
 
 // shared host helpers (fused.hip)
-int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsafe, double *Rnew,
-                       double *Unew, hipStream_t s);
+int estep_compact_lazy(const lakde_estep_args &a, const float *G, int32_t *unsafe, Fx *Rnew,
+                       Fx *Unew, hipStream_t s);
 int estep_count(const lakde_estep_args &a, hipStream_t s,
                 unsigned long long *flag_words = nullptr, const int32_t *status_flag = nullptr,
                 const int32_t *unsafe_flag = nullptr, uint32_t flag_seq = 0,
                 int32_t *abort_flag = nullptr);
 int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
-                           const float *c_prev, float *c, float *G, double *U, double *U_old,
-                           float *old_colsum, double *ws, LazyGuard guard, hipStream_t s);
+                           const float *c_prev, float *c, float *G, Fx *U, Fx *U_old, Fx *ws,
+                           LazyGuard guard, hipStream_t s);
 void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit);
 int estep_fill_params(const lakde_estep_args &a, EParams *p);
 int pred_fill_params(const float *rec, int64_t N, int64_t M, float *out, void *workspace,

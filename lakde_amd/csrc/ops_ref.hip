@@ -157,8 +157,8 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
         wts[e] = wv;
         if (!GRAPH && old.colsum != nullptr && pbase == 0) {
           const int64_t ci = (int64_t)col[c0 + e];
-          atomicAdd(&old.colsum[ci], (float)wv);
-          atomicAdd(&old.stored[ci], (double)value[c0 + e]);
+          fx_atomic_add(&old.colsum[ci], (float)wv);
+          fx_atomic_add(&old.stored[ci], (float)value[c0 + e]);
         }
       }
       __syncthreads();
@@ -385,10 +385,10 @@ __global__ void __launch_bounds__(256) fill_kernel(float *p, int64_t n, float v)
 __global__ void __launch_bounds__(256) colsum_kernel(const int32_t *__restrict__ col,
                                                     const float *__restrict__ val,
                                                     const int32_t *__restrict__ nnz_dev,
-                                                    float *__restrict__ sums) {
+                                                    Fx *__restrict__ acc) {
   const int64_t nnz = *nnz_dev;
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  for (; i < nnz; i += (int64_t)gridDim.x * blockDim.x) atomicAdd(&sums[col[i]], val[i]);
+  for (; i < nnz; i += (int64_t)gridDim.x * blockDim.x) fx_atomic_add(&acc[col[i]], val[i]);
 }
 
 // posterior contributions of one block (int32 CSR) fused with the start of
@@ -396,14 +396,14 @@ __global__ void __launch_bounds__(256) colsum_kernel(const int32_t *__restrict__
 // column sums of the block's entries, and the unsafe flag cleared
 int posterior_contribs_lazy(const int32_t *rowptr, const int32_t *col, const float *value,
                             const float *colscale, const float *Xn, const float *Xm, int64_t nb,
-                            int64_t D, float nu0, float *S, float *nu, float *old_colsum,
-                            double *U_old, int32_t *unsafe, hipStream_t s) {
-  LAKDE_CHECK_ARG(rowptr && col && value && colscale && Xn && Xm && S && nu && old_colsum &&
+                            int64_t D, float nu0, float *S, float *nu, Fx *old_fx, Fx *U_old,
+                            int32_t *unsafe, hipStream_t s) {
+  LAKDE_CHECK_ARG(rowptr && col && value && colscale && Xn && Xm && S && nu && old_fx &&
                       U_old && unsafe && nb >= 1 && D >= 1 && D <= 512,
                   "bad args");
   return launch_outer<float, int32_t, int32_t, false>(rowptr, col, value, Xn, Xm, nb, D, 0, S, nu,
                                                       nu0, s, colscale,
-                                                      OldSums{old_colsum, U_old, unsafe});
+                                                      OldSums{old_fx, U_old, unsafe});
 }
 
 }  // namespace lakde
@@ -506,14 +506,13 @@ int lakde_fill_f32(float *p, int64_t n, float v, lakde_stream_t stream) {
 }
 
 int lakde_colsum(const int32_t *col, const float *val, const int32_t *nnz_dev, int64_t N,
-                 float *sums, lakde_stream_t stream) {
-  LAKDE_CHECK_ARG(col && val && nnz_dev && sums && N >= 0, "bad args");
-  int rc = lakde_fill_f32(sums, N, 0.f, stream);
-  if (rc) return rc;
-  hipLaunchKernelGGL(colsum_kernel, dim3(1024), dim3(256), 0, as_stream(stream), col, val, nnz_dev,
-                     sums);
+                 float *sums, lakde_fx128 *ws, lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(col && val && nnz_dev && sums && ws && N >= 0, "bad args");
+  if (N == 0) return LAKDE_OK;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(colsum_kernel, dim3(1024), dim3(256), 0, s, col, val, nnz_dev, ws);
   LAKDE_LAUNCHED();
-  return LAKDE_OK;
+  return colsum_finish(N, ws, sums, s);
 }
 
 }  // extern "C"

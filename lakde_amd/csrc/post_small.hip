@@ -32,8 +32,8 @@ struct SmallPost {
   int graph_dtype;
   int64_t K;
   float nu0;                    // nu = nu0 + sum of the row's responsibilities
-  float *old_colsum;            // optional lazy rebalance old column sums (true values)
-  double *U_old;                //   and stored values
+  Fx *old_colsum;               // optional lazy rebalance old column sums (true values)
+  Fx *U_old;                    //   and stored values (order-independent sums)
   int32_t *unsafe;              //   unsafe-step flag, cleared by the launch
   const int32_t *abort;         // optional: the launch does nothing when *abort != 0
 };
@@ -205,8 +205,8 @@ __global__ void __launch_bounds__(kPostWaves * kWave) post_small_kernel(SmallPos
     for (int i = 0; i < EB; ++i) {
       const float wt = p.colscale != nullptr ? u[i] * g[i] : u[i];
       if (c[i] >= 0 && p.old_colsum != nullptr) {
-        atomicAdd(&p.old_colsum[c[i]], wt);
-        atomicAdd(&p.U_old[c[i]], (double)u[i]);
+        fx_atomic_add(&p.old_colsum[c[i]], wt);
+        fx_atomic_add(&p.U_old[c[i]], u[i]);
       }
 #pragma unroll
       for (int ii = 0; ii < D; ++ii) {
@@ -330,7 +330,7 @@ static int launch_post_small(const SmallPost &p, hipStream_t s) {
 int block_posterior(const lakde_posterior_args &post, const float *X, int64_t N, int64_t row0,
                     const int32_t *rowptr, const int32_t *col, const float *val,
                     const float *colscale, const void *graph, int graph_dtype, int64_t K,
-                    float nu0, float *old_colsum, double *U_old, int32_t *unsafe, hipStream_t s,
+                    float nu0, Fx *old_colsum, Fx *U_old, int32_t *unsafe, hipStream_t s,
                     const int32_t *abort) {
   const int64_t D = post.D, nb = post.nb;
   LAKDE_CHECK_ARG(D >= 1 && D <= 63 && nb >= 0 && row0 >= 0 && row0 + nb <= N, "bad sizes");
@@ -340,6 +340,8 @@ int block_posterior(const lakde_posterior_args &post, const float *X, int64_t N,
   LAKDE_CHECK_ARG(graph == nullptr || graph_dtype == LAKDE_I32 || graph_dtype == LAKDE_I64,
                   "graph must be int32 or int64");
   LAKDE_CHECK_ARG((old_colsum == nullptr) == (U_old == nullptr), "old sums come in pairs");
+  LAKDE_CHECK_ARG(post.prior_mode != LAKDE_PRIOR_KNN_CACHED || D > kSmallMaxD,
+                  "the cached kNN prior needs D >= 7");
   if (nb == 0) return LAKDE_OK;
   if (D <= kSmallMaxD) {
     LAKDE_CHECK_ARG(post.prior_mode != LAKDE_PRIOR_KNN ||

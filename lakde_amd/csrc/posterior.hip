@@ -105,6 +105,48 @@ __device__ float stabilized_chol_wave(const float *M, float *L, int D, int ld, f
   return j;
 }
 
+// packed lower triangles of the kNN prior cache: entry n holds prior_j =
+// prior + j0 I, then its Cholesky factor, row-major lower (i, k <= i)
+__host__ __device__ constexpr int64_t prior_cache_floats(int D) {
+  return (int64_t)D * (D + 1);
+}
+
+// The kNN prior's factor of one block with its persistent jitter, written to
+// the cache (lakde/local_knn_kde.py:195-207, the first call; the prior and
+// its jitter do not change during a fit, so every later call of the
+// reference returns the same factor)
+__global__ void __launch_bounds__(64) prior_cache_kernel(int D, const float *__restrict__ prior,
+                                                        float *__restrict__ prior_jitter,
+                                                        float *__restrict__ logdet_prior,
+                                                        float *__restrict__ cache,
+                                                        int32_t *__restrict__ status) {
+  extern __shared__ float smem[];
+  const int ld = post_ld(D);
+  const int64_t n = blockIdx.x;
+  const int lane = threadIdx.x;
+  float *R0 = smem, *R2 = smem + D * ld;
+  const int64_t DD = (int64_t)D * D;
+  load_batched(prior + n * DD, D * D, lane,
+               [&](int t, float v) { R0[(t / D) * ld + (t % D)] = v; });
+  __syncthreads();
+  bool ok;
+  const float j0 = stabilized_chol_wave(R0, R2, D, ld, prior_jitter[n], &ok);
+  if (!ok && lane == 0) *status = 1;
+  if (lane == 0) prior_jitter[n] = j0;
+  double ld0 = lane < D ? log((double)R2[lane * ld + lane]) : 0.0;
+  ld0 = wave_sum(ld0);
+  if (lane == 0) logdet_prior[n] = (float)(2.0 * ld0);
+  const int P = D * (D + 1) / 2;
+  float *c = cache + n * prior_cache_floats(D);
+  for (int t = lane; t < P; t += kWave) {
+    int i, k;
+    tri_index(t, &i, &k);
+    const float v = R0[i * ld + k];
+    c[t] = i == k ? v + j0 : v;
+    c[P + t] = R2[i * ld + k];
+  }
+}
+
 __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   extern __shared__ float smem[];
   const int D = (int)a.D;
@@ -120,8 +162,19 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   const int64_t DD = (int64_t)D * D;
 
   // ---- prior
-  const float *P = (a.prior_mode == LAKDE_PRIOR_KNN) ? a.prior + n * DD : a.prior;
-  load_batched(P, D * D, lane, [&](int t, float v) { R0[(t / D) * ld + (t % D)] = v; });
+  const bool cached = a.prior_mode == LAKDE_PRIOR_KNN_CACHED;
+  const int Pt = D * (D + 1) / 2;
+  if (cached) {
+    // prior_j, packed lower (the factorisation reads the lower triangle)
+    load_batched(a.prior + n * prior_cache_floats(D), Pt, lane, [&](int t, float v) {
+      int i, k;
+      tri_index(t, &i, &k);
+      R0[i * ld + k] = v;
+    });
+  } else {
+    const float *P = (a.prior_mode == LAKDE_PRIOR_KNN) ? a.prior + n * DD : a.prior;
+    load_batched(P, D * D, lane, [&](int t, float v) { R0[(t / D) * ld + (t % D)] = v; });
+  }
   __syncthreads();
   if (a.prior_mode == LAKDE_PRIOR_KNN) {
     bool ok;
@@ -196,10 +249,18 @@ __global__ void __launch_bounds__(64) posterior_kernel(lakde_posterior_args a) {
   // factors) and the longest lane runs D(D+1)/2 steps instead of D^2.  The
   // shared prior's T is the dense prior covariance (all i, all k <= j).
   float *Tm = R3;
-  const bool tri = a.prior_mode == LAKDE_PRIOR_KNN;
-  const float *Tsrc = tri ? a.prior_tril + n * DD : a.prior;
+  const bool tri = a.prior_mode != LAKDE_PRIOR_SHARED;
   __syncthreads();
-  load_batched(Tsrc, D * D, lane, [&](int t, float v) { Tm[(t / D) * ld + (t % D)] = v; });
+  if (cached) {
+    load_batched(a.prior + n * prior_cache_floats(D) + Pt, Pt, lane, [&](int t, float v) {
+      int i, k;
+      tri_index(t, &i, &k);
+      Tm[i * ld + k] = v;
+    });
+  } else {
+    const float *Tsrc = tri ? a.prior_tril + n * DD : a.prior;
+    load_batched(Tsrc, D * D, lane, [&](int t, float v) { Tm[(t / D) * ld + (t % D)] = v; });
+  }
   __syncthreads();
   float tr = 0.f;
   if (lane < D) {
@@ -245,13 +306,36 @@ extern "C" int lakde_posterior(const lakde_posterior_args *args, lakde_stream_t 
   LAKDE_CHECK_ARG(a.prior && a.S && a.nu && a.a && a.b && a.W_jitter && a.Xn && a.Linv &&
                       a.logdet_W && a.trace && a.rec && a.status,
                   "null pointer");
-  LAKDE_CHECK_ARG(a.prior_mode == LAKDE_PRIOR_KNN || a.prior_mode == LAKDE_PRIOR_SHARED,
+  LAKDE_CHECK_ARG(a.prior_mode == LAKDE_PRIOR_KNN || a.prior_mode == LAKDE_PRIOR_SHARED ||
+                      a.prior_mode == LAKDE_PRIOR_KNN_CACHED,
                   "bad prior mode");
   if (a.prior_mode == LAKDE_PRIOR_KNN)
     LAKDE_CHECK_ARG(a.prior_jitter && a.prior_tril && a.logdet_prior, "kNN prior outputs missing");
   if (a.nb == 0) return LAKDE_OK;
   size_t lds = (size_t)4 * a.D * post_ld((int)a.D) * sizeof(float);
   hipLaunchKernelGGL(posterior_kernel, dim3((unsigned)a.nb), dim3(64), lds, as_stream(stream), a);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+extern "C" int64_t lakde_prior_cache_floats(int64_t D) {
+  return (D >= 1 && D <= kPostMaxD) ? prior_cache_floats((int)D) : -1;
+}
+
+extern "C" int lakde_knn_prior_cache(const float *X, int64_t N, int64_t D, int64_t row0,
+                                     int64_t nb, const void *graph, int graph_dtype, int64_t K,
+                                     float *prior_scratch, float *prior_jitter,
+                                     float *logdet_prior, float *cache, int32_t *status,
+                                     lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 1 && D <= kPostMaxD, "D must be in [1, 63]");
+  LAKDE_CHECK_ARG(X && graph && prior_scratch && prior_jitter && logdet_prior && cache && status,
+                  "null pointer");
+  if (nb == 0) return LAKDE_OK;
+  int rc = lakde_knn_prior_cov(X, N, D, row0, nb, graph, graph_dtype, K, prior_scratch, stream);
+  if (rc) return rc;
+  size_t lds = (size_t)2 * D * post_ld((int)D) * sizeof(float);
+  hipLaunchKernelGGL(prior_cache_kernel, dim3((unsigned)nb), dim3(64), lds, as_stream(stream),
+                     (int)D, prior_scratch, prior_jitter, logdet_prior, cache, status);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }

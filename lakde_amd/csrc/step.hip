@@ -99,8 +99,7 @@ int step_rebalance(const lakde_step_args *a, hipStream_t s) {
   if (!unsafe) {
     // the compaction stored u and accumulated the block's sums: normalise
     return rebalance_lazy_guarded(N, nullptr, nullptr, nullptr, e.c_old, e.c, a->G, a->U,
-                                  a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws,
-                                  LazyGuard{nullptr, 0, nullptr}, s);
+                                  a->U_old, a->lazy_ws, LazyGuard{nullptr, 0, nullptr}, s);
   }
   // the stored form cannot represent this step: fold, rebalance the true
   // values exactly, recompute the totals, clear the per-step sums
@@ -113,8 +112,7 @@ int step_rebalance(const lakde_step_args *a, hipStream_t s) {
   rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all, a->G,
                             a->U, s);
   if (rc) return rc;
-  LAKDE_HIP(hipMemsetAsync(a->U_old, 0, (size_t)N * sizeof(double), s));
-  LAKDE_HIP(hipMemsetAsync(const_cast<float *>(e.old_colsum), 0, (size_t)N * sizeof(float), s));
+  LAKDE_HIP(hipMemsetAsync(a->U_old, 0, (size_t)N * sizeof(lakde_fx128), s));
   return LAKDE_OK;
 }
 
@@ -154,15 +152,22 @@ int step_begin(const lakde_step_args *a, hipStream_t s, const int32_t *abort) {
     if (rc) return rc;
   }
   const bool lazy = a->G != nullptr;
+  // the block's old column sums are order-independent fixed-point sums
+  // (lazy: accumulated by the contributions kernel into the third part of
+  // lazy_ws), finished into the floats the E-step reads
+  Fx *old_fx = lazy ? a->lazy_ws + 2 * N : nullptr;
   rc = block_posterior(p, e.X, N, e.row0, e.rowptr, e.out_col, e.out_val, a->G, a->knn_graph,
-                       a->knn_graph_dtype, a->knn_k, a->nu0,
-                       lazy ? const_cast<float *>(e.old_colsum) : nullptr,
-                       lazy ? a->U_old : nullptr, lazy ? unsafe : nullptr, s, abort);
+                       a->knn_graph_dtype, a->knn_k, a->nu0, old_fx, lazy ? a->U_old : nullptr,
+                       lazy ? unsafe : nullptr, s, abort);
   if (rc) return rc;
-  if (a->G == nullptr) {
-    rc = lakde_colsum(e.out_col, e.out_val, e.nnz_dev, N, const_cast<float *>(e.old_colsum), s);
-    if (rc) return rc;
+  if (lazy) {
+    rc = colsum_finish(N, old_fx, const_cast<float *>(e.old_colsum), s);
+  } else {
+    Fx *acc = (Fx *)((char *)a->rebal_ws + ((size_t)N * 4 + 15) / 16 * 16);
+    rc = lakde_colsum(e.out_col, e.out_val, e.nnz_dev, N, const_cast<float *>(e.old_colsum), acc,
+                      s);
   }
+  if (rc) return rc;
   if (a->bf16_frag != nullptr) {
     rc = a->screen_xb != nullptr
              ? lakde_pack_screen(e.rec, nb, D, a->screen_origin, a->bf16_frag, s)
@@ -227,8 +232,7 @@ int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
     rc = estep_compact_lazy(e, a->G, unsafe, a->lazy_ws, a->lazy_ws + N, s);
     if (rc) return rc;
     rc = rebalance_lazy_guarded(N, nullptr, nullptr, nullptr, e.c_old, e.c, a->G, a->U,
-                                a->U_old, const_cast<float *>(e.old_colsum), a->lazy_ws,
-                                LazyGuard{e.nnz_dev, a->cap, unsafe}, s);
+                                a->U_old, a->lazy_ws, LazyGuard{e.nnz_dev, a->cap, unsafe}, s);
     if (rc) return rc;
   } else {
     rc = lakde_estep_compact(&e, s);

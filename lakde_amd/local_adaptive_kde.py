@@ -126,7 +126,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
             self._lazy = F.LazyRebalance(N, dev)
         else:
             self._colsum = torch.empty(N, device=dev)
-            self._rebal_ws = torch.empty(2 * N, device=dev)
+            self._rebal_ws = F.rebalance_workspace(N, dev)
         self._host_flags = torch.zeros(self.arena.num_blocks + 2, dtype=torch.int32,
                                        pin_memory=True)
         self._flags = self._host_flags.numpy()
@@ -207,10 +207,14 @@ class LocallyAdaptiveKDE(AbstractKDE):
         """Posterior outputs of a partial step: (nu, logdet_W, trace, logdet_prior)."""
         raise NotImplementedError
 
+    def _ensure_prior_cache(self, X):
+        """Fill a per-fit prior cache before the first posterior (kNN model)."""
+
     def _block_posterior(self, X, b, nu_out, logdet_out, trace_out, logdet_prior_out,
                          update_b):
         """Posterior of block b (lakde/local_adaptive_kde.py:66-96): writes the
         records rec[s:e]; optionally the tau update of b[s:e]."""
+        self._ensure_prior_cache(X)
         N, D = X.shape
         s, e, nb = self._span(b, N)
         arena = self.arena
@@ -325,6 +329,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
         with one host synchronisation per step.  ``count_steps``: add the
         completed steps to iter_steps with the rest of the bookkeeping."""
         self._count_steps = count_steps
+        self._ensure_prior_cache(X)
         lazy = self._lazy
         t = float(sparsity_threshold or 0.0)
         bf16 = self.precision == "bf16"

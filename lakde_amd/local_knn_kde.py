@@ -7,7 +7,7 @@ import numpy as np
 import torch
 
 from . import _fused as F
-from ._lib import PRIOR_KNN
+from ._lib import PRIOR_KNN, PRIOR_KNN_CACHED
 from .local_adaptive_kde import LocallyAdaptiveKDE
 from .utils import compute_knn_graph
 
@@ -28,6 +28,8 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
                          estep_skip)
         self.k_or_knn_graph = k_or_knn_graph
         self.k = None
+        self._prior_cache = None
+        self._prior_cached = False
         self.knn_graph = None
         self.logdet_sigma_0 = None
         self.sigma_0_lambda_mm_trace = None
@@ -57,12 +59,52 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
         self.logdet_W = X.new_empty(N)
         self.sigma_0_lambda_mm_trace = X.new_empty(N)
         self._alloc(X)
+        self._init_prior_cache(X)
+
+    # lakde/local_knn_kde.py:195-207 recomputes the kNN prior covariance and
+    # its jittered factor at every partial step and prediction.  X, the
+    # graph and (once the first factorisation has set it) the persistent
+    # jitter do not change during a fit, so every such call returns the same
+    # factor: for D >= 7 the prior + jitter and its factor are computed once
+    # per fit into a device cache (D (D + 1) floats per point, 16 GB at
+    # N = 10^6, D = 63), which the posterior kernel reads.  (D <= 6: the
+    # fused small-dimension posterior computes the prior in place.)
+    prior_cache = True
+    PRIOR_CACHE_MAX_BYTES = 64 << 30
+    _prior_cache = None
+    _prior_cached = False
+
+    def _init_prior_cache(self, X):
+        N, D = X.shape
+        self._prior_cache = None
+        self._prior_cached = False
+        if self.prior_cache and D >= 7:
+            floats = F.prior_cache_floats(D)
+            if N * floats * 4 <= self.PRIOR_CACHE_MAX_BYTES:
+                self._prior_cache = torch.empty(N, floats, device=X.device)
+
+    def _ensure_prior_cache(self, X):
+        if self._prior_cache is None or self._prior_cached:
+            return
+        N = X.size(0)
+        for b in self._blocks(N):
+            s, e, nb = self._span(b, N)
+            F.knn_prior_cache(X, s, self.knn_graph[s:e], self._prior_buf[:nb],
+                              self.sigma_0_jitter[s:e], self.logdet_sigma_0[s:e],
+                              self._prior_cache[s:e], self.arena.status_ptr)
+        self._sync(None)  # a prior that does not factor raises, as in the reference
+        self._prior_cached = True
+        self._step_cache = {}
 
     # lakde/local_knn_kde.py:195-207 (prior + its jittered Cholesky happen in
     # the posterior kernel)
     def _prior_spec(self, X, b, nb):
         N = X.size(0)
         s, e, _ = self._span(b, N)
+        if self._prior_cache is not None:
+            if not self._prior_cached:
+                raise RuntimeError("the kNN prior cache is not filled")
+            return PRIOR_KNN_CACHED, self._prior_cache[s:e], None, None, None
         return (PRIOR_KNN, self._prior_buf[:nb], self.sigma_0_jitter[s:e], self._prior_tril[:nb],
                 self.knn_graph[s:e])
 
@@ -81,6 +123,7 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
         same values), so that the screening E-step can start its passes from
         a lower bound of every normaliser."""
         N = X.size(0)
+        self._ensure_prior_cache(X)
         for b in self._blocks(N):
             s, e, nb = self._span(b, N)
             self._block_posterior(X, b, self.nu[s:e], self.logdet_W[s:e],
@@ -125,3 +168,4 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
             setattr(self, k, state_dict[k].to(X.device).clone())
         self._load_blocks(list(state_dict["rnm_blocks"]), X)
         self._finish_load(X)
+        self._init_prior_cache(X)

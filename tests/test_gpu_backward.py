@@ -201,7 +201,7 @@ def _hier_fixed_inputs(m, Xd, keep):
             m.W_inv_jitter.double()[keep])
 
 
-def _hier_J64(P, Y, X64, fixed, nu_shared, chunk=1024):
+def _hier_J64(P, Y, X64, fixed, nu_shared, chunk=1024, per_point=False):
     """J = sum_m log p(y_m) of the hierarchical model as a float64 function of
     the shared prior factor P and the query points Y: the chain of
     lakde/local_hierarchical_kde.py:269-271 and lakde/local_adaptive_kde.py:83-96,
@@ -225,7 +225,8 @@ def _hier_J64(P, Y, X64, fixed, nu_shared, chunk=1024):
         lp = coeff[:, None] - h[:, None] * torch.log1p((z * z).sum(1))
         lse = torch.logsumexp(lp, 0)
         acc = lse if acc is None else torch.logaddexp(acc, lse)
-    return (acc - math.log(N)).sum()
+    out = acc - math.log(N)
+    return out if per_point else out.sum()
 
 
 @pytest.mark.parametrize("N,M,k,iterations", [(2000, 500, 128, 1), (30000, 1000, 256, 0)])
@@ -250,7 +251,13 @@ def test_c4_hierarchical_gradients_latent_d43(N, M, k, iterations):
       the error of the directional derivative divided by |grad| |v| (the
       largest directional derivative; random directions in these M x 43-
       and 946-dimensional spaces are nearly orthogonal to the gradient, so
-      their own relative error is reported, not bounded).
+      their own relative error is reported, not bounded), at most 1e-5 (Y)
+      and 1e-4 (P) plus the difference quotient's own float64 rounding
+      floor, 4 u sum_m (|J+_m| + |J-_m|) / (2h) / (|grad| |v|) (u = 2^-53;
+      the per-point log densities are differenced before they are summed).
+      Where the prior factor barely moves J (after a sweep the data terms
+      dominate the posteriors) that floor is what the check can resolve,
+      and it is printed.
 
     A posterior whose float64 chain would not factor makes
     hierarchical_factors raise (none does here)."""
@@ -305,30 +312,36 @@ def test_c4_hierarchical_gradients_latent_d43(N, M, k, iterations):
         Pd = P.detach()
         Yb = Yd0.double()
         for what, base, g, g64 in (("P", Pd, gP, g64P), ("Y", Yb, gY, g64Y)):
-            errs, rels = [], []
+            errs, rels, floors, excess = [], [], [], []
             gn = float(g64.norm())
+            bar = 1e-5 if what == "Y" else 1e-4
             for _ in range(200):
                 v = torch.tensor(rng.standard_normal(tuple(base.shape)), device=DEV,
                                  dtype=torch.float64)
                 if what == "P":
                     v = torch.tril(v)
                     v *= float(Pd.norm()) / float(v.norm())
-                    Jp = _hier_J64(Pd + h * v, Yb, X64, fixed, m.nu_shared)
-                    Jm = _hier_J64(Pd - h * v, Yb, X64, fixed, m.nu_shared)
+                    Jp = _hier_J64(Pd + h * v, Yb, X64, fixed, m.nu_shared, per_point=True)
+                    Jm = _hier_J64(Pd - h * v, Yb, X64, fixed, m.nu_shared, per_point=True)
                 else:
                     v /= float(v.norm())
-                    Jp = _hier_J64(Pd, Yb + h * v, X64, fixed, m.nu_shared)
-                    Jm = _hier_J64(Pd, Yb - h * v, X64, fixed, m.nu_shared)
-                fd = float(Jp - Jm) / (2 * h)
+                    Jp = _hier_J64(Pd, Yb + h * v, X64, fixed, m.nu_shared, per_point=True)
+                    Jm = _hier_J64(Pd, Yb - h * v, X64, fixed, m.nu_shared, per_point=True)
+                fd = float((Jp - Jm).sum()) / (2 * h)
+                scale = gn * float(v.norm())
+                floor = 4 * 2.0 ** -53 * float((Jp.abs() + Jm.abs()).sum()) / (2 * h) / scale
                 an = float((g * v).sum())
-                errs.append(abs(an - fd) / (gn * float(v.norm())))
+                err = abs(an - fd) / scale
+                errs.append(err)
+                floors.append(floor)
+                excess.append(err - floor)
                 rels.append(abs(an - fd) / max(abs(fd), 1e-300))
-            res[what] = (max(errs), float(np.median(rels)), max(rels))
+            res[what] = (max(errs), float(np.median(rels)), max(rels), max(floors), max(excess))
             print(f"C4: d/d{what} vs central differences (200 directions): error / (|grad| |v|) "
-                  f"max {res[what][0]:.2e}; relative median {res[what][1]:.2e} "
+                  f"max {res[what][0]:.2e} (rounding floor of the difference quotient up to "
+                  f"{res[what][3]:.2e}); relative median {res[what][1]:.2e} "
                   f"max {res[what][2]:.2e}", flush=True)
-    assert res["Y"][0] <= 1e-5
-    assert res["P"][0] <= 1e-4
+            assert res[what][4] <= bar, (what, res[what])
 
 
 def test_knn_prior_scale_gradient():
