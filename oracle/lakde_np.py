@@ -319,6 +319,13 @@ def calc_log_rho_block(X_test, X_train, nu, W_triu, logdet_W, dt):
 
 # ------------------------------------------------------------ VI core (L3)
 
+def _as_np(x):
+    """A torch tensor (any device) or array-like as a numpy array."""
+    if hasattr(x, "detach"):
+        x = x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
 class _LocallyAdaptiveKDE:
     """lakde/local_adaptive_kde.py:47-223 (plus the fit driver of
     lakde/kdes.py:34-87)."""
@@ -446,6 +453,33 @@ class _LocallyAdaptiveKDE:
                 acc[j:j + bs] = logaddexp(acc[j:j + bs], logsumexp0(lp.astype(dt)))
         return (dt(-math.log(N)) + acc).astype(dt)
 
+    # lakde/kdes.py:115-125, lakde/local_adaptive_kde.py:272-309: the state
+    # under the reference's keys (numpy arrays; rnm_blocks a list of
+    # (indices (2, nnz) int32, values)).  load_state_dict takes the same
+    # mapping with torch tensors or arrays, as torch.load returns the
+    # reference's saved state.
+    _STATE_POINT = ("nu", "a", "b", "rnm_log_consts", "logdet_W", "W_inv_jitter")
+
+    def state_dict(self):
+        d = {"block_size": self.block_size, "verbose": False, "iter_steps": self.iter_steps,
+             "nu_0": self.nu_0, "a_0": self.a_0, "b_0": self.b_0,
+             "rnm_blocks": [(ix.copy(), v.copy()) for ix, v in self.rnm_blocks],
+             "rnm_active_contribs": self.dt(sum(v.size for _, v in self.rnm_blocks))}
+        for k in self._STATE_POINT:
+            d[k] = getattr(self, k).copy()
+        return d
+
+    def load_state_dict(self, d):
+        dt = self.dt
+        self.block_size = int(d["block_size"])
+        self.iter_steps = int(d["iter_steps"])
+        for k in ("nu_0", "a_0", "b_0"):
+            setattr(self, k, dt(_as_np(d[k])))
+        for k in self._STATE_POINT:
+            setattr(self, k, np.array(_as_np(d[k]), dt))
+        self.rnm_blocks = [(np.array(_as_np(ix), np.int32), np.array(_as_np(v), dt))
+                           for ix, v in d["rnm_blocks"]]
+
     # lakde/kdes.py:34-87
     def fit(self, X, iterations=200, sparsity_threshold=None, callbacks=()):
         X = np.ascontiguousarray(X, self.dt)
@@ -527,6 +561,22 @@ class LocalKNearestKDE(_LocallyAdaptiveKDE):
         self._init_rnm(X, self.knn_graph, sparsity_threshold)
         self.logdet_sigma_0 = np.empty(N, self.dt)
         self.sigma_0_lambda_mm_trace = np.empty(N, self.dt)
+
+    # lakde/local_knn_kde.py:343-367
+    def state_dict(self):
+        d = super().state_dict()
+        d.update({"knn_graph": self.knn_graph.copy(), "k": self.k,
+                  "logdet_sigma_0": self.logdet_sigma_0.copy(),
+                  "sigma_0_lambda_mm_trace": self.sigma_0_lambda_mm_trace.copy(),
+                  "sigma_0_jitter": self.sigma_0_jitter.copy()})
+        return d
+
+    def load_state_dict(self, d):
+        super().load_state_dict(d)
+        self.knn_graph = np.array(_as_np(d["knn_graph"]), np.int32)
+        self.k = int(d["k"])
+        for k in ("logdet_sigma_0", "sigma_0_lambda_mm_trace", "sigma_0_jitter"):
+            setattr(self, k, np.array(_as_np(d[k]), self.dt))
 
     # lakde/local_knn_kde.py:195-207
     def calc_prior_cov(self, X, block_idx):
@@ -616,6 +666,28 @@ class LocalHierarchicalKDE(_LocallyAdaptiveKDE):
         self.W_shared_triu = np.float64(self.nu_0) * sigma_0_tril.T / self.nu_shared
         num_blocks = (N - 1) // self.block_size + 1
         self.expect_lambda_contribs = np.empty((num_blocks, D, D), np.float64)
+
+    # lakde/local_hierarchical_kde.py:463-490 (the shared-prior state keeps
+    # the dtypes it has: float64 in the reference)
+    _STATE_SHARED = ("nu_shared", "lambda_0", "lambda_0_triu", "W_shared_triu",
+                     "expect_lambda_contribs")
+
+    def state_dict(self):
+        d = super().state_dict()
+        d.update({"k": self.k, "nu_shared_0": self.nu_shared_0,
+                  "W_shared_inv_jitter": self.W_shared_inv_jitter})
+        for k in self._STATE_SHARED:
+            d[k] = np.array(getattr(self, k))
+        return d
+
+    def load_state_dict(self, d):
+        super().load_state_dict(d)
+        self.k = int(d["k"])
+        self.nu_shared_0 = self.dt(_as_np(d["nu_shared_0"]))
+        self.W_shared_inv_jitter = _as_np(d["W_shared_inv_jitter"])[()]
+        for k in self._STATE_SHARED:
+            setattr(self, k, np.array(_as_np(d[k]), np.float64))
+        self.nu_shared = np.float64(self.nu_shared)
 
     # lakde/local_hierarchical_kde.py:269-271
     def calc_prior_cov(self, *_):

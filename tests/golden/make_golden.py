@@ -317,6 +317,44 @@ def case_hier_nojit():
     _fit_case("hier_normal4_nojit", lakde.LocalHierarchicalKDE, X, Y, g, 512, 0.0, 4)
 
 
+def _state_case(name, model_cls, X, Y, graph, block_size, threshold, before, after):
+    """A reference model fitted `before` sweeps and written as its own save()
+    writes it (torch.save of state_dict(), lakde/kdes.py:115-135), then
+    fitted `after` more sweeps: the state file (name.pt) and the
+    continuation's outputs (name.npz).  LocalHierarchicalKDE keeps
+    rnm_blocks in a numpy object array (lakde/local_hierarchical_kde.py:175),
+    which only an unpickling loader can read back; the fixture stores the
+    same (indices, values) tuples as a list, so it loads with
+    torch.load(weights_only=True)."""
+    Xt, Yt = torch.from_numpy(X), torch.from_numpy(Y)
+    m = model_cls(k_or_knn_graph=torch.from_numpy(graph), block_size=block_size, verbose=False,
+                  logs=False)
+    m.fit(Xt, iterations=before, callbacks=[], sparsity_threshold=threshold)
+    d = m.state_dict()
+    if isinstance(d["rnm_blocks"], np.ndarray):
+        d["rnm_blocks"] = list(d["rnm_blocks"])
+    path = os.path.join(HERE, name + ".pt")
+    torch.save(d, path)
+    print("wrote", path, "%.1f KB" % (os.path.getsize(path) / 1024))
+    m.fit(Xt, iterations=after, callbacks=[], sparsity_threshold=threshold)
+    extra = {}
+    if hasattr(m, "W_shared_triu"):
+        extra["W_shared_triu"] = _np(m.W_shared_triu)
+    _save(name, X=X, Y=Y, graph=graph, block_size=block_size, threshold=threshold,
+          before=before, after=after, log_pred=_np(m.log_pred_density(Xt, Yt)),
+          log_consts=_np(m.rnm_log_consts), nu=_np(m.nu), b=_np(m.b),
+          W_inv_jitter=_np(m.W_inv_jitter), **extra)
+
+
+def case_state_interop():
+    X, Y = _mix6_small()
+    g = _ref_graph(X, 64, False)
+    _state_case("state_knn_mix6", lakde.LocalKNearestKDE, X, Y, g, 512, 1 / 64, 2, 2)
+    X, Y = synthetic.make("circle8", 1000, 200)
+    g = _ref_graph(X, 20, True)
+    _state_case("state_hier_circle8", lakde.LocalHierarchicalKDE, X, Y, g, 512, 1 / 20, 2, 2)
+
+
 def case_knn_circle_full():
     w = synthetic.WORKLOADS["circle8"]
     X, Y = synthetic.make("circle8")
@@ -338,7 +376,7 @@ if __name__ == "__main__":
         sys.exit(0)
     cases = [case_ops, case_knn_circle_small, case_knn_mix6_small, case_knn_degenerate_small,
              case_hier_latent_small, case_hier_mix6_small, case_hier_smooth_small,
-             case_hier_nojit]
+             case_hier_nojit, case_state_interop]
     if args.full:
         cases.append(case_knn_circle_full)
     for c in cases:

@@ -264,6 +264,57 @@ def test_prior_cache_matches_recomputed_prior(precision):
         np.testing.assert_array_equal(a, b, err_msg=f"state item {k}")
 
 
+@pytest.mark.parametrize("name,model", [("state_knn_mix6", "knn"), ("state_hier_circle8", "hier")])
+def test_reference_state_interop(golden, name, model):
+    """Checkpoints between the reference and lakde_amd
+    (lakde/kdes.py:115-135, local_adaptive_kde.py:272-309,
+    local_knn_kde.py:343-367, local_hierarchical_kde.py:463-490).
+    Forward: a state the reference wrote (its state_dict() after 2 sweeps,
+    saved with torch.save and read here with torch.load(weights_only=True))
+    loads into the lakde_amd model, and 2 more sweeps on the GPU reproduce
+    the reference's continuation (held-out mean within 1e-4 relative,
+    normalisers, nu, b within 1e-4, equal jitters, shared prior factor
+    within 1e-5 of its largest entry).  Reverse: the lakde_amd state_dict,
+    through torch.save and the same safe loader, has the reference's keys
+    and loads into the oracle (the reference's algorithm on the CPU), whose
+    next sweep matches the GPU model's next sweep at the same bars."""
+    import io
+
+    from lakde_amd import LocalHierarchicalKDE, LocalKNearestKDE
+    from test_oracle_golden import _check_continuation, load_reference_state
+    d = golden(name)
+    sd = load_reference_state(name)
+    X, Y = _t(d["X"]), _t(d["Y"])
+    t = float(d["threshold"])
+    cls, ocls = ((LocalKNearestKDE, O.LocalKNearestKDE) if model == "knn"
+                 else (LocalHierarchicalKDE, O.LocalHierarchicalKDE))
+    m = cls(block_size=int(d["block_size"]))
+    m.load_state_dict(sd, X)
+    m.fit(X, iterations=int(d["after"]), sparsity_threshold=t)
+    lp = m.log_pred_density(X, Y).cpu().numpy()
+    _check_continuation(lp, m.rnm_log_consts.cpu().numpy(), m.nu.cpu().numpy(),
+                        m.b.cpu().numpy(), m.W_inv_jitter.cpu().numpy(), d, f"{name} forward",
+                        m.W_shared_triu.cpu().numpy() if model == "hier" else None)
+    # reverse
+    buf = io.BytesIO()
+    torch.save(m.state_dict(), buf)
+    buf.seek(0)
+    ours = torch.load(buf, weights_only=True, map_location="cpu")
+    assert sorted(ours) == sorted(sd)
+    mo = ocls(block_size=int(d["block_size"]))
+    mo.load_state_dict(ours)
+    mo.fit(d["X"], iterations=1, sparsity_threshold=t)
+    m.fit(X, iterations=1, sparsity_threshold=t)
+    ref = {"log_pred": mo.log_pred_density(d["X"], d["Y"]), "log_consts": mo.rnm_log_consts,
+           "nu": mo.nu, "b": mo.b, "W_inv_jitter": mo.W_inv_jitter}
+    if model == "hier":
+        ref["W_shared_triu"] = mo.W_shared_triu
+    _check_continuation(m.log_pred_density(X, Y).cpu().numpy(), m.rnm_log_consts.cpu().numpy(),
+                        m.nu.cpu().numpy(), m.b.cpu().numpy(), m.W_inv_jitter.cpu().numpy(), ref,
+                        f"{name} reverse", m.W_shared_triu.cpu().numpy() if model == "hier"
+                        else None)
+
+
 def test_knn_model_matches_oracle_trajectory():
     """Per-sweep log normalisers against the float32 oracle on a fresh
     seeded sample (not one of the reference fixtures)."""

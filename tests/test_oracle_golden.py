@@ -270,3 +270,50 @@ def test_hierarchical_fit_matches_reference(golden, name):
     lp = m.log_pred_density(X, d["Y"])
     rel = abs(lp.mean() - d["log_pred"].mean()) / abs(d["log_pred"].mean())
     assert rel <= bar, rel
+
+
+# reference-written states (tests/golden/make_golden.py case_state_interop:
+# the reference's state_dict() after `before` sweeps, saved with torch.save)
+# and the reference's continuation `after` more sweeps
+STATE_CASES = {"state_knn_mix6": "knn", "state_hier_circle8": "hier"}
+
+
+def load_reference_state(name):
+    import os
+
+    import torch
+    from conftest import GOLDEN
+    return torch.load(os.path.join(GOLDEN, name + ".pt"), weights_only=True)
+
+
+def _check_continuation(lp, lc, nu, b, jit, d, tag, wst=None):
+    assert abs(lp.mean() - d["log_pred"].mean()) <= 1e-4 * abs(d["log_pred"].mean()), tag
+    np.testing.assert_allclose(lc, d["log_consts"], rtol=1e-4, atol=1e-4, err_msg=tag)
+    np.testing.assert_allclose(nu, d["nu"], rtol=1e-4, err_msg=tag)
+    np.testing.assert_allclose(b, d["b"], rtol=1e-4, err_msg=tag)
+    np.testing.assert_array_equal(jit, d["W_inv_jitter"], err_msg=tag)
+    if wst is not None:
+        np.testing.assert_allclose(wst, d["W_shared_triu"], rtol=0,
+                                   atol=1e-5 * np.abs(d["W_shared_triu"]).max(), err_msg=tag)
+
+
+@pytest.mark.parametrize("name", sorted(STATE_CASES))
+def test_oracle_continues_reference_state(golden, name):
+    """A state the reference wrote (its state_dict keys and layout,
+    lakde/local_adaptive_kde.py:272-309, local_knn_kde.py:343-367,
+    local_hierarchical_kde.py:463-490) loads into the oracle, and `after`
+    more sweeps reproduce the reference's continuation: held-out mean within
+    1e-4 relative, normalisers, nu and b within 1e-4, equal jitters, the
+    shared prior factor within 1e-5 of its largest entry.  The oracle's own
+    state_dict has the reference's keys."""
+    d = golden(name)
+    sd = load_reference_state(name)
+    cls = O.LocalKNearestKDE if STATE_CASES[name] == "knn" else O.LocalHierarchicalKDE
+    m = cls(block_size=int(d["block_size"]))
+    m.load_state_dict(sd)
+    assert m.iter_steps == int(sd["iter_steps"]) > 0
+    m.fit(d["X"], iterations=int(d["after"]), sparsity_threshold=float(d["threshold"]))
+    lp = m.log_pred_density(d["X"], d["Y"])
+    _check_continuation(lp, m.rnm_log_consts, m.nu, m.b, m.W_inv_jitter, d, name,
+                        m.W_shared_triu if STATE_CASES[name] == "hier" else None)
+    assert sorted(m.state_dict()) == sorted(sd)
