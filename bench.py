@@ -10,11 +10,13 @@ path beside it.
 
 A step is one VI iteration: one sweep of partial steps over all 1954 kernel
 blocks (lakde/kdes.py:64-67), with the data resident in HBM.  The K timed
-iterations run in one fit() call of the reference API.  A whole sweep of this
-workload takes seconds, so the number of iterations actually timed (and of
-warm-up iterations) is the largest that keeps the run within --budget-s of
-wall time, at most K (and W); the line reports the numbers run, with the
-requested ones beside them.  Prints one JSON line.
+iterations run through fit() of the reference API, one call per iteration,
+inside one timed region.  A whole sweep of this workload takes seconds, so
+the number of iterations actually timed (and of warm-up iterations) is the
+largest that keeps the run within --budget-s of wall time, at most K (and W);
+the line reports the numbers run, with the requested ones beside them.
+Beside the headline: one whole FP32 iteration of the same workload, the
+mix32_d6 FP32 path and the mix64_d21 precision form.  Prints one JSON line.
 """
 
 import argparse
@@ -58,7 +60,8 @@ def _parse():
                     help="wall-time budget of the whole run, which caps the iterations run")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-secondary", action="store_true", help="skip the mix32_d6 FP32 block")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the mix32_d6 FP32 block and the mix64_d21 precision block")
     ap.add_argument("--no-tuning", action="store_true", help="skip the prior-scale gradient leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="finish with a plain device copy of this many MiB (a known-size "
@@ -368,6 +371,61 @@ def secondary(steps=20, warmup=3):
                                      "the pairs its box bound excludes"}}
 
 
+def precision_block(name="mix64_d21", sweeps=2):
+    """The precision form against the plain FP32 path on the d=21 workload
+    (mix64_d21: N=300,000, D=21, k=128, threshold 1/128): BF16 and FP32 VI
+    iterations/s, each model from its own initialisation (the precision
+    form's), one warm-up iteration, then `sweeps` timed iterations through
+    fit(); |BF16 - FP32| of the held-out log density after the warm-up
+    iteration; both predictive kernels against float64 arithmetic on the
+    same records (10,000 test points)."""
+    import torch
+
+    from lakde_amd import LocalKNearestKDE, synthetic
+    from lakde_amd.utils import compute_knn_graph
+    w = synthetic.WORKLOADS[name]
+    Xh, Yh = synthetic.make(name, n_test=10000)
+    N, D = Xh.shape
+    X = torch.from_numpy(Xh).cuda()
+    Ys = torch.from_numpy(Yh).cuda()
+    t = w["threshold"]
+    graph = compute_knn_graph(X, w["k"], 1024, include_self=False)
+    res = {"workload": f"{name}: LocalKNearestKDE fit, N={N}, D={D}, k={w['k']}, "
+                       f"threshold=1/128, block_size=512", "steps": sweeps, "warmup": 1}
+    lps = {}
+    for prec in ("bf16", "fp32"):
+        m = LocalKNearestKDE(k_or_knn_graph=graph, block_size=512, precision="bf16")
+        t0 = time.perf_counter()
+        m.init_parameters_(X, t)
+        m.init_summaries_(X, t)
+        torch.cuda.synchronize()
+        t_init = time.perf_counter() - t0
+        m.precision = prec
+        _sweep(m, X, t)
+        lps[prec] = m.log_pred_density(X, Ys)
+        lp64 = pred_fp64(m.rec, D, Ys)
+        e = (lps[prec].double() - lp64).abs()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(sweeps):
+            m.fit(X, iterations=1, sparsity_threshold=t)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[prec] = {"vi_iters_per_s": round(sweeps / el, 4), "init_s": round(t_init, 2),
+                     "pred_err_vs_fp64_nats": {"mean": float(e.mean()), "max": float(e.max())}}
+        log(f"{name} {prec}: {el / sweeps:.3f} s per VI iteration, predictive vs float64 "
+            f"{float(e.mean()):.2e} nats mean")
+        del m
+        torch.cuda.empty_cache()
+    d = (lps["bf16"] - lps["fp32"]).abs()
+    res["bf16_vs_fp32_after_one_iteration_nats"] = {"mean": float(d.mean()),
+                                                    "max": float(d.max()),
+                                                    "points": int(Ys.size(0))}
+    res["speedup_bf16_over_fp32"] = round(res["bf16"]["vi_iters_per_s"]
+                                          / res["fp32"]["vi_iters_per_s"], 3)
+    return res
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -479,8 +537,9 @@ def main():
 
     # the iterations that fit the budget (a reserve for the legs after the
     # timed region: prediction, tuning, CPU baseline, secondary block)
-    reserve = 25.0 + (0 if args.no_tuning else 25.0) + (0 if args.no_cpu_baseline else 45.0) \
-        + (0 if args.no_secondary else 15.0)
+    # (about twice the measured durations of the legs after the timed region)
+    reserve = 15.0 + (0 if args.no_tuning else 40.0) + (0 if args.no_cpu_baseline else 20.0) \
+        + (0 if args.no_secondary else 30.0)
     # a second warm-up iteration through fit() (the timed path; the first
     # went through the per-step API) gives the iteration time the budget uses
     warm_done, t_it = 1, t_sweep
@@ -578,13 +637,18 @@ def main():
     del model
     torch.cuda.empty_cache()
 
-    sec = None
+    sec = c3 = None
     if not args.no_secondary:
         try:
             sec = secondary()
         except Exception as e:
             log(f"secondary block failed: {e!r}")
             sec = {"error": repr(e)}
+        try:
+            c3 = precision_block()
+        except Exception as e:
+            log(f"mix64_d21 precision block failed: {e!r}")
+            c3 = {"error": repr(e)}
 
     value = steps / elapsed
     out = {
@@ -620,6 +684,7 @@ def main():
         "cpu_baseline": cpu,
         "prior_scale_tuning": tuning,
         "secondary": sec,
+        "precision_d21": c3,
         "wall_s": round(time.perf_counter() - t_start, 1),
     }
     print(json.dumps(out), flush=True)

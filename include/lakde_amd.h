@@ -400,6 +400,10 @@ typedef struct {
   const void *screen_xb;
   const float *screen_xnorm;
   const float *screen_origin;
+  /* 1: bf16_frag holds lakde_m32_frag_bytes(D) per kernel and the step runs
+   * the FP32 matrix-core E-step (lakde_pack_m32 + lakde_estep_main_m32,
+   * with screen_xnorm and screen_origin; screen_xb may be NULL) */
+  int fp32_mfma;
 } lakde_step_args;
 
 /* A 256-byte host-coherent, device-mapped buffer (hipHostMalloc,
@@ -514,6 +518,22 @@ int lakde_screen_knn_bound(const float *rec, const float *X, int64_t N, int64_t 
                            lakde_stream_t stream);
 int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, const void *xb,
                             const float *xnorm, const float *lbound, lakde_stream_t stream);
+
+/* The plain FP32 E-step on the matrix cores (17 <= D <= 63): every pair's
+ * quadratic form y = Linv_n (x_m - x_n) on v_mfma_f32_32x32x2_f32 (FP32
+ * products and accumulation, the difference x_m - x_n formed in FP32), its
+ * log rho in the log-sum-exp and the normaliser update; the keep test runs
+ * on the exact values of the compaction for the pairs an error bound cannot
+ * rule out.  Outputs as lakde_estep_main (follow with lakde_estep_finish).
+ *   lakde_pack_m32: per kernel of a block (lakde_m32_frag_bytes(D) bytes
+ *     each, 16-byte aligned) from its records and the origin o of
+ *     lakde_screen_points (xnorm there: |x - o| per point).
+ * (lakde/local_adaptive_kde.py:36-44,98-153 as lakde_estep_main.) */
+int64_t lakde_m32_frag_bytes(int64_t D);
+int lakde_pack_m32(const float *rec, int64_t nb, int64_t D, const float *origin, void *frag,
+                   lakde_stream_t stream);
+int lakde_estep_main_m32(const lakde_estep_args *args, const void *frag, const float *xnorm,
+                         lakde_stream_t stream);
 
 /* lakde_pred_density with the quadratic forms on the matrix cores
  * (frag: N fragment blocks; Y: M x D float32 test points). */

@@ -200,12 +200,13 @@ def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=
 
 
 def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
-          arena=None, b=None, compact_only=False, bf16=None, order=None, screen=None):
+          arena=None, b=None, compact_only=False, bf16=None, order=None, screen=None, m32=None):
     """One E-step launch sequence.  ``bf16=frag`` (the block's
     lakde_pack_frag_bf16 blocks) runs the main (quadratic form / LSE /
     ballot) kernel on the matrix cores in bf16x3; ``screen=(frag, points)``
     (the block's lakde_pack_screen blocks and the fit's ScreenPoints) runs
-    the screening E-step."""
+    the screening E-step; ``m32=(frag, points)`` (lakde_pack_m32 blocks)
+    the plain FP32 E-step on the matrix cores."""
     args = estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old, old_colsum, arena, b,
                       order)
     lib = _lib.load()
@@ -213,7 +214,10 @@ def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
     if compact_only:
         check(lib.lakde_estep_compact(args, st), "estep_compact")
         return
-    if screen is not None:
+    if m32 is not None:
+        frag, pts = m32
+        check(lib.lakde_estep_main_m32(args, ptr(frag), ptr(pts.xnorm), st), "estep_main_m32")
+    elif screen is not None:
         frag, pts = screen
         lb = pts.lbound if mode == ESTEP_ACCUM else None
         check(lib.lakde_estep_main_screen(args, ptr(frag), ptr(pts.xb), ptr(pts.xnorm), ptr(lb),
@@ -254,6 +258,17 @@ class ScreenPoints:
                                                  ptr(self.lbound), stream_of(X)),
               "screen_knn_bound")
         return self.lbound
+
+
+def m32_frag_bytes(D):
+    return int(_lib.load().lakde_m32_frag_bytes(D))
+
+
+def pack_m32(rec, D, origin, out):
+    nb = rec.size(0)
+    check(_lib.load().lakde_pack_m32(ptr(rec), nb, D, ptr(origin), ptr(out), stream_of(rec)),
+          "pack_m32")
+    return out
 
 
 def pack_screen(rec, D, origin, out):

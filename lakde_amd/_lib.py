@@ -56,7 +56,7 @@ class StepArgs(ctypes.Structure):
         ("G", _vp), ("U", _vp), ("U_old", _vp), ("lazy_ws", _vp), ("rebal_ws", _vp),
         ("fold_first", _int), ("bf16_frag", _vp), ("host_flags", _vp), ("ev_main", _vp * 2),
         ("ev_flags", _vp), ("flag_words", _vp), ("abort_flag", _vp),
-        ("screen_xb", _vp), ("screen_xnorm", _vp), ("screen_origin", _vp),
+        ("screen_xb", _vp), ("screen_xnorm", _vp), ("screen_origin", _vp), ("fp32_mfma", _int),
     ]
 
 
@@ -105,6 +105,9 @@ _SIGNATURES = {
     "lakde_bf16_frag_bytes": (_i64, [_i64]),
     "lakde_pack_frag_bf16": (_int, [_vp, _i64, _i64, _vp, _vp]),
     "lakde_estep_main_bf16": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp]),
+    "lakde_m32_frag_bytes": (_i64, [_i64]),
+    "lakde_pack_m32": (_int, [_vp, _i64, _i64, _vp, _vp, _vp]),
+    "lakde_estep_main_m32": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp, _vp]),
     "lakde_pred_density_bf16": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
     "lakde_screen_frag_bytes": (_i64, [_i64]),
     "lakde_screen_points_bytes": (_sz, [_i64, _i64]),

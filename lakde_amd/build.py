@@ -42,7 +42,8 @@ D_GROUPS = [(1, 6), (7, 12), (13, 18), (19, 24), (25, 32), (33, 40), (41, 48), (
 def _gen_inst(lo, hi):
     path = os.path.join(BUILD, f"fused_d{lo}_{hi}.hip")
     lines = ['#include "%s"' % os.path.join(CSRC, "fused_impl.hpp"),
-             '#include "%s"' % os.path.join(CSRC, "screen.hpp"), "namespace lakde {"]
+             '#include "%s"' % os.path.join(CSRC, "screen.hpp"),
+             '#include "%s"' % os.path.join(CSRC, "mfma32.hpp"), "namespace lakde {"]
     for d in range(lo, hi + 1):
         lines += [
             f"template int launch_estep_main<{d}>(const EParams &, int, hipStream_t);",
@@ -58,6 +59,10 @@ def _gen_inst(lo, hi):
             "__bf16 *, float *, hipStream_t);",
             f"template int launch_screen_knn_bound<{d}>(const float *, const float *, int64_t, "
             "const void *, int, int64_t, int, float *, hipStream_t);",
+            f"template int launch_estep_m32<{d}>(const EParams &, const float *, const float *, "
+            "int, hipStream_t);",
+            f"template int launch_pack_m32<{d}>(const float *, int64_t, const float *, float *, "
+            "hipStream_t);",
         ]
     lines.append("}")
     text = "\n".join(lines) + "\n"

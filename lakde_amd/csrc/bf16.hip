@@ -17,11 +17,13 @@
 // C holds 16 floats per lane, column = lane % 32, row = 8 * (r / 4) +
 // 4 * (lane / 32) + r % 4 for register r.
 #include "fused_impl.hpp"
+#include "mfma32.hpp"
 #include "screen.hpp"
 
 namespace lakde {
 
 LAKDE_FOR_EACH_D(LAKDE_DECL_SCREEN_D)
+LAKDE_FOR_EACH_D(LAKDE_DECL_M32_D)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -492,6 +494,48 @@ int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, cons
 #undef CASE
   }
   if (rc) set_error("lakde_estep_main_screen: launch failed");
+  return rc;
+}
+
+// ------------------------------------------- FP32 matrix-core E-step (mfma32.hpp)
+int64_t lakde_m32_frag_bytes(int64_t D) {
+  return (D >= 17 && D <= 63) ? m32_frag_floats((int)D) * 4 : -1;
+}
+
+int lakde_pack_m32(const float *rec, int64_t nb, int64_t D, const float *origin, void *frag,
+                   lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 17 && D <= 63, "the FP32 matrix-core E-step needs 17 <= D <= 63");
+  LAKDE_CHECK_ARG(rec && origin && frag && nb >= 0, "bad args");
+  LAKDE_CHECK_ARG(((uintptr_t)frag & 15) == 0, "frag must be 16-byte aligned");
+  hipStream_t s = as_stream(stream);
+  int rc = LAKDE_EINVAL;
+  switch (D) {
+#define CASE(d) case d: rc = launch_pack_m32<d>(rec, nb, origin, (float *)frag, s); break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) set_error("lakde_pack_m32: launch failed");
+  return rc;
+}
+
+int lakde_estep_main_m32(const lakde_estep_args *args, const void *frag, const float *xnorm,
+                         lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr && frag && xnorm, "null args");
+  LAKDE_CHECK_ARG(args->D >= 17 && args->D <= 63, "the FP32 matrix-core E-step needs 17 <= D <= 63");
+  LAKDE_CHECK_ARG(args->nb <= 32767, "the FP32 matrix-core E-step takes blocks of at most 32767 kernels");
+  LAKDE_CHECK_ARG(args->col_perm == nullptr, "the FP32 matrix-core E-step takes columns in index order");
+  LAKDE_CHECK_ARG(((uintptr_t)frag & 15) == 0, "frag must be 16-byte aligned");
+  EParams p;
+  int rc = estep_fill_params(*args, &p);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  rc = LAKDE_EINVAL;
+  switch (args->D) {
+#define CASE(d) case d: rc = launch_estep_m32<d>(p, (const float *)frag, xnorm, args->mode, s); break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) set_error("lakde_estep_main_m32: launch failed");
   return rc;
 }
 

@@ -169,9 +169,10 @@ int step_begin(const lakde_step_args *a, hipStream_t s, const int32_t *abort) {
   }
   if (rc) return rc;
   if (a->bf16_frag != nullptr) {
-    rc = a->screen_xb != nullptr
-             ? lakde_pack_screen(e.rec, nb, D, a->screen_origin, a->bf16_frag, s)
-             : lakde_pack_frag_bf16(e.rec, nb, D, a->bf16_frag, s);
+    rc = a->fp32_mfma            ? lakde_pack_m32(e.rec, nb, D, a->screen_origin, a->bf16_frag, s)
+         : a->screen_xb != nullptr ? lakde_pack_screen(e.rec, nb, D, a->screen_origin,
+                                                       a->bf16_frag, s)
+                                   : lakde_pack_frag_bf16(e.rec, nb, D, a->bf16_frag, s);
     if (rc) return rc;
   }
   return LAKDE_OK;
@@ -191,7 +192,9 @@ int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
     em.lazy_unsafe = unsafe;
   }
   if (a->ev_main[0]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[0], s));
-  if (a->bf16_frag != nullptr && a->screen_xb != nullptr)
+  if (a->bf16_frag != nullptr && a->fp32_mfma)
+    rc = lakde_estep_main_m32(&em, a->bf16_frag, a->screen_xnorm, s);
+  else if (a->bf16_frag != nullptr && a->screen_xb != nullptr)
     rc = lakde_estep_main_screen(&em, a->bf16_frag, a->screen_xb, a->screen_xnorm, nullptr, s);
   else if (a->bf16_frag != nullptr)
     rc = lakde_estep_main_bf16(&em, a->bf16_frag, s);

@@ -43,7 +43,12 @@ class LocallyAdaptiveKDE(AbstractKDE):
     BF16 matrix cores with a rigorous error bound and recomputes exactly in
     FP32 the pairs that can matter to a normaliser or a kept responsibility;
     the predictive density runs bf16x3 split precision on the matrix cores).
+    The FP32 E-step runs on the FP32 matrix cores for 17 <= D <= 63
+    (``fp32_mfma``, on by default), on the vector ALUs otherwise.
     """
+
+    fp32_mfma = True
+    _screen = None
 
     def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32",
                  rebalance="lazy", estep_skip=True):
@@ -140,13 +145,17 @@ class LocallyAdaptiveKDE(AbstractKDE):
         if self.estep_skip and self.precision == "fp32" and D <= 16:
             self._col_order = column_order(X)
         self._screen = None
-        if self.precision == "bf16":
-            if not 8 <= D <= 63:
-                raise ValueError("the bf16 precision form needs 8 <= D <= 63")
-            # screening E-step: per-fit points, per-block fragments
+        if self.precision == "bf16" and not 8 <= D <= 63:
+            raise ValueError("the bf16 precision form needs 8 <= D <= 63")
+        if 8 <= D <= 63:
+            # the matrix-core E-steps (the screening form; FP32 for D >= 17):
+            # per-fit points, per-block fragments (either form may run: the
+            # precision can change between steps)
             self._screen = F.ScreenPoints(X)
             self._fb = F.screen_frag_bytes(D)
-            self._frag_blk = torch.empty(bs * self._fb, dtype=torch.uint8, device=dev)
+            self._m32_fb = F.m32_frag_bytes(D) if D >= 17 else 0
+            self._frag_blk = torch.empty(bs * max(self._fb, self._m32_fb), dtype=torch.uint8,
+                                         device=dev)
 
     def _init_arena(self, X, graph, sparsity_threshold):
         """Initial responsibilities 1/init_k on the kNN graph, laid out per
@@ -239,7 +248,8 @@ class LocallyAdaptiveKDE(AbstractKDE):
         s, e, nb = self._span(b, N)
         c = self.rnm_log_consts
         F.estep(X, self.rec[s:e], s, mode, threshold, c, self._ews, arena=self.arena, b=b,
-                screen=self._screen_operands(b, N), order=self._col_order)
+                screen=self._screen_operands(b, N), m32=self._m32_operands(b, N),
+                order=self._col_order)
         nnz = self._sync(b)
         if nnz > self.arena.cap:
             self.arena.grow(nnz)
@@ -260,13 +270,27 @@ class LocallyAdaptiveKDE(AbstractKDE):
         F.pack_screen(self.rec[s:e], self.rec_D, self._screen.origin, frag)
         return frag, self._screen
 
+    def _uses_m32(self):
+        return (self.precision == "fp32" and self.fp32_mfma and self._screen is not None
+                and self.rec_D >= 17 and self._col_order is None)
+
+    def _m32_operands(self, b, N):
+        """(fragments of block b, the fit's points) for the FP32 matrix-core
+        E-step, or None."""
+        if not self._uses_m32():
+            return None
+        s, e, nb = self._span(b, N)
+        frag = self._frag_blk[:nb * self._m32_fb]
+        F.pack_m32(self.rec[s:e], self.rec_D, self._screen.origin, frag)
+        return frag, self._screen
+
     # ------------------------------------------------------------- E-step
     def _init_lse_pass(self, X, b):
         """lakde/local_knn_kde.py:157-166: c = logaddexp(c, LSE over block b)."""
         N = X.size(0)
         s, e, nb = self._span(b, N)
         F.estep(X, self.rec[s:e], s, ESTEP_ACCUM, 0.0, self.rnm_log_consts, self._ews,
-                screen=self._screen_operands(b, N))
+                screen=self._screen_operands(b, N), m32=self._m32_operands(b, N))
 
     def _init_fixed_pass(self, X, b, sparsity_threshold):
         """partial_expectation_step(update_normalization=False)."""
@@ -333,14 +357,16 @@ class LocallyAdaptiveKDE(AbstractKDE):
         lazy = self._lazy
         t = float(sparsity_threshold or 0.0)
         bf16 = self.precision == "bf16"
-        frag = self._frag_blk.data_ptr() if bf16 else None
-        sc = self._screen if bf16 else None
+        m32 = self._uses_m32()
+        frag = self._frag_blk.data_ptr() if (bf16 or m32) else None
+        sc = self._screen if (bf16 or m32) else None
         for b in range(first, first + count):
             a = self._step_args(X, b)
             a.estep.threshold = t
             a.post.prior = self._prior_ptr(b, a.post.prior)
             a.bf16_frag = frag
-            a.screen_xb = sc.xb.data_ptr() if sc is not None else None
+            a.screen_xb = sc.xb.data_ptr() if bf16 else None
+            a.fp32_mfma = 1 if m32 else 0
             a.screen_xnorm = sc.xnorm.data_ptr() if sc is not None else None
             a.screen_origin = sc.origin.data_ptr() if sc is not None else None
             ev = self._estep_timer.pair() if self._estep_timer is not None else None
@@ -379,13 +405,13 @@ class LocallyAdaptiveKDE(AbstractKDE):
                 old = self._step_cache["arr"][b]
                 keep = (old.estep.threshold, old.post.prior, old.bf16_frag, old.ev_main[0],
                         old.ev_main[1], old.fold_first)
-                keep_sc = (old.screen_xb, old.screen_xnorm, old.screen_origin)
+                keep_sc = (old.screen_xb, old.screen_xnorm, old.screen_origin, old.fp32_mfma)
                 arena.grow(int(self._flags[b]))
                 self._step_cache = {}
                 for j in range(b, end):
                     a = self._step_args(X, j)
                     a.estep.threshold, a.post.prior, a.bf16_frag = keep[0], keep[1], keep[2]
-                    a.screen_xb, a.screen_xnorm, a.screen_origin = keep_sc
+                    a.screen_xb, a.screen_xnorm, a.screen_origin, a.fp32_mfma = keep_sc
                     if j == b:
                         a.ev_main[0], a.ev_main[1], a.fold_first = keep[3], keep[4], keep[5]
                 a = self._step_cache["arr"][b]

@@ -129,11 +129,11 @@ class LocalKNearestKDE(LocallyAdaptiveKDE):
             self._block_posterior(X, b, self.nu[s:e], self.logdet_W[s:e],
                                   self.sigma_0_lambda_mm_trace[s:e], self.logdet_sigma_0[s:e],
                                   update_b=False)
-        if self._screen is not None:
+        if self.precision == "bf16":
             self._screen.knn_bound(self.rec, X, self.knn_graph)
         for b in self._blocks(N):
             self._init_lse_pass(X, b)
-        if self._screen is not None:
+        if self.precision == "bf16":
             self._screen.lbound = None
         self._active = 0.0
         for b in self._blocks(N):
