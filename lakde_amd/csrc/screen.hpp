@@ -58,10 +58,31 @@ constexpr int kKeepQ = 4;      // keep-candidate queue entries per lane
 
 __host__ __device__ constexpr int sc_kp(int D) { return (D + 15) / 16 * 16; }
 __host__ __device__ constexpr int sc_dp(int D) { return (D + 31) / 32 * 32; }
-// fragment block per kernel: U hi [Dp x Kp] bf16, -c_n [Dp] f32, scal[4]:
-// ka F_n, kb F_n b_n, E-step const, E-step exponent (-nu/2)
+// MFMA A tiles of U (32 rows x 16 columns) that are not all zero: K-step s
+// of row tile rt holds nonzeros iff 16 s <= 32 rt + 31 (U is lower
+// triangular); stored K-step-major in that order
+__host__ __device__ constexpr bool sc_tile_used(int rt, int s) { return s * 16 <= rt * 32 + 31; }
+__host__ __device__ constexpr int sc_ntiles(int D) {
+  int t = 0;
+  for (int s = 0; s < sc_kp(D) / 16; ++s)
+    for (int rt = 0; rt < sc_dp(D) / 32; ++rt) t += sc_tile_used(rt, s) ? 1 : 0;
+  return t;
+}
+__host__ __device__ constexpr int sc_tile_index(int D, int rt_, int s_) {
+  int t = 0;
+  for (int s = 0; s < sc_kp(D) / 16; ++s)
+    for (int rt = 0; rt < sc_dp(D) / 32; ++rt) {
+      if (s == s_ && rt == rt_) return t;
+      t += sc_tile_used(rt, s) ? 1 : 0;
+    }
+  return -1;
+}
+// fragment block per kernel, laid out as its LDS image: the used A tiles of
+// bf16(U) (1 KB each: lane l's 8 bf16 operand values at byte 16 l, row
+// 32 rt + l % 32, columns 16 s + 8 (l / 32) + [0, 8)), then -c_n [Dp] f32,
+// then scal[4]: ka F_n, kb F_n b_n, E-step const, E-step exponent (-nu/2)
 __host__ __device__ constexpr int64_t sc_frag_bytes(int D) {
-  return (int64_t)sc_dp(D) * sc_kp(D) * 2 + (int64_t)sc_dp(D) * 4 + 16;
+  return (int64_t)sc_ntiles(D) * 1024 + (int64_t)sc_dp(D) * 4 + 16;
 }
 
 // Parameters of the screening launches beyond EParams.
@@ -107,14 +128,21 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
   const int lane = threadIdx.x;
   const float *r = rec + n * R;
   uint8_t *f = frag + n * sc_frag_bytes(D);
+  constexpr int NT = sc_ntiles(D);
   __bf16 *uh = reinterpret_cast<__bf16 *>(f);
-  float *cneg = reinterpret_cast<float *>(uh + Dp * Kp);
+  float *cneg = reinterpret_cast<float *>(f + NT * 1024);
   float *scal = cneg + Dp;
-  for (int t = lane; t < Dp * Kp; t += 64) {
-    const int i = t / Kp, j = t - i * Kp;
-    const float v = (i < D && j <= i) ? r[i * (i + 1) / 2 + j] : 0.f;
-    uh[t] = (__bf16)v;
-  }
+  for (int s = 0; s < Kp / 16; ++s)
+    for (int rt = 0; rt < Dp / 32; ++rt) {
+      if (!sc_tile_used(rt, s)) continue;
+      __bf16 *tile = uh + sc_tile_index(D, rt, s) * 512 + lane * 8;
+      const int i = rt * 32 + (lane & 31);
+      for (int e = 0; e < 8; ++e) {
+        const int j = s * 16 + 8 * (lane >> 5) + e;
+        const float v = (i < D && j <= i) ? r[i * (i + 1) / 2 + j] : 0.f;
+        tile[e] = (__bf16)v;
+      }
+    }
   // c_n row by row (lane = row), |U|_F^2 and |x_n - o|^2 in double
   double f2 = 0.0, b2 = 0.0;
   float c = 0.f;
@@ -169,58 +197,40 @@ __global__ void __launch_bounds__(256) screen_knn_bound_kernel(
 }
 
 // ---------------------------------------------------------------- kernel
-// LDS image of one kernel's fragment block: U hi rows padded to KpL bf16 (16
-// extra bytes per row, so that the 32 rows a K-step reads start in
-// different banks), then -c_n and scal.  A stage holds kScreenKps kernels.
-constexpr int kScreenKps = 2;
+// LDS stages of the kernel loop: kScreenKps consecutive fragment blocks
+// (contiguous in global memory, each already in its LDS layout), two stages
+// in a ring.  A stage is copied with direct global-to-LDS loads (16 bytes
+// per lane, 1 KB per wave instruction, no registers): the copy of stage
+// i + 1 is issued at the start of iteration i and has the whole iteration
+// to land; the barrier at the end of the iteration waits for it
+// (vmcnt(0)) and frees stage i for the copy after next.
+constexpr int kScreenKps = 4;
 
 template <int D>
 struct ScreenLds {
-  static constexpr int Kp = sc_kp(D), Dp = sc_dp(D), KpL = Kp + 8;
-  static constexpr int kUBytes = Dp * KpL * 2;
-  static constexpr int kOne = kUBytes + Dp * 4 + 16;               // one kernel in LDS
-  static constexpr int kBytes = kScreenKps * kOne;
-  static constexpr int kGranU = Dp * Kp / 8;                       // 16-byte granules of U
-  static constexpr int kGranOne = (int)(sc_frag_bytes(D) / 16);    // ... of a fragment block
-  static constexpr int kGran = kScreenKps * kGranOne;
-  static constexpr int kPer = (kGran + 255) / 256;                 // granules per thread
+  static constexpr int64_t kOne = sc_frag_bytes(D);  // one kernel, 16-byte multiple
+  static constexpr int kBytes = (int)(kScreenKps * kOne);
+  static constexpr int kInst = (kBytes + 1023) / 1024;  // wave instructions per full stage
 };
 
-// Cooperative copy of kScreenKps consecutive fragment blocks (contiguous in
-// global memory) into one LDS stage: every load issued into registers
-// before the current stage's work, the stores after it.
+typedef __attribute__((address_space(3))) void *sc_lds_ptr;
+
+// issue the copy of `kernels` consecutive fragment blocks from src into the
+// LDS stage dst (wave instruction j by wave j % 4; lane l moves the 16 bytes
+// at 1024 j + 16 l, lanes past the end are inactive)
 template <int D>
-struct ScreenStage {
+__device__ __forceinline__ void screen_stage_copy(const uint8_t *__restrict__ src, uint8_t *dst,
+                                                  int kernels, int w, int lane) {
   using L = ScreenLds<D>;
-  sc_u32x4 g[L::kPer];
-  __device__ __forceinline__ void load(const uint8_t *__restrict__ src, int kernels) {
-    const sc_u32x4 *s = reinterpret_cast<const sc_u32x4 *>(src);
-    const int lim = kernels * L::kGranOne;
+  const int bytes = (int)(kernels * L::kOne);
 #pragma unroll
-    for (int i = 0; i < L::kPer; ++i) {
-      const int q = (int)threadIdx.x + 256 * i;
-      if (q < lim) g[i] = s[q];
-    }
+  for (int i = 0; i < (L::kInst + 3) / 4; ++i) {
+    const int j = w + 4 * i;  // wave-uniform
+    const int off = j * 1024 + lane * 16;
+    if (j * 1024 < bytes && off < bytes)
+      __builtin_amdgcn_global_load_lds(src + off, (sc_lds_ptr)(dst + j * 1024), 16, 0, 0);
   }
-  __device__ __forceinline__ void store(uint8_t *dst, int kernels) const {
-    const int lim = kernels * L::kGranOne;
-#pragma unroll
-    for (int i = 0; i < L::kPer; ++i) {
-      const int q = (int)threadIdx.x + 256 * i;
-      if (q < lim) {
-        const int k = q / L::kGranOne, qq = q - k * L::kGranOne;
-        int off = k * L::kOne;
-        if (qq < L::kGranU) {
-          const int row = qq / (L::Kp / 8), cg = qq - row * (L::Kp / 8);
-          off += (row * L::KpL + cg * 8) * 2;
-        } else {
-          off += L::kUBytes + (qq - L::kGranU) * 16;
-        }
-        *reinterpret_cast<sc_u32x4 *>(dst + off) = g[i];
-      }
-    }
-  }
-};
+}
 
 // per-wave flat pair list of the exact phase: entry j holds (kernel << 8 |
 // lane) until lane j % 64 has read it, then the pair's exact log rho
@@ -245,13 +255,20 @@ template <int D, int MODE>
 __global__ void __launch_bounds__(256, 2)
 estep_screen_kernel(EParams p, SParams sp) {
   using L = ScreenLds<D>;
-  constexpr int Kp = L::Kp, KpL = L::KpL;
+  constexpr int Kp = sc_kp(D);
   constexpr int RT = sc_dp(D) / 32, KS = Kp / 16;
   constexpr int64_t R = rec_floats(D);
   constexpr int64_t FB = sc_frag_bytes(D);
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2][L::kBytes];
+  constexpr int kRing = 2 * L::kBytes > L::kBytes + (int)sizeof(ScreenWaveList[4])
+                            ? 2 * L::kBytes
+                            : L::kBytes + (int)sizeof(ScreenWaveList[4]);
+  // the two stages of the kernel loop (two arrays, so that the compiler can
+  // tell the copy into one from the reads of the other); the exact phase
+  // after the loop reuses the memory for the waves' pair lists
+  __shared__ __attribute__((aligned(1024))) uint8_t ring0[kRing - L::kBytes];
+  __shared__ __attribute__((aligned(1024))) uint8_t ring1[L::kBytes];
   __shared__ int16_t sq[kScreenQ][256];
-  __shared__ ScreenWaveList wl[4];
+  ScreenWaveList *wl = reinterpret_cast<ScreenWaveList *>(ring0);
   __shared__ int16_t kq_n[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
   __shared__ float kq_x[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
 
@@ -299,8 +316,7 @@ estep_screen_kernel(EParams p, SParams sp) {
   // bounds [lo, hi] of the exact log rho of kernel n (LDS image fb) at this
   // lane's column; -inf for the excluded diagonal and invalid columns
   auto bounds = [&](const uint8_t *fb, int64_t n, float &hi, float &lo) {
-    const __bf16 *U = reinterpret_cast<const __bf16 *>(fb);
-    const float *cneg = reinterpret_cast<const float *>(fb + L::kUBytes);
+    const float *cneg = reinterpret_cast<const float *>(fb + sc_ntiles(D) * 1024);
     const float *scal = cneg + sc_dp(D);
     sc_f32x16 acc[RT][2];
 #pragma unroll
@@ -322,9 +338,9 @@ estep_screen_kernel(EParams p, SParams sp) {
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
-        if (s * 16 > rt * 32 + 31) continue;  // U is lower triangular
+        if (!sc_tile_used(rt, s)) continue;  // U is lower triangular
         const sc_bf16x8 a = *reinterpret_cast<const sc_bf16x8 *>(
-            U + (rt * 32 + (lane & 31)) * KpL + s * 16 + 8 * (lane >> 5));
+            fb + sc_tile_index(D, rt, s) * 1024 + lane * 16);
         acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[0][s], acc[rt][0], 0, 0, 0);
         acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[1][s], acc[rt][1], 0, 0, 0);
       }
@@ -337,11 +353,16 @@ estep_screen_kernel(EParams p, SParams sp) {
         q0 = fmaf(acc[rt][0][r], acc[rt][0][r], q0);
         q1 = fmaf(acc[rt][1][r], acc[rt][1][r], q1);
       }
-    const bool upper = lane >= 32;
-    const float recv = __shfl_xor(upper ? q0 : q1, 32, kWave);
-    const float qt = (upper ? q1 : q0) + recv;  // Q~ of this lane's column
+    // lanes 0-31 hold rows of column tile 0's columns, lanes 32-63 the other
+    // rows: after the swap lane l < 32 has both halves of tile 0's column l
+    // and lane l >= 32 both halves of tile 1's column l - 32 (column l of
+    // the wave)
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(q0), __float_as_uint(q1),
+                                                     false, false);
+    const float qt = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);  // Q~ of this lane's column
     const float E = fmaf(scal[0], am, scal[1]);
-    const float sqt = sqrtf(qt);
+    // v_sqrt_f32 (within 1 ulp; the 4e-6 factors below cover it)
+    const float sqt = __builtin_amdgcn_sqrtf(qt);
     const float rl = fmaf(sqt, 1.f - 4e-6f, -E);
     const float qlb = rl > 0.f ? rl * rl * (1.f - 1e-5f) : 0.f;
     const float rh = fmaf(sqt, 1.f + 4e-6f, E);
@@ -382,24 +403,28 @@ estep_screen_kernel(EParams p, SParams sp) {
         }
       }
     };
-    ScreenStage<D> st;
-    st.load(sp.frag, nb < kScreenKps ? (int)nb : kScreenKps);
-    st.store(lds[0], nb < kScreenKps ? (int)nb : kScreenKps);
+    auto stage_kernels = [&](int64_t n0) -> int {
+      return n0 >= nb ? 0 : (nb - n0 < kScreenKps ? (int)(nb - n0) : kScreenKps);
+    };
+    screen_stage_copy<D>(sp.frag, ring0, stage_kernels(0), w, lane);
     __syncthreads();
-    int buf = 0;
-    for (int64_t n0 = 0; n0 < nb; n0 += kScreenKps) {
+    // one stage: copy the next into the other array, screen this one's kernels
+    auto stage = [&](int64_t n0, const uint8_t *fb, uint8_t *next) {
       const int64_t nx = n0 + kScreenKps;
-      const int kn = nx >= nb ? 0 : (nb - nx < kScreenKps ? (int)(nb - nx) : kScreenKps);
-      st.load(sp.frag + nx * FB, kn);
-      const uint8_t *fb = lds[buf];
-      float hi0, lo0, hi1, lo1;
-      bounds(fb, n0, hi0, lo0);
-      bounds(fb + L::kOne, n0 + 1, hi1, lo1);
-      consider(n0, hi0, lo0);
-      if (n0 + 1 < nb) consider(n0 + 1, hi1, lo1);
-      st.store(lds[buf ^ 1], kn);
+      screen_stage_copy<D>(sp.frag + nx * FB, next, stage_kernels(nx), w, lane);
+#pragma unroll
+      for (int kk = 0; kk < kScreenKps; ++kk) {
+        if (n0 + kk < nb) {
+          float hi, lo;
+          bounds(fb + kk * L::kOne, n0 + kk, hi, lo);
+          consider(n0 + kk, hi, lo);
+        }
+      }
       __syncthreads();
-      buf ^= 1;
+    };
+    for (int64_t n0 = 0; n0 < nb; n0 += 2 * kScreenKps) {
+      stage(n0, ring0, ring1);
+      if (n0 + kScreenKps < nb) stage(n0 + kScreenKps, ring1, ring0);
     }
     // exact phase (wave-local): compact the lanes' queues into one list,
     // one pair per lane, in kernel order per lane
