@@ -16,7 +16,8 @@ the number of iterations actually timed (and of warm-up iterations) is the
 largest that keeps the run within --budget-s of wall time, at most K (and W);
 the line reports the numbers run, with the requested ones beside them.
 Beside the headline: one whole FP32 iteration of the same workload, the
-mix32_d6 FP32 path and the mix64_d21 precision form.  Prints one JSON line.
+mix32_d6 FP32 path, the mix64_d21 precision form and the latent_d43
+differentiation form (LocalHierarchicalKDE).  Prints one JSON line.
 """
 
 import argparse
@@ -426,6 +427,75 @@ def precision_block(name="mix64_d21", sweeps=2):
     return res
 
 
+def c4_block(N=30000, M=3000, k=256, reps=3):
+    """The differentiation form on latent_d43 (N=30,000, M=3,000, D=43, k=256,
+    threshold 1/256): LocalHierarchicalKDE initialisation, the held-out log
+    density with its gradient w.r.t. the query points and the 43 x 43 shared
+    prior factor (HIP backward and the float64 D x D chain,
+    lakde/local_hierarchical_kde.py:260-271 feeding
+    lakde/local_adaptive_kde.py:200-223), timed over `reps` calls after one
+    warm-up call; then one VI iteration is attempted (the reference's own fit
+    of this data stops with "Stabilization did not converge")."""
+    import torch
+
+    from lakde_amd import LocalHierarchicalKDE, synthetic
+    from lakde_amd.autograd import log_pred_density
+    Xh, Yh = synthetic.make("latent_d43", N, M)
+    X = torch.from_numpy(Xh).cuda()
+    Y = torch.from_numpy(Yh).cuda()
+    D = X.size(1)
+    t = 1 / k
+    m = LocalHierarchicalKDE(k_or_knn_graph=k, block_size=512)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.init_parameters_(X, t)
+    m.init_summaries_(X, t)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+
+    def fwd_bwd():
+        P = m.W_shared_triu.T.detach().clone().requires_grad_(True)
+        Yd = Y.clone().requires_grad_(True)
+        out = log_pred_density(m, X, Yd, prior_factor=P)
+        out.sum().backward()
+        return float(out.detach().double().mean()), P.grad, Yd.grad
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        J, gP, gY = fwd_bwd()
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / reps
+    with torch.no_grad():
+        log_pred_density(m, X, Y)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        log_pred_density(m, X, Y)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+    res = {"workload": f"latent_d43: LocalHierarchicalKDE, N={N}, M={M}, D={D}, k={k}, "
+                       f"threshold=1/{k}, block_size=512, state after initialisation",
+           "init_s": round(t_init, 2),
+           "forward_s": round(t_fwd, 4), "forward_test_pts_per_s": round(M / t_fwd, 1),
+           "grad_s": round(t_grad, 4), "grad_test_pts_per_s": round(M / t_grad, 1),
+           "mean_log_density": J,
+           "grad_norms": {"query_points": float(gY.norm()), "prior_factor": float(gP.norm())},
+           "note": "grad: forward + backward w.r.t. the query points and the shared prior factor "
+                   "(float64 D x D chain included), per call"}
+    log(f"latent_d43: init {t_init:.2f}s, log density {t_fwd * 1e3:.1f} ms, with gradients "
+        f"{t_grad * 1e3:.1f} ms for {M} test points")
+    t0 = time.perf_counter()
+    try:
+        _sweep(m, X, t)
+        torch.cuda.synchronize()
+        res["vi_iteration_s"] = round(time.perf_counter() - t0, 3)
+    except RuntimeError as e:
+        res["vi_iteration"] = f"stopped like the reference's fit: {e}"
+    log(f"latent_d43: VI iteration {res.get('vi_iteration_s', res.get('vi_iteration'))}")
+    return res
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -539,7 +609,7 @@ def main():
     # timed region: prediction, tuning, CPU baseline, secondary block)
     # (about twice the measured durations of the legs after the timed region)
     reserve = 15.0 + (0 if args.no_tuning else 40.0) + (0 if args.no_cpu_baseline else 20.0) \
-        + (0 if args.no_secondary else 30.0)
+        + (0 if args.no_secondary else 45.0)
     # a second warm-up iteration through fit() (the timed path; the first
     # went through the per-step API) gives the iteration time the budget uses
     warm_done, t_it = 1, t_sweep
@@ -637,7 +707,7 @@ def main():
     del model
     torch.cuda.empty_cache()
 
-    sec = c3 = None
+    sec = c3 = c4 = None
     if not args.no_secondary:
         try:
             sec = secondary()
@@ -649,6 +719,11 @@ def main():
         except Exception as e:
             log(f"mix64_d21 precision block failed: {e!r}")
             c3 = {"error": repr(e)}
+        try:
+            c4 = c4_block()
+        except Exception as e:
+            log(f"latent_d43 differentiation block failed: {e!r}")
+            c4 = {"error": repr(e)}
 
     value = steps / elapsed
     out = {
@@ -685,6 +760,7 @@ def main():
         "prior_scale_tuning": tuning,
         "secondary": sec,
         "precision_d21": c3,
+        "differentiation_d43": c4,
         "wall_s": round(time.perf_counter() - t_start, 1),
     }
     print(json.dumps(out), flush=True)

@@ -57,6 +57,7 @@ typedef struct {
 #define LAKDE_I64 1
 #define LAKDE_F32 2
 #define LAKDE_F64 3
+#define LAKDE_F16 4  /* storage only: the reference ops' float16 form computes in float32 */
 
 /* E-step modes (lakde_estep) */
 #define LAKDE_ESTEP_ACCUM 0  /* c[m] = logaddexp(c[m], LSE_n log rho_nm)         */
@@ -84,8 +85,10 @@ int lakde_ind2ptr(const void *ind, int ind_dtype, int64_t nnz, int64_t M,
 /* out[n] = sum_{e in [rowptr[n], rowptr[n+1])} value[e] d_e d_e^T,
  * d_e = Xm[col[e]] - Xn[n]; out is N x D x D contiguous, both triangles.
  * rowptr (N+1) and col are int32/int64; value, Xn (N x D), Xm (M x D) and out
- * are fp_dtype (F32/F64), all contiguous.
- * Replaces W_inv_cuda (csrc/cuda/W_inv_cuda.h:5-7; lakde/kernels.py:102-113). */
+ * are fp_dtype (F16/F32/F64; F16 values are read into and summed in float32,
+ * the result rounded once to float16), all contiguous.
+ * Replaces W_inv_cuda (csrc/cuda/W_inv_cuda.h:5-7, its dispatch over float16,
+ * float32 and float64 at csrc/cuda/W_inv_cuda.cu:102-103; lakde/kernels.py:102-113). */
 int lakde_w_inv(const void *rowptr, int rowptr_dtype, const void *col, int col_dtype,
                 const void *value, const void *Xn, const void *Xm, int64_t N, int64_t M,
                 int64_t D, int fp_dtype, void *out, lakde_stream_t stream);
@@ -93,7 +96,9 @@ int lakde_w_inv(const void *rowptr, int rowptr_dtype, const void *col, int col_d
 /* out[n, m] = || W[n]^T (Xm[m] - Xn[n]) ||^2 with W[n] upper triangular
  * (entries below the diagonal are ignored).  Strides are in elements and may
  * be zero or negative-free arbitrary (stride-0 broadcasts are allowed).
- * fp_dtype F32/F64.  Replaces chol_quad_form_cuda (csrc/cuda/chol_quad_form.h:5-6;
+ * fp_dtype F16/F32/F64 (F16: float32 arithmetic, the result rounded to
+ * float16).  Replaces chol_quad_form_cuda (csrc/cuda/chol_quad_form.h:5-6,
+ * dispatched over the same three types at csrc/cuda/chol_quad_form.cu:105-106;
  * lakde/kernels.py:115-120). */
 int lakde_chol_quad_form(const void *W, int64_t sW0, int64_t sW1, int64_t sW2,
                          const void *Xn, int64_t sXn0, int64_t sXn1,

@@ -13,7 +13,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LAKDE_AMD_LIB", os.path.join(_HERE, "liblakde_amd.so"))
 
-LAKDE_I32, LAKDE_I64, LAKDE_F32, LAKDE_F64 = 0, 1, 2, 3
+LAKDE_I32, LAKDE_I64, LAKDE_F32, LAKDE_F64, LAKDE_F16 = 0, 1, 2, 3, 4
 LAKDE_OK, LAKDE_EINVAL, LAKDE_ECAPACITY, LAKDE_ESTABILIZE = 0, 1, 4, 5
 ESTEP_ACCUM, ESTEP_FIXED, ESTEP_UPDATE = 0, 1, 2
 PRIOR_KNN, PRIOR_SHARED, PRIOR_KNN_CACHED = 0, 1, 2
@@ -178,7 +178,9 @@ def fp_code(dtype):
         return LAKDE_F32
     if dtype == torch.float64:
         return LAKDE_F64
-    raise TypeError(f"floating tensors must be float32 or float64, got {dtype}")
+    if dtype == torch.float16:
+        return LAKDE_F16
+    raise TypeError(f"floating tensors must be float16, float32 or float64, got {dtype}")
 
 
 def require_gpu(*tensors):

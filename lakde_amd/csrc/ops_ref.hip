@@ -3,6 +3,8 @@
 // covariance, posterior contributions), written for gfx950 wave64.
 #include <mutex>
 
+#include <hip/hip_fp16.h>
+
 #include "common.hpp"
 
 namespace lakde {
@@ -57,11 +59,13 @@ constexpr int kStageBatch = 8;        // staging loads in flight per thread
 constexpr int kTileMinD = 23;         // below: P <= 256, the entry-group split
 constexpr int kTileMaxD = 63;
 
-template <typename T, typename RP, typename CI, bool GRAPH>
+// S: storage type of value, Xn, Xm and out (the arithmetic is in T; S =
+// __half with T = float for the float16 form of W_inv)
+template <typename T, typename RP, typename CI, bool GRAPH, typename S = T>
 __global__ void __launch_bounds__(kOuterThreads)
 outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
-                 const T *__restrict__ value, const T *__restrict__ Xn,
-                 const T *__restrict__ Xm, int64_t D, int64_t K, T *__restrict__ out,
+                 const S *__restrict__ value, const S *__restrict__ Xn,
+                 const S *__restrict__ Xm, int64_t D, int64_t K, S *__restrict__ out,
                  T *__restrict__ nu_out, T nu0, const T *__restrict__ colscale,
                  OldSums old) {
   __shared__ T stage[kStageElems];
@@ -76,7 +80,7 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
   if (GRAPH) { beg = n * K; end = beg + K; }
   else { beg = (int64_t)rowptr[n]; end = (int64_t)rowptr[n + 1]; }
 
-  for (int64_t d = tid; d < D; d += kOuterThreads) xn_s[d] = Xn[n * D + d];
+  for (int64_t d = tid; d < D; d += kOuterThreads) xn_s[d] = (T)Xn[n * D + d];
   if (old.unsafe_reset != nullptr && n == 0 && tid == 0) *old.unsafe_reset = 0;
   T nu_acc = nu0;
 
@@ -143,7 +147,7 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
           const int t = t0 + u * kOuterThreads;
-          xv[u] = t < nt ? Xm[cidx[u] * D + (t - (t / Di) * Di)] : T(0);
+          xv[u] = t < nt ? (T)Xm[cidx[u] * D + (t - (t / Di) * Di)] : T(0);
         }
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
@@ -152,13 +156,13 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
         }
       }
       for (int64_t e = tid; e < ne; e += kOuterThreads) {
-        T wv = GRAPH ? T(1) : value[c0 + e];
+        T wv = GRAPH ? T(1) : (T)value[c0 + e];
         if (!GRAPH && colscale != nullptr) wv = wv * colscale[(int64_t)col[c0 + e]];
         wts[e] = wv;
         if (!GRAPH && old.colsum != nullptr && pbase == 0) {
           const int64_t ci = (int64_t)col[c0 + e];
           fx_atomic_add(&old.colsum[ci], (float)wv);
-          fx_atomic_add(&old.stored[ci], (float)value[c0 + e]);
+          fx_atomic_add(&old.stored[ci], (float)(T)value[c0 + e]);
         }
       }
       __syncthreads();
@@ -214,48 +218,49 @@ outer_sum_kernel(const RP *__restrict__ rowptr, const CI *__restrict__ col,
           if (i < D && j < D && i <= j) {
             T v = acc[a2 * 3 + b2];
             if (GRAPH) v = v / T(K);
-            out[(n * D + i) * D + j] = v;
-            out[(n * D + j) * D + i] = v;
+            out[(n * D + i) * D + j] = (S)v;
+            out[(n * D + j) * D + i] = (S)v;
           }
         }
     } else if (g == 0 && active && !tiled) {
       for (int q = 0; q < np; ++q) {
         T v = acc[q];
         if (GRAPH) v = v / T(K);
-        out[(n * D + pi[q]) * D + pj[q]] = v;
-        out[(n * D + pj[q]) * D + pi[q]] = v;
+        out[(n * D + pi[q]) * D + pj[q]] = (S)v;
+        out[(n * D + pj[q]) * D + pi[q]] = (S)v;
       }
     }
   }
   if (!GRAPH && nu_out != nullptr && tid == 0) nu_out[n] = nu_acc;
 }
 
-template <typename T, typename RP, typename CI, bool GRAPH>
+template <typename T, typename RP, typename CI, bool GRAPH, typename S = T>
 static int launch_outer(const void *rowptr, const void *col, const void *value, const void *Xn,
                         const void *Xm, int64_t N, int64_t D, int64_t K, void *out, void *nu,
                         T nu0, hipStream_t s, const T *colscale = nullptr,
                         OldSums old = OldSums{nullptr, nullptr, nullptr}) {
   if (N == 0) return LAKDE_OK;
-  hipLaunchKernelGGL((outer_sum_kernel<T, RP, CI, GRAPH>), dim3((unsigned)N), dim3(kOuterThreads),
-                     0, s, (const RP *)rowptr, (const CI *)col, (const T *)value, (const T *)Xn,
-                     (const T *)Xm, D, K, (T *)out, (T *)nu, nu0, colscale, old);
+  hipLaunchKernelGGL((outer_sum_kernel<T, RP, CI, GRAPH, S>), dim3((unsigned)N),
+                     dim3(kOuterThreads), 0, s, (const RP *)rowptr, (const CI *)col,
+                     (const S *)value, (const S *)Xn, (const S *)Xm, D, K, (S *)out, (T *)nu, nu0,
+                     colscale, old);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }
 
-template <typename T>
+template <typename T, typename S = T>
 static int dispatch_outer_idx(int rp_dt, int col_dt, const void *rowptr, const void *col,
                               const void *value, const void *Xn, const void *Xm, int64_t N,
                               int64_t D, void *out, void *nu, T nu0, hipStream_t s,
                               const T *cs = nullptr) {
   if (rp_dt == LAKDE_I32 && col_dt == LAKDE_I32)
-    return launch_outer<T, int32_t, int32_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
+    return launch_outer<T, int32_t, int32_t, false, S>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   if (rp_dt == LAKDE_I32 && col_dt == LAKDE_I64)
-    return launch_outer<T, int32_t, int64_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
+    return launch_outer<T, int32_t, int64_t, false, S>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   if (rp_dt == LAKDE_I64 && col_dt == LAKDE_I32)
-    return launch_outer<T, int64_t, int32_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
+    return launch_outer<T, int64_t, int32_t, false, S>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   if (rp_dt == LAKDE_I64 && col_dt == LAKDE_I64)
-    return launch_outer<T, int64_t, int64_t, false>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
+    return launch_outer<T, int64_t, int64_t, false, S>(rowptr, col, value, Xn, Xm, N, D, 0, out, nu, nu0, s, cs);
   set_error("unsupported index dtype combination (int32/int64 only)");
   return LAKDE_EDTYPE;
 }
@@ -271,12 +276,12 @@ struct QfCfg {
   static constexpr int NK = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 
-template <typename T, int DP>
+template <typename T, int DP, typename S = T>
 __global__ void __launch_bounds__(256)
-quad_form_kernel(const T *__restrict__ W, int64_t sW0, int64_t sW1, int64_t sW2,
-                 const T *__restrict__ Xn, int64_t sXn0, int64_t sXn1,
-                 const T *__restrict__ Xm, int64_t sXm0, int64_t sXm1,
-                 T *__restrict__ out, int64_t so0, int64_t so1, int64_t N, int64_t M, int D) {
+quad_form_kernel(const S *__restrict__ W, int64_t sW0, int64_t sW1, int64_t sW2,
+                 const S *__restrict__ Xn, int64_t sXn0, int64_t sXn1,
+                 const S *__restrict__ Xm, int64_t sXm0, int64_t sXm1,
+                 S *__restrict__ out, int64_t so0, int64_t so1, int64_t N, int64_t M, int D) {
   constexpr int NK = QfCfg<T, DP>::NK;
   __shared__ T Ls[NK][DP][DP];
   __shared__ T xs[NK][DP];
@@ -287,19 +292,19 @@ quad_form_kernel(const T *__restrict__ W, int64_t sW0, int64_t sW1, int64_t sW2,
     int k = t / (DP * DP), r = t - k * DP * DP, i = r / DP, j = r - i * DP;
     int64_t n = n0 + k;
     T v = T(0);
-    if (n < N && i < D && j <= i) v = W[n * sW0 + (int64_t)j * sW1 + (int64_t)i * sW2];
+    if (n < N && i < D && j <= i) v = (T)W[n * sW0 + (int64_t)j * sW1 + (int64_t)i * sW2];
     Ls[k][i][j] = v;
   }
   for (int t = tid; t < NK * DP; t += 256) {
     int k = t / DP, j = t - k * DP;
     int64_t n = n0 + k;
-    xs[k][j] = (n < N && j < D) ? Xn[n * sXn0 + (int64_t)j * sXn1] : T(0);
+    xs[k][j] = (n < N && j < D) ? (T)Xn[n * sXn0 + (int64_t)j * sXn1] : T(0);
   }
   __syncthreads();
   if (m >= M) return;
   T xm[DP];
 #pragma unroll
-  for (int j = 0; j < DP; ++j) xm[j] = (j < D) ? Xm[m * sXm0 + (int64_t)j * sXm1] : T(0);
+  for (int j = 0; j < DP; ++j) xm[j] = (j < D) ? (T)Xm[m * sXm0 + (int64_t)j * sXm1] : T(0);
   for (int k = 0; k < NK; ++k) {
     int64_t n = n0 + k;
     if (n >= N) break;
@@ -315,17 +320,17 @@ quad_form_kernel(const T *__restrict__ W, int64_t sW0, int64_t sW1, int64_t sW2,
       for (int j = 0; j <= i; ++j) y = fma(Ls[k][i][j], d[j], y);
       q = fma(y, y, q);
     }
-    out[n * so0 + m * so1] = q;
+    out[n * so0 + m * so1] = (S)q;
   }
 }
 
 // D above 64: one thread per (n, m) pair, factor read from global memory.
-template <typename T>
+template <typename T, typename S = T>
 __global__ void __launch_bounds__(256)
-quad_form_generic_kernel(const T *__restrict__ W, int64_t sW0, int64_t sW1, int64_t sW2,
-                         const T *__restrict__ Xn, int64_t sXn0, int64_t sXn1,
-                         const T *__restrict__ Xm, int64_t sXm0, int64_t sXm1,
-                         T *__restrict__ out, int64_t so0, int64_t so1, int64_t N, int64_t M,
+quad_form_generic_kernel(const S *__restrict__ W, int64_t sW0, int64_t sW1, int64_t sW2,
+                         const S *__restrict__ Xn, int64_t sXn0, int64_t sXn1,
+                         const S *__restrict__ Xm, int64_t sXm0, int64_t sXm1,
+                         S *__restrict__ out, int64_t so0, int64_t so1, int64_t N, int64_t M,
                          int64_t D) {
   int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t n = blockIdx.y;
@@ -334,43 +339,43 @@ quad_form_generic_kernel(const T *__restrict__ W, int64_t sW0, int64_t sW1, int6
   for (int64_t i = 0; i < D; ++i) {
     T y = T(0);
     for (int64_t j = 0; j <= i; ++j) {
-      T dj = Xm[m * sXm0 + j * sXm1] - Xn[n * sXn0 + j * sXn1];
-      y = fma(W[n * sW0 + j * sW1 + i * sW2], dj, y);
+      T dj = (T)Xm[m * sXm0 + j * sXm1] - (T)Xn[n * sXn0 + j * sXn1];
+      y = fma((T)W[n * sW0 + j * sW1 + i * sW2], dj, y);
     }
     q = fma(y, y, q);
   }
-  out[n * so0 + m * so1] = q;
+  out[n * so0 + m * so1] = (S)q;
 }
 
-template <typename T, int DP>
+template <typename T, int DP, typename S = T>
 static int launch_qf(const void *W, int64_t sW0, int64_t sW1, int64_t sW2, const void *Xn,
                      int64_t sXn0, int64_t sXn1, const void *Xm, int64_t sXm0, int64_t sXm1,
                      void *out, int64_t so0, int64_t so1, int64_t N, int64_t M, int64_t D,
                      hipStream_t s) {
   constexpr int NK = QfCfg<T, DP>::NK;
   dim3 grid((unsigned)ceil_div(M, 256), (unsigned)ceil_div(N, NK));
-  hipLaunchKernelGGL((quad_form_kernel<T, DP>), grid, dim3(256), 0, s, (const T *)W, sW0, sW1,
-                     sW2, (const T *)Xn, sXn0, sXn1, (const T *)Xm, sXm0, sXm1, (T *)out, so0,
+  hipLaunchKernelGGL((quad_form_kernel<T, DP, S>), grid, dim3(256), 0, s, (const S *)W, sW0, sW1,
+                     sW2, (const S *)Xn, sXn0, sXn1, (const S *)Xm, sXm0, sXm1, (S *)out, so0,
                      so1, N, M, (int)D);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }
 
-template <typename T>
+template <typename T, typename S = T>
 static int dispatch_qf(const void *W, int64_t sW0, int64_t sW1, int64_t sW2, const void *Xn,
                        int64_t sXn0, int64_t sXn1, const void *Xm, int64_t sXm0, int64_t sXm1,
                        void *out, int64_t so0, int64_t so1, int64_t N, int64_t M, int64_t D,
                        hipStream_t s) {
 #define QF_ARGS W, sW0, sW1, sW2, Xn, sXn0, sXn1, Xm, sXm0, sXm1, out, so0, so1, N, M, D, s
-  if (D <= 4) return launch_qf<T, 4>(QF_ARGS);
-  if (D <= 8) return launch_qf<T, 8>(QF_ARGS);
-  if (D <= 16) return launch_qf<T, 16>(QF_ARGS);
-  if (D <= 32) return launch_qf<T, 32>(QF_ARGS);
-  if (D <= 64) return launch_qf<T, 64>(QF_ARGS);
+  if (D <= 4) return launch_qf<T, 4, S>(QF_ARGS);
+  if (D <= 8) return launch_qf<T, 8, S>(QF_ARGS);
+  if (D <= 16) return launch_qf<T, 16, S>(QF_ARGS);
+  if (D <= 32) return launch_qf<T, 32, S>(QF_ARGS);
+  if (D <= 64) return launch_qf<T, 64, S>(QF_ARGS);
 #undef QF_ARGS
   dim3 grid((unsigned)ceil_div(M, 256), (unsigned)N);
-  hipLaunchKernelGGL((quad_form_generic_kernel<T>), grid, dim3(256), 0, s, (const T *)W, sW0, sW1,
-                     sW2, (const T *)Xn, sXn0, sXn1, (const T *)Xm, sXm0, sXm1, (T *)out, so0,
+  hipLaunchKernelGGL((quad_form_generic_kernel<T, S>), grid, dim3(256), 0, s, (const S *)W, sW0,
+                     sW1, sW2, (const S *)Xn, sXn0, sXn1, (const S *)Xm, sXm0, sXm1, (S *)out, so0,
                      so1, N, M, D);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
@@ -442,7 +447,10 @@ int lakde_w_inv(const void *rowptr, int rowptr_dtype, const void *col, int col_d
   if (fp_dtype == LAKDE_F64)
     return dispatch_outer_idx<double>(rowptr_dtype, col_dtype, rowptr, col, value, Xn, Xm, N, D,
                                       out, nullptr, 0.0, s);
-  set_error("lakde_w_inv: value dtype must be float32/float64");
+  if (fp_dtype == LAKDE_F16)
+    return dispatch_outer_idx<float, __half>(rowptr_dtype, col_dtype, rowptr, col, value, Xn, Xm,
+                                             N, D, out, nullptr, 0.f, s);
+  set_error("lakde_w_inv: value dtype must be float16/float32/float64");
   return LAKDE_EDTYPE;
 }
 
@@ -461,7 +469,10 @@ int lakde_chol_quad_form(const void *W, int64_t sW0, int64_t sW1, int64_t sW2, c
   if (fp_dtype == LAKDE_F64)
     return dispatch_qf<double>(W, sW0, sW1, sW2, Xn, sXn0, sXn1, Xm, sXm0, sXm1, out, so0, so1,
                                N, M, D, s);
-  set_error("lakde_chol_quad_form: dtype must be float32/float64");
+  if (fp_dtype == LAKDE_F16)
+    return dispatch_qf<float, __half>(W, sW0, sW1, sW2, Xn, sXn0, sXn1, Xm, sXm0, sXm1, out, so0,
+                                      so1, N, M, D, s);
+  set_error("lakde_chol_quad_form: dtype must be float16/float32/float64");
   return LAKDE_EDTYPE;
 }
 

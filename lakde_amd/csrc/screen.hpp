@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) screen_knn_bound_kernel(
 // i + 1 is issued at the start of iteration i and has the whole iteration
 // to land; the barrier at the end of the iteration waits for it
 // (vmcnt(0)) and frees stage i for the copy after next.
-constexpr int kScreenKps = 4;
+constexpr int kScreenKps = 2;
 
 template <int D>
 struct ScreenLds {
@@ -252,7 +252,7 @@ __device__ __forceinline__ float log_rho_lane(const float *rec, const float *__r
 }
 
 template <int D, int MODE>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, 3)
 estep_screen_kernel(EParams p, SParams sp) {
   using L = ScreenLds<D>;
   constexpr int Kp = sc_kp(D);

@@ -12,8 +12,10 @@ the ops directly keeps working.
 
 Differences from the reference, all deliberate: CPU tensors raise (there is no
 fallback path); unsupported dtypes raise instead of NaN-filling ``out``
-(csrc/cuda/W_inv_cuda.cu:132-134, convert_cuda.cu:60-62); float16 is not
-supported; ``chol_quad_form`` reads only the upper triangle of W_triu.
+(csrc/cuda/W_inv_cuda.cu:132-134, convert_cuda.cu:60-62); float16 inputs
+are read into float32 arithmetic and the result is rounded once to float16
+(the reference computes in half arithmetic); ``chol_quad_form`` reads only
+the upper triangle of W_triu.
 """
 
 import torch

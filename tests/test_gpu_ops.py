@@ -77,6 +77,39 @@ def test_w_inv_matches_oracle(K, fp, D):
     assert np.all(_np(out)[3] == 0)
 
 
+@pytest.mark.parametrize("D", [2, 6, 43, 63])
+def test_w_inv_float16(K, D):
+    """float16 inputs (csrc/cuda/W_inv_cuda.cu:102-103 dispatches kHalf):
+    read into float32 arithmetic, the result rounded once to float16, so
+    each entry is within half an ulp of float16 (2^-11 relative) of the
+    float64 sum over the same float16 inputs, plus the float32 accumulation
+    error (1e-5 of the row's largest entry)."""
+    rng = np.random.default_rng(16 + D)
+    N, M = 29, 200
+    counts = rng.integers(0, 60, size=N)
+    counts[2] = 0
+    rowptr = np.concatenate([[0], np.cumsum(counts)])
+    col = rng.integers(0, M, size=rowptr[-1])
+    val = rng.random(rowptr[-1]).astype(np.float16)
+    Xn = rng.standard_normal((N, D)).astype(np.float16)
+    Xm = rng.standard_normal((M, D)).astype(np.float16)
+    ref = O.calc_W_inv(rowptr, col, val.astype(np.float64), Xn.astype(np.float64),
+                       Xm.astype(np.float64), dt=np.float64)
+    f16 = torch.float16
+    out = K.calc_W_inv(_t(rowptr, torch.int32), _t(col, torch.int64), _t(val, f16), _t(Xn, f16),
+                       _t(Xm, f16))
+    assert out.dtype == f16
+    got = _np(out).astype(np.float64)
+    scale = np.abs(ref).max(axis=(1, 2), keepdims=True) + 1e-30
+    assert np.all(np.abs(got - ref) <= 2.0 ** -11 * np.abs(ref) + 1e-5 * scale)
+    assert np.all(got[2] == 0)
+    # the registered op writes a float16 out in place
+    o2 = torch.empty_like(out)
+    r = torch.ops.lakde.W_inv_cuda(_t(rowptr), _t(col), _t(val, f16), _t(Xn, f16), _t(Xm, f16), o2)
+    assert r.data_ptr() == o2.data_ptr()
+    np.testing.assert_array_equal(_np(o2), _np(out))
+
+
 def test_w_inv_golden(K, golden):
     g = golden("ops")
     out = K.calc_W_inv(_t(g["winv_rowptr"]), _t(g["winv_col"]), _t(g["winv_val"]),
@@ -107,6 +140,25 @@ def test_chol_quad_form_matches_oracle(K, fp, D):
     Q = _np(K.chol_quad_form(W_dev, _t(Xn, fp), _t(Xm, fp)))
     tol = 3e-5 if fp == torch.float32 else 1e-12
     np.testing.assert_allclose(Q, ref, rtol=tol, atol=tol * ref.mean())
+
+
+@pytest.mark.parametrize("D", [1, 5, 17, 63])
+def test_chol_quad_form_float16(K, D):
+    """float16 inputs (csrc/cuda/chol_quad_form.cu:105-106 dispatches kHalf):
+    float32 arithmetic, the result rounded once to float16, against float64
+    over the same float16 inputs: 2^-11 relative plus the float32 error."""
+    rng = np.random.default_rng(300 + D)
+    N, M = 11, 300
+    Wt = _factors(rng, N, D).astype(np.float16)
+    Xn = rng.standard_normal((N, D)).astype(np.float16)
+    Xm = rng.standard_normal((M, D)).astype(np.float16)
+    ref = O.chol_quad_form(Wt.astype(np.float64), Xn.astype(np.float64), Xm.astype(np.float64),
+                           dt=np.float64)
+    f16 = torch.float16
+    Q = K.chol_quad_form(_t(np.swapaxes(Wt, 1, 2), f16).mT, _t(Xn, f16), _t(Xm, f16))
+    assert Q.dtype == f16
+    got = _np(Q).astype(np.float64)
+    assert np.all(np.abs(got - ref) <= 2.0 ** -11 * np.abs(ref) + 3e-5 * ref.mean())
 
 
 def test_chol_quad_form_stride0_and_views(K):
