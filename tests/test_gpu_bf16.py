@@ -135,6 +135,10 @@ def test_screen_estep_matches_fp32_kernels(name, n, k, t):
         np.testing.assert_array_equal(r1, r0)
         np.testing.assert_array_equal(c1, c0)
         np.testing.assert_array_equal(v1, v0)
+        # every stored responsibility passes the reference's threshold test
+        # (lakde/local_adaptive_kde.py:139-143: threshold_ zeroes r <= t, nonzero
+        # keeps the rest; t = 0 keeps r > 0)
+        assert np.all(v1 > t), (name, b, float(v1.min()))
 
 
 @pytest.mark.parametrize("name,n,k", [("mix128_d63", 3000, 96), ("latent_d43", 2500, 32)])
