@@ -500,7 +500,9 @@ int lakde_estep_main_bf16(const lakde_estep_args *args, const void *frag,
  * log-sum-exp, normaliser update and keep test run on those exact values.
  * Outputs as lakde_estep_main (follow with lakde_estep_finish).
  *   lakde_screen_points: xb (lakde_screen_points_bytes(N, D) bytes) =
- *     bf16(X - o) zero padded per point, xnorm (N floats) = |x - o| rounded
+ *     bf16(X - o) zero padded per point, xnorm (2N floats): xnorm[m] =
+ *     |x_m - o| and xnorm[N + m] = |fl(x_m - o) - bf16(x_m - o)| (the norm
+ *     of the rounding residual that the error bound uses), both rounded
  *     up; once per fit (X and o fixed).
  *   lakde_pack_screen: per kernel of a block (lakde_screen_frag_bytes(D)
  *     bytes each) from its records and o.
@@ -545,6 +547,22 @@ int lakde_estep_main_m32(const lakde_estep_args *args, const void *frag, const f
 int lakde_pred_density_bf16(const float *rec, const void *frag, int64_t N, const float *Y,
                             int64_t M, int64_t D, float *out, void *workspace,
                             size_t workspace_bytes, lakde_stream_t stream);
+
+/* The predictive density of the precision form with the screen of
+ * lakde_estep_main_screen: every (kernel, test point) pair is bounded on the
+ * BF16 matrix cores, pairs whose upper bound lies more than
+ * log(kernels per split) + log(1e6) nats below a running lower bound of
+ * their split's partial log-sum-exp are dropped (together below 1e-6 of it,
+ * i.e. 1e-6 nats), the others are evaluated exactly in FP32 with the terms
+ * of lakde_pred_density.  frag: lakde_pack_screen of all N kernels (16-byte
+ * aligned); yb, ynorm: lakde_screen_points of Y with the origin frag was
+ * packed with; workspace: lakde_pred_screen_workspace(N, M) bytes.
+ * (lakde/local_adaptive_kde.py:200-223 as lakde_pred_density.) */
+size_t lakde_pred_screen_workspace(int64_t N, int64_t M);
+int lakde_pred_density_screen(const float *rec, const void *frag, int64_t N, const float *Y,
+                              int64_t M, int64_t D, const void *yb, const float *ynorm,
+                              float *out, void *workspace, size_t workspace_bytes,
+                              lakde_stream_t stream);
 
 /* ------------------------------------------------------------------ *
  * Differentiation form                                                *

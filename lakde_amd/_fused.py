@@ -235,15 +235,18 @@ def screen_frag_bytes(D):
 
 class ScreenPoints:
     """The screening E-step's per-fit operands: the origin o (the mean of
-    X), bf16(x_m - o) zero padded per point and |x_m - o| rounded up."""
+    X, or the given one: test points are prepared with the origin of the
+    training points the fragments were packed with), bf16(x_m - o) zero
+    padded per point, and per point |x_m - o| then the norm of its bf16
+    rounding residual (2N floats, rounded up)."""
 
-    def __init__(self, X):
+    def __init__(self, X, origin=None):
         N, D = X.shape
         lib = _lib.load()
-        self.origin = X.mean(0).contiguous()
+        self.origin = X.mean(0).contiguous() if origin is None else origin.contiguous()
         self.xb = torch.empty(max(int(lib.lakde_screen_points_bytes(N, D)), 1), dtype=torch.uint8,
                               device=X.device)
-        self.xnorm = torch.empty(N, device=X.device)
+        self.xnorm = torch.empty(2 * N, device=X.device)  # |x - o|, then the bf16 residual norms
         check(lib.lakde_screen_points(ptr(X), N, D, ptr(self.origin), ptr(self.xb),
                                       ptr(self.xnorm), stream_of(X)), "screen_points")
         self.lbound = None  # lower bounds of the normalisers for ACCUM passes
@@ -368,6 +371,21 @@ def pack_frag_bf16(rec, D, out):
     nb = rec.size(0)
     check(_lib.load().lakde_pack_frag_bf16(ptr(rec), nb, D, ptr(out), stream_of(rec)),
           "pack_frag_bf16")
+    return out
+
+
+def pred_density_screen(rec, frag, Y, pts, out):
+    """The precision form's predictive density with the BF16 screen (frag:
+    pack_screen of all N kernels; pts: ScreenPoints of Y with the fit's
+    origin)."""
+    N = rec.size(0)
+    M, D = Y.shape
+    lib = _lib.load()
+    nbytes = int(lib.lakde_pred_screen_workspace(N, M))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=Y.device)
+    check(lib.lakde_pred_density_screen(ptr(rec), ptr(frag), N, ptr(Y), M, D, ptr(pts.xb),
+                                        ptr(pts.xnorm), ptr(out), ptr(ws), nbytes, stream_of(Y)),
+          "pred_density_screen")
     return out
 
 

@@ -109,6 +109,9 @@ _SIGNATURES = {
     "lakde_pack_m32": (_int, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "lakde_estep_main_m32": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp, _vp]),
     "lakde_pred_density_bf16": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "lakde_pred_screen_workspace": (_sz, [_i64, _i64]),
+    "lakde_pred_density_screen": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _sz,
+                                         _vp]),
     "lakde_screen_frag_bytes": (_i64, [_i64]),
     "lakde_screen_points_bytes": (_sz, [_i64, _i64]),
     "lakde_screen_points": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp]),

@@ -59,6 +59,8 @@ def _gen_inst(lo, hi):
             "__bf16 *, float *, hipStream_t);",
             f"template int launch_screen_knn_bound<{d}>(const float *, const float *, int64_t, "
             "const void *, int, int64_t, int, float *, hipStream_t);",
+            f"template int launch_pred_screen<{d}>(const PParams &, const uint8_t *, "
+            "const __bf16 *, const float *, float, hipStream_t);",
             f"template int launch_estep_m32<{d}>(const EParams &, const float *, const float *, "
             "int, hipStream_t);",
             f"template int launch_pack_m32<{d}>(const float *, int64_t, const float *, float *, "

@@ -562,4 +562,71 @@ int lakde_pred_density_bf16(const float *rec, const void *frag, int64_t N, const
   return pred_combine(p, stream);
 }
 
+// ------------------------------------ screening predictive density (screen.hpp)
+// Kernel splits sized so that the launch is about eight waves of workgroups
+// at three workgroups per CU (the per-split work then balances), with at
+// least 512 kernels per split.
+static void pred_screen_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit) {
+  const int64_t groups = ceil_div(M > 0 ? M : 1, 256);
+  int64_t ns = ceil_div(6144, groups);
+  const int64_t max_ns = ceil_div(N, 512);
+  if (ns > max_ns) ns = max_ns;
+  if (ns < 1) ns = 1;
+  *per_split = ceil_div(N, ns);
+  *nsplit = ceil_div(N, *per_split);
+}
+
+static size_t pred_screen_part_bytes(int64_t ns, int64_t M) {
+  return ((size_t)ns * (size_t)M * 4 + 255) / 256 * 256;
+}
+
+size_t lakde_pred_screen_workspace(int64_t N, int64_t M) {
+  if (N < 1 || M < 0) return 0;
+  int64_t per, ns;
+  pred_screen_geometry(N, M, &per, &ns);
+  return 2 * pred_screen_part_bytes(ns, M);
+}
+
+int lakde_pred_density_screen(const float *rec, const void *frag, int64_t N, const float *Y,
+                              int64_t M, int64_t D, const void *yb, const float *ynorm,
+                              float *out, void *workspace, size_t workspace_bytes,
+                              lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(D >= 8 && D <= 63, "the screening form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(rec && frag && Y && yb && ynorm && out && workspace && N >= 1 && M >= 0,
+                  "bad args");
+  LAKDE_CHECK_ARG(N < (1LL << 31), "at most 2^31 - 1 kernels");
+  LAKDE_CHECK_ARG(workspace_bytes >= lakde_pred_screen_workspace(N, M), "workspace too small");
+  LAKDE_CHECK_ARG(((uintptr_t)frag & 15) == 0, "frag must be 16-byte aligned");
+  if (M == 0) return LAKDE_OK;
+  PParams p;
+  p.N = N;
+  p.M = M;
+  pred_screen_geometry(N, M, &p.per_split, &p.nsplit);
+  p.rec = rec;
+  p.Y = Y;
+  p.part_m = (float *)workspace;
+  p.part_s = (float *)((char *)workspace + pred_screen_part_bytes(p.nsplit, M));
+  p.out = out;
+  // all dropped terms of a split together stay below 1e-6 of its partial
+  // sum (1e-6 nats): per_split e^-margin = 1e-6.  The Student-t terms fall
+  // off only logarithmically in Q, so every nat of margin keeps many more
+  // pairs in the exact phase than in the E-step
+  const float margin = (float)(std::log((double)p.per_split) + std::log(1e6));
+  hipStream_t s = as_stream(stream);
+  int rc = LAKDE_EINVAL;
+  switch (D) {
+#define CASE(d)                                                                                  \
+  case d:                                                                                        \
+    rc = launch_pred_screen<d>(p, (const uint8_t *)frag, (const __bf16 *)yb, ynorm, margin, s); \
+    break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) {
+    set_error("lakde_pred_density_screen: launch failed");
+    return rc;
+  }
+  return pred_combine(p, stream);
+}
+
 }  // extern "C"

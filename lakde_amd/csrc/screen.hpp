@@ -5,14 +5,22 @@
 // form
 //     y~ = bf16(U_n) bf16(x_m - o) - c_n,   c_n = U_n (x_n - o) in FP32,
 // with U_n = Linv_n, o a fixed origin (the data mean), c_n loaded as the
-// accumulators' initial value and bf16(x_m - o) prepared once per fit.  With
-// u = 2^-8 (bf16 round to nearest), F_n = |U_n|_F, a_m = |x_m - o| and
-// b_n = |x_n - o| (all rounded up), the error of every row of y~ is bounded
-// by (2u + u^2 + FP32 accumulation) sum_j |U_ij| |x_mj - o_j| plus the FP32
-// rounding of c_n, so by Cauchy-Schwarz
-//     | y~ - U_n (x_m - x_n) |_2 <= E_nm = ka F_n a_m + kb F_n b_n
-// (ka = 0.00785 >= 2u + u^2 + 65 * 2^-24 (1 + u)^2, kb = 1.2e-5).  Hence
-// the exact log rho = const_n - nu_n/2 Q lies between
+// accumulators' initial value and bf16(x_m - o) prepared once per fit.
+// Write v = fl(x_m - o) = (x_m - o)(1 + eps) (|eps| <= 2^-24), bf16(v) =
+// v - r and bf16(U) = U - dU, with the residuals r and dU exact in FP32.  The
+// error of y~ is -U r - dU v + dU r + U (x_m - o) eps, plus the FP32
+// accumulation of the 65 terms of each row (<= 65 * 2^-24 (1 + u)^2
+// sum_j |U_ij| |x_mj - o_j|, u = 2^-8) and the FP32 rounding of c_n, so by
+// Cauchy-Schwarz
+//     | y~ - U_n (x_m - x_n) |_2 <= E_nm = P_n r_m + Q_n a_m + C_n
+// with F_n = |U_n|_F, dF_n = |dU_n|_F, r_m = |r|_2, a_m = |x_m - o|,
+// b_n = |x_n - o| (all rounded up), P_n = F_n (1 + u) (U r and dU r, with
+// dF_n <= u F_n), Q_n = dF_n (1 + 2^-24) + 67 * 2^-24 F_n (dU v, the
+// accumulation and eps) and C_n = kb F_n b_n (kb = 1.2e-5 >= 65 * 2^-24
+// (1 + u)^2, the rounding of c_n).  The actual residuals are about 0.4 u of
+// their operands' norms on average, against the worst case u of the bound
+// (2u + u^2) F_n a_m that the form without them would need.  Hence the exact
+// log rho = const_n - nu_n/2 Q lies between
 //     lo = const_n - nu_n/2 (sqrt Q~ + E)^2   and   hi = const_n - nu_n/2 max(sqrt Q~ - E, 0)^2
 // (each widened by the FP32 rounding of Q~ and of the bound itself).
 //
@@ -51,7 +59,9 @@ typedef float sc_f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t sc_u32x4 __attribute__((ext_vector_type(4)));
 
 // error-bound coefficients (see above)
-constexpr float kScreenKa = 0.00785f;
+constexpr double kScreenU = 0x1p-8;     // bf16 unit roundoff
+// FP32 accumulation of 65 terms (65 (1 + u)^2 * 2^-24) and fl(x - o) (2^-24)
+constexpr double kScreenAcc = 67.0 * 0x1p-24;
 constexpr float kScreenKb = 1.2e-5f;
 constexpr int kScreenQ = 16;   // survivor queue entries per lane
 constexpr int kKeepQ = 4;      // keep-candidate queue entries per lane
@@ -80,9 +90,11 @@ __host__ __device__ constexpr int sc_tile_index(int D, int rt_, int s_) {
 // fragment block per kernel, laid out as its LDS image: the used A tiles of
 // bf16(U) (1 KB each: lane l's 8 bf16 operand values at byte 16 l, row
 // 32 rt + l % 32, columns 16 s + 8 (l / 32) + [0, 8)), then -c_n [Dp] f32,
-// then scal[4]: ka F_n, kb F_n b_n, E-step const, E-step exponent (-nu/2)
+// then scal[8]: P_n, Q_n, C_n (the error bound), E-step const, E-step
+// exponent (-nu/2), three unused
+constexpr int kScreenScal = 8;
 __host__ __device__ constexpr int64_t sc_frag_bytes(int D) {
-  return (int64_t)sc_ntiles(D) * 1024 + (int64_t)sc_dp(D) * 4 + 16;
+  return (int64_t)sc_ntiles(D) * 1024 + (int64_t)sc_dp(D) * 4 + 4 * kScreenScal;
 }
 
 // Parameters of the screening launches beyond EParams.
@@ -95,7 +107,9 @@ struct SParams {
 };
 
 // ------------------------------------------------------------- preparation
-// bf16(x_m - o) (zero padded to Kp) and |x_m - o| rounded up, per point.
+// bf16(x_m - o) (zero padded to Kp) per point; xnorm[m] = |x_m - o| and
+// xnorm[N + m] = |fl(x_m - o) - bf16(x_m - o)|, the norm of the rounding
+// residual, both rounded up.
 template <int D>
 __global__ void __launch_bounds__(256) screen_points_kernel(const float *__restrict__ X, int64_t N,
                                                            const float *__restrict__ origin,
@@ -104,7 +118,7 @@ __global__ void __launch_bounds__(256) screen_points_kernel(const float *__restr
   constexpr int Kp = sc_kp(D);
   const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (m >= N) return;
-  double s = 0.0;
+  double s = 0.0, rs = 0.0;
 #pragma unroll
   for (int j = 0; j < Kp; ++j) {
     float v = 0.f;
@@ -112,9 +126,13 @@ __global__ void __launch_bounds__(256) screen_points_kernel(const float *__restr
       v = X[m * D + j] - origin[j];
       s += (double)v * (double)v;
     }
-    xb[m * Kp + j] = (__bf16)v;
+    const __bf16 vb = (__bf16)v;
+    const double rj = (double)v - (double)(float)vb;
+    rs += rj * rj;
+    xb[m * Kp + j] = vb;
   }
   xnorm[m] = (float)(sqrt(s) * (1.0 + 1e-6));
+  xnorm[N + m] = (float)(sqrt(rs) * (1.0 + 1e-6));
 }
 
 // Fragment blocks of nb kernels from their FP32 records: one wave per kernel.
@@ -143,8 +161,9 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
         tile[e] = (__bf16)v;
       }
     }
-  // c_n row by row (lane = row), |U|_F^2 and |x_n - o|^2 in double
-  double f2 = 0.0, b2 = 0.0;
+  // c_n row by row (lane = row), |U|_F^2, |U - bf16(U)|_F^2 and |x_n - o|^2
+  // in double
+  double f2 = 0.0, d2 = 0.0, b2 = 0.0;
   float c = 0.f;
   if (lane < D) {
     const float *row = r + lane * (lane + 1) / 2;
@@ -152,19 +171,27 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
       const float dj = r[rec_x(D) + j] - origin[j];
       c = fmaf(row[j], dj, c);
       f2 += (double)row[j] * (double)row[j];
+      const double e = (double)row[j] - (double)(float)(__bf16)row[j];
+      d2 += e * e;
     }
     const float dl = r[rec_x(D) + lane] - origin[lane];
     b2 = (double)dl * (double)dl;
   }
   if (lane < Dp) cneg[lane] = lane < D ? -c : 0.f;
   f2 = wave_sum(f2);
+  d2 = wave_sum(d2);
   b2 = wave_sum(b2);
   if (lane == 0) {
-    const double F = sqrt(f2) * (1.0 + 1e-6), b = sqrt(b2) * (1.0 + 1e-6);
-    scal[0] = (float)((double)kScreenKa * F * (1.0 + 1e-6));
-    scal[1] = (float)((double)kScreenKb * F * b * (1.0 + 1e-6));
-    scal[2] = r[rec_c(D) + 0];
-    scal[3] = r[rec_c(D) + 1];
+    const double up = 1.0 + 1e-6;
+    const double F = sqrt(f2) * up, dF = sqrt(d2) * up, b = sqrt(b2) * up;
+    scal[0] = (float)(F * (1.0 + kScreenU) * up);
+    scal[1] = (float)((dF * (1.0 + 0x1p-24) + kScreenAcc * F) * up);
+    scal[2] = (float)((double)kScreenKb * F * b * up);
+    scal[3] = r[rec_c(D) + 0];
+    scal[4] = r[rec_c(D) + 1];
+    scal[5] = 0.f;
+    scal[6] = 0.f;
+    scal[7] = 0.f;
   }
 }
 
@@ -296,6 +323,7 @@ estep_screen_kernel(EParams p, SParams sp) {
     }
   }
   const float am = valid ? sp.xnorm[m] : 0.f;
+  const float rm = valid ? sp.xnorm[p.N + m] : 0.f;
 
   float lb = neg_inf();   // UPDATE: bound of the normaliser from the old mass
   float cm = 0.f;         // ACCUM, FIXED: the given normaliser; later the new one
@@ -360,13 +388,13 @@ estep_screen_kernel(EParams p, SParams sp) {
     const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(q0), __float_as_uint(q1),
                                                      false, false);
     const float qt = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);  // Q~ of this lane's column
-    const float E = fmaf(scal[0], am, scal[1]);
+    const float E = fmaf(scal[0], rm, fmaf(scal[1], am, scal[2]));
     // v_sqrt_f32 (within 1 ulp; the 4e-6 factors below cover it)
     const float sqt = __builtin_amdgcn_sqrtf(qt);
     const float rl = fmaf(sqt, 1.f - 4e-6f, -E);
     const float qlb = rl > 0.f ? rl * rl * (1.f - 1e-5f) : 0.f;
     const float rh = fmaf(sqt, 1.f + 4e-6f, E);
-    const float cst = scal[2], expo = scal[3];
+    const float cst = scal[3], expo = scal[4];
     const float slack = fmaf(1e-4f, fabsf(cst), 0.01f);
     hi = fmaf(expo, qlb, cst) + slack;
     lo = fmaf(expo, rh * rh * (1.f + 1e-5f), cst) - slack;
@@ -521,6 +549,245 @@ estep_screen_kernel(EParams p, SParams sp) {
   }
 }
 
+// ----------------------------------------------------- predictive density
+// log p(y_m) = -log N + LSE_n t_nm, t_nm = c_n - h_n log1p(Q_nm) (the
+// predictive constants of the record, lakde/local_adaptive_kde.py:209-223),
+// with the screen above: every (kernel, test point) pair is bounded on the
+// BF16 matrix cores (the same fragments and error bound E; t is decreasing
+// in Q, so Q in [(sqrt Q~ - E)^2, (sqrt Q~ + E)^2] bounds t), a pair whose
+// upper bound lies more than `margin` below a running lower bound B of its
+// split's partial log-sum-exp (the largest lower bound so far, then the
+// largest exact term) is dropped (all dropped terms of a split together are
+// below n_split e^-margin of its partial sum: the host sets that to 1e-6),
+// and the others are queued per
+// test point and evaluated exactly in FP32 (log_pred<D>, the terms of the
+// FP32 predictive kernel).  A wave evaluates its queues whenever one of
+// them could overflow in the next stage, and at the end of the split.
+// Grid: (test-point groups of 256, kernel splits); partial (max, sum-exp)
+// per split, combined by pred_combine_kernel.
+constexpr int kPredQ = 12;  // queued pairs per test point between evaluations
+
+struct PredWaveList {
+  int32_t ent[kPredQ * kWave];
+};
+
+// the exact term of one (kernel, test point) pair; not inlined, so that its
+// registers (the point, the record values in flight) are not live across
+// the screening loop of the caller, where they would force spills
+template <int D>
+__device__ __attribute__((noinline)) float log_pred_lane(const float *rec,
+                                                         const float *__restrict__ yrow) {
+  const float *r = reinterpret_cast<const float *>(__builtin_assume_aligned(rec, 16));
+  float yv[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) yv[t] = yrow[t];
+  return log_pred<D>(r, yv);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256, 3)
+pred_screen_kernel(PParams p, const uint8_t *__restrict__ frag, const __bf16 *__restrict__ yb,
+                   const float *__restrict__ ynorm, float margin) {
+  using L = ScreenLds<D>;
+  constexpr int Kp = sc_kp(D);
+  constexpr int RT = sc_dp(D) / 32, KS = Kp / 16;
+  constexpr int64_t R = rec_floats(D);
+  constexpr int64_t FB = sc_frag_bytes(D);
+  // two stage arrays (see estep_screen_kernel); the queues and the waves'
+  // lists are live across stages, so they have their own memory
+  __shared__ __attribute__((aligned(1024))) uint8_t ring0[L::kBytes];
+  __shared__ __attribute__((aligned(1024))) uint8_t ring1[L::kBytes];
+  __shared__ int32_t sq[kPredQ][256];
+  __shared__ PredWaveList wl[4];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int64_t base = (int64_t)blockIdx.x * 256 + w * kWave;
+  const int64_t m = base + lane;
+  const bool valid = m < p.M;
+  const int64_t split = blockIdx.y;
+  const int64_t sbeg = split * p.per_split;
+  const int64_t send = (sbeg + p.per_split) < p.N ? (sbeg + p.per_split) : p.N;
+  const int64_t nk = send - sbeg;
+  const uint8_t *fsplit = frag + sbeg * FB;
+
+  // the lane's B fragments; reloaded after every exact evaluation (which
+  // needs the registers) instead of being held through it
+  sc_bf16x8 xf[2][KS];
+  auto load_xf = [&]() {
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int64_t col = base + ct * 32 + (lane & 31);
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        if (col < p.M)
+          xf[ct][s] =
+              *reinterpret_cast<const sc_bf16x8 *>(yb + col * Kp + s * 16 + 8 * (lane >> 5));
+        else
+          xf[ct][s] = __builtin_bit_cast(sc_bf16x8, (sc_u32x4){0u, 0u, 0u, 0u});
+      }
+    }
+  };
+  load_xf();
+  const float am = valid ? ynorm[m] : 0.f;
+  const float rm = valid ? ynorm[p.M + m] : 0.f;
+  float B = neg_inf();
+  Lse lse;
+  lse.init();
+  int cnt = 0;
+
+  // bounds [lo, hi] of the exact term of kernel n (LDS image fb) at this
+  // lane's test point
+  auto bounds = [&](const uint8_t *fb, int64_t n, float &hi, float &lo) {
+    const float *cneg = reinterpret_cast<const float *>(fb + sc_ntiles(D) * 1024);
+    const float *scal = cneg + sc_dp(D);
+    const float *rc = p.rec + n * R + rec_c(D);
+    const float cst = rc[2], h = rc[3];
+    sc_f32x16 acc[RT][2];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      sc_f32x16 ci;
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const sc_f32x4 v =
+            *reinterpret_cast<const sc_f32x4 *>(cneg + rt * 32 + 8 * q4 + 4 * (lane >> 5));
+        ci[4 * q4 + 0] = v[0];
+        ci[4 * q4 + 1] = v[1];
+        ci[4 * q4 + 2] = v[2];
+        ci[4 * q4 + 3] = v[3];
+      }
+      acc[rt][0] = ci;
+      acc[rt][1] = ci;
+    }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        if (!sc_tile_used(rt, s)) continue;
+        const sc_bf16x8 a = *reinterpret_cast<const sc_bf16x8 *>(
+            fb + sc_tile_index(D, rt, s) * 1024 + lane * 16);
+        acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[0][s], acc[rt][0], 0, 0, 0);
+        acc[rt][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[1][s], acc[rt][1], 0, 0, 0);
+      }
+    }
+    float q0 = 0.f, q1 = 0.f;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        q0 = fmaf(acc[rt][0][r], acc[rt][0][r], q0);
+        q1 = fmaf(acc[rt][1][r], acc[rt][1][r], q1);
+      }
+    // the halves of each test point's sum of squares (see estep_screen_kernel)
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(q0), __float_as_uint(q1),
+                                                     false, false);
+    const float qt = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+    const float E = fmaf(scal[0], rm, fmaf(scal[1], am, scal[2]));
+    const float sqt = __builtin_amdgcn_sqrtf(qt);  // within 1 ulp, covered below
+    const float rl = fmaf(sqt, 1.f - 4e-6f, -E);
+    const float qlb = rl > 0.f ? rl * rl * (1.f - 1e-5f) : 0.f;
+    const float rh = fmaf(sqt, 1.f + 4e-6f, E);
+    const float qub = rh * rh * (1.f + 1e-5f);
+    // log1p(q) as ln 2 * v_log_f32(1 + q): the rounding of 1 + q (2^-24)
+    // and the 1-ulp logarithm move h log1p by far less than the slack, which
+    // also covers the rounding of the products and of the exact term
+    constexpr float kLn2 = 0.693147182f;
+    const float lhi = h * (kLn2 * __builtin_amdgcn_logf(1.f + qub));
+    const float llo = h * (kLn2 * __builtin_amdgcn_logf(1.f + qlb));
+    const float slack = fmaf(1e-4f, fabsf(cst) + lhi, 0.01f);
+    hi = (cst - llo) + slack;
+    lo = (cst - lhi) - slack;
+    if (!valid) {
+      hi = neg_inf();
+      lo = neg_inf();
+    }
+  };
+
+  // exact evaluation of the wave's queued pairs (wave-local), one per lane,
+  // added to the owning lanes' log-sum-exps
+  auto evaluate = [&]() {
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int t = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += t;
+    }
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, kWave - 1, kWave);
+    PredWaveList &W = wl[w];
+    // entry: (queue slot << 6) | owning lane; replaced by the pair's term
+    for (int k = 0; k < cnt; ++k) W.ent[excl + k] = (k << 6) | lane;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (int j0 = 0; j0 < total; j0 += kWave) {
+      const int j = j0 + lane;
+      if (j < total) {
+        const int e = W.ent[j];
+        const int owner = e & (kWave - 1);
+        const int64_t n = sq[e >> 6][w * kWave + owner];
+        W.ent[j] = __float_as_int(log_pred_lane<D>(p.rec + n * R, p.Y + (base + owner) * D));
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = 0; k < cnt; ++k) lse.add1(__int_as_float(W.ent[excl + k]));
+    B = fmaxf(B, lse.m);
+    cnt = 0;
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  auto stage_kernels = [&](int64_t k0) -> int {
+    return k0 >= nk ? 0 : (nk - k0 < kScreenKps ? (int)(nk - k0) : kScreenKps);
+  };
+  screen_stage_copy<D>(fsplit, ring0, stage_kernels(0), w, lane);
+  __syncthreads();
+  auto stage = [&](int64_t k0, const uint8_t *fb, uint8_t *nxt) {
+    const int64_t kx = k0 + kScreenKps;
+    screen_stage_copy<D>(fsplit + kx * FB, nxt, stage_kernels(kx), w, lane);
+#pragma unroll
+    for (int kk = 0; kk < kScreenKps; ++kk) {
+      if (k0 + kk < nk) {
+        const int64_t n = sbeg + k0 + kk;
+        float hi, lo;
+        bounds(fb + kk * L::kOne, n, hi, lo);
+        B = fmaxf(B, lo);
+        // NaN bounds survive (and propagate through the exact terms)
+        if (valid && !(hi <= B - margin)) {
+          sq[cnt][tid] = (int32_t)n;
+          ++cnt;
+        }
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(cnt > kPredQ - kScreenKps) != 0) {
+      evaluate();
+      load_xf();
+    }
+    __syncthreads();
+  };
+  for (int64_t k0 = 0; k0 < nk; k0 += 2 * kScreenKps) {
+    stage(k0, ring0, ring1);
+    if (k0 + kScreenKps < nk) stage(k0 + kScreenKps, ring1, ring0);
+  }
+  if (__builtin_amdgcn_ballot_w64(cnt > 0) != 0) evaluate();
+  if (valid) {
+    p.part_m[split * p.M + m] = lse.m;
+    p.part_s[split * p.M + m] = lse.s;
+  }
+}
+
+template <int D>
+int launch_pred_screen(const PParams &p, const uint8_t *frag, const __bf16 *yb,
+                       const float *ynorm, float margin, hipStream_t s) {
+  if constexpr (D < 8) {
+    return LAKDE_EINVAL;
+  } else {
+    const dim3 grid((unsigned)ceil_div(p.M, 256), (unsigned)p.nsplit);
+    hipLaunchKernelGGL((pred_screen_kernel<D>), grid, dim3(256), 0, s, p, frag, yb, ynorm, margin);
+    return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+  }
+}
+
 template <int D>
 int launch_estep_screen(const EParams &p, const SParams &sp, int mode, hipStream_t s) {
   if constexpr (D < 8) {
@@ -584,6 +851,8 @@ int launch_screen_knn_bound(const float *rec, const float *X, int64_t N, const v
   extern template int launch_pack_screen<d>(const float *, int64_t, const float *, uint8_t *,  \
                                             hipStream_t);                                       \
   extern template int launch_screen_points<d>(const float *, int64_t, const float *, __bf16 *,  \
-                                              float *, hipStream_t);
+                                              float *, hipStream_t);                             \
+  extern template int launch_pred_screen<d>(const PParams &, const uint8_t *, const __bf16 *,   \
+                                            const float *, float, hipStream_t);
 
 }  // namespace lakde

@@ -42,12 +42,21 @@ class LocallyAdaptiveKDE(AbstractKDE):
     (the precision form, 8 <= D <= 63: the E-step screens every pair on the
     BF16 matrix cores with a rigorous error bound and recomputes exactly in
     FP32 the pairs that can matter to a normaliser or a kept responsibility;
-    the predictive density runs bf16x3 split precision on the matrix cores).
+    the predictive density runs bf16x3 split precision on the matrix cores,
+    or, with ``pred_screen``, screens the same way as the E-step and
+    evaluates the terms that can matter exactly in FP32).
     The FP32 E-step runs on the FP32 matrix cores for 17 <= D <= 63
     (``fp32_mfma``, on by default), on the vector ALUs otherwise.
     """
 
     fp32_mfma = True
+    # precision form: the predictive density with bf16x3 terms for every
+    # pair (False: 1.4e-4 nats from float64 at mix128_d63, 31k test points/s)
+    # or with the BF16 screen and exact FP32 terms for the pairs within the
+    # margin (True: 3.8e-6 nats, 18k test points/s; the Student-t terms fall
+    # off only logarithmically, so about a tenth of the pairs need the
+    # exact term)
+    pred_screen = False
     _screen = None
 
     def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32",
@@ -521,6 +530,14 @@ class LocallyAdaptiveKDE(AbstractKDE):
         self._sync(None)
         out = torch.empty(Y.size(0), device=Y.device)
         if self.precision == "bf16":
+            if self.pred_screen:
+                if self._screen is None:
+                    self._screen = F.ScreenPoints(X)
+                frag = torch.empty(N * F.screen_frag_bytes(D), dtype=torch.uint8,
+                                   device=Y.device)
+                F.pack_screen(self.rec, D, self._screen.origin, frag)
+                pts = F.ScreenPoints(Y, origin=self._screen.origin)
+                return F.pred_density_screen(self.rec, frag, Y, pts, out)
             frag = torch.empty(N * F.bf16_frag_bytes(D), dtype=torch.uint8, device=Y.device)
             F.pack_frag_bf16(self.rec, D, frag)
             return F.pred_density_bf16(self.rec, frag, Y, out)

@@ -11,7 +11,8 @@ FIXED-mode CSR identical; the bf16x3 E-step normalisers within 2e-2 nats of
 the FP32 kernels per column; the fitted model's held-out log density within
 1e-3 nats/point (mean absolute) of the FP32 path and of the FP64 oracle;
 the bf16x3 predictive kernel within 1e-2 nats per point of the FP32 kernel
-on the same posterior.
+on the same posterior; the screening predictive density within 1e-4 nats
+per point of the FP32 kernel and of float64 arithmetic on the same records.
 """
 
 import numpy as np
@@ -236,3 +237,72 @@ def test_bf16_predictive_matches_fp32():
     d = np.abs(outbf - out32)
     print(f"predictive |bf16 - fp32|: max {d.max():.3e} mean {d.mean():.3e} nats")
     assert d.max() < 1e-2
+    # the model's precision-form predictive density, bf16x3 (default) and
+    # screened (pred_screen), on the same fitted state: both against the
+    # FP32 kernel's values (log_pred_density recomputes the same posteriors)
+    model.precision = "bf16"
+    lp_x3 = model.log_pred_density(_t(X), Yd).cpu().numpy()
+    model.pred_screen = True
+    lp_sc = model.log_pred_density(_t(X), Yd).cpu().numpy()
+    assert np.abs(lp_x3 - out32).max() < 1e-2
+    assert np.abs(lp_sc - out32).max() < 1e-4
+
+
+def _pred64(rec, D, Y):
+    """log p(y) in float64 arithmetic from the same float32 records
+    (lakde/local_adaptive_kde.py:209-223)."""
+    r = rec.double().cpu().numpy()
+    N = r.shape[0]
+    P = D * (D + 1) // 2
+    ti = np.tril_indices(D)
+    L = np.zeros((N, D, D))
+    L[:, ti[0], ti[1]] = r[:, :P]
+    xc, coeff, h = r[:, P:P + D], r[:, P + D + 2], r[:, P + D + 3]
+    Y64 = np.asarray(Y, np.float64)
+    out = np.empty(Y64.shape[0])
+    for i in range(Y64.shape[0]):
+        z = np.einsum("nij,nj->ni", L, Y64[i][None] - xc)
+        t = coeff - h * np.log1p((z * z).sum(1))
+        mx = t.max()
+        out[i] = mx + np.log(np.exp(t - mx).sum()) - np.log(N)
+    return out
+
+
+@pytest.mark.parametrize("name,n,m,k", [("mix64_d21", 3000, 300, 32),
+                                        ("mix128_d63", 6000, 150, 96),
+                                        ("latent_d43", 2500, 77, 32),
+                                        ("mix64_d21", 21000, 1, 32)])
+def test_screen_predictive_matches_fp32(name, n, m, k):
+    """The screening predictive density (BF16 screen with a rigorous bound,
+    exact FP32 terms for the pairs within the margin of the running bound)
+    against the FP32 predictive kernel on the same records: within 1e-4
+    nats per point (float32 summation order; the dropped terms are below
+    1e-6 of each split's partial sum), and both within 1e-4 relative
+    of the same density in float64 arithmetic.  The shapes cover one and many
+    kernel splits (21,000 kernels, one test point: 42 splits of 500) and
+    partial workgroups of test points."""
+    from lakde_amd import _fused as F
+    from lakde_amd import synthetic
+    X, Y = synthetic.make(name, n, m)
+    graph = O.compute_knn_graph(X, k)
+    model, _ = _fit(X, Y, graph, "fp32", 512, 1 / k, 1)
+    N, D = X.shape
+    Xd, Yd = _t(X), _t(Y)
+    pts_x = F.ScreenPoints(Xd)
+    frag = torch.empty(N * F.screen_frag_bytes(D), dtype=torch.uint8, device=DEV)
+    F.pack_screen(model.rec, D, pts_x.origin, frag)
+    pts_y = F.ScreenPoints(Yd, origin=pts_x.origin)
+    out32 = F.pred_density(model.rec, Yd, torch.empty(m, device=DEV)).cpu().numpy()
+    outsc = F.pred_density_screen(model.rec, frag, Yd, pts_y,
+                                  torch.empty(m, device=DEV)).cpu().numpy()
+    ref = _pred64(model.rec, D, Y[:min(m, 60)])
+    d = np.abs(outsc - out32)
+    print(f"[{name}] N={N} M={m}: |screen - fp32| max {d.max():.2e}, "
+          f"|screen - float64| max {np.abs(outsc[:ref.size] - ref).max():.2e}", flush=True)
+    assert np.all(np.isfinite(outsc))
+    assert d.max() <= 1e-4
+    # float32 arithmetic against float64: 1e-4 relative (latent_d43's
+    # near-singular posteriors put the FP32 kernel itself 1.5e-4 nats away)
+    bar = 1e-4 * np.maximum(1.0, np.abs(ref))
+    assert np.all(np.abs(outsc[:ref.size] - ref) <= bar)
+    assert np.all(np.abs(out32[:ref.size] - ref) <= bar)
