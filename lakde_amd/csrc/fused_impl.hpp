@@ -381,8 +381,9 @@ estep_main_kernel(const float *__restrict__ rec, const float *__restrict__ X, EP
 // lane walks the segment's chunks in column order and writes each kept
 // entry at rowptr[n] + seg_end[seg - 1][n] + (kept so far),
 // recomputing it with the same device functions (identical bits) ->
-// canonical CSR.
-template <int D>
+// canonical CSR.  VALUES = false (large D): only the rows and columns are
+// written here, and estep_values_kernel computes the values.
+template <int D, bool VALUES = true>
 __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
   const int64_t nnz = *p.nnz_dev;
   if (nnz > p.cap) return;
@@ -404,6 +405,17 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
   int64_t ms[KB];
   int k = 0;
   auto flush = [&]() {
+    if constexpr (!VALUES) {
+      for (int i = 0; i < k; ++i) {
+        if (pos >= 0 && pos < p.cap) {  // never write outside the arena slots
+          p.out_row[pos] = (int32_t)n;
+          p.out_col[pos] = (int32_t)ms[i];
+        }
+        ++pos;
+      }
+      k = 0;
+      return;
+    }
     float xv[KB][D];
 #pragma unroll
     for (int i = 0; i < KB; ++i)
@@ -458,6 +470,48 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
     }
   }
   if (k > 0) flush();
+}
+
+// Values of the compacted entries for large D: a wave per row n (the block's
+// kernel), lanes = the row's entries, so the kernel's record is read once
+// per wave with scalar loads instead of once per lane (the compaction's
+// lanes are 64 different kernels).  Same device functions as the fused
+// compaction: identical values, conversion and column sums.
+template <int D>
+__global__ void __launch_bounds__(256) estep_values_kernel(const float *__restrict__ recs,
+                                                          const float *__restrict__ X,
+                                                          EParams p) {
+  const int64_t nnz = *p.nnz_dev;
+  if (nnz > p.cap) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t n = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (n >= p.nb) return;
+  constexpr int64_t R = rec_floats(D);
+  const float *rec = recs + n * R;
+  const bool to_stored = p.lazy_Rnew != nullptr && *p.lazy_unsafe == 0;
+  const int64_t beg = p.rowptr[n], end = p.rowptr[n + 1];
+  for (int64_t e0 = beg; e0 < end; e0 += kWave) {
+    const int64_t e = e0 + lane;
+    if (e < end) {
+      const int64_t m = p.out_col[e];
+      float xv[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) xv[j] = X[m * D + j];
+      const float x = log_rho<D>(rec, xv);
+      const float cm = p.c[m];
+      const float d = __fsub_rn(x, cm);
+      float r = 0.f;
+      keep_resp(d, p.threshold, p.log_t_lo, &r);
+      float v = r;
+      if (to_stored) {
+        const float coeff = clamp_nan(expf(p.c_old[m] - cm), FLT_MIN, FLT_MAX);
+        v = r / (p.lazy_G[m] * coeff);
+        fx_atomic_add(&p.lazy_Rnew[m], r);
+        fx_atomic_add(&p.lazy_Unew[m], v);
+      }
+      p.out_val[e] = v;
+    }
+  }
 }
 
 // ------------------------------------------------------------ predictive
@@ -609,12 +663,21 @@ int launch_estep_main(const EParams &p, int mode, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
 }
 
+// from this dimension on the values of the compacted entries are computed
+// by estep_values_kernel (a wave per row) instead of in the mask walk
+constexpr int kValuesMinD = 24;
+
 template <int D>
 int launch_estep_compact(const EParams &p, hipStream_t s) {
   if (p.nb == 0) return LAKDE_OK;
-  hipLaunchKernelGGL((estep_compact_kernel<D>),
-                     dim3((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4)), dim3(256), 0,
-                     s, p);
+  const dim3 grid((unsigned)ceil_div(p.nb, kWave), (unsigned)(kESeg / 4));
+  if constexpr (D >= kValuesMinD) {
+    hipLaunchKernelGGL((estep_compact_kernel<D, false>), grid, dim3(256), 0, s, p);
+    hipLaunchKernelGGL((estep_values_kernel<D>), dim3((unsigned)ceil_div(p.nb, 4)), dim3(256), 0,
+                       s, p.rec, p.X, p);
+  } else {
+    hipLaunchKernelGGL((estep_compact_kernel<D, true>), grid, dim3(256), 0, s, p);
+  }
   return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
 }
 
