@@ -679,7 +679,9 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
                 "traffic": traffic, "traffic_note": tnote,
-                "kernel": f"estep_screen_kernel<{D},UPDATE>", "kernel_ms": round(k_ms, 4),
+                "kernel": f"estep_screen_kernel<{D},UPDATE> in its split form (the kernel ranges' "
+                          f"screens, the per-column combine and the keep pass, one event pair "
+                          f"around the three launches)", "kernel_ms": round(k_ms, 4),
                 "launches_timed": k_n, "pairs_per_launch": pairs,
                 "achieved_executed_mfma": round(executed, 2),
                 "frac_executed_mfma": round(executed / PEAK_BF16_TFLOPS, 4),

@@ -409,6 +409,10 @@ typedef struct {
    * the FP32 matrix-core E-step (lakde_pack_m32 + lakde_estep_main_m32,
    * with screen_xnorm and screen_origin; screen_xb may be NULL) */
   int fp32_mfma;
+  /* optional (with the screening E-step): lakde_screen_split_workspace(N, nb)
+   * bytes; the step runs lakde_estep_main_screen_split.  NULL: unsplit */
+  void *screen_split_ws;
+  size_t screen_split_bytes;
 } lakde_step_args;
 
 /* A 256-byte host-coherent, device-mapped buffer (hipHostMalloc,
@@ -525,6 +529,18 @@ int lakde_screen_knn_bound(const float *rec, const float *X, int64_t N, int64_t 
                            lakde_stream_t stream);
 int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, const void *xb,
                             const float *xnorm, const float *lbound, lakde_stream_t stream);
+/* lakde_estep_main_screen in the split form (the same outputs up to float32
+ * summation order): in UPDATE mode the block's kernels are divided into
+ * ranges of about 128, screened by separate workgroups, and each column's
+ * ranges are combined by a second kernel (normaliser update, unsafe-step
+ * test, keep tests; a third kernel does the keep pass for the columns whose
+ * keep candidates overflowed).  Other modes: as lakde_estep_main_screen
+ * without a bound.  split_ws: lakde_screen_split_workspace(N, nb) bytes,
+ * no contents needed on entry. */
+size_t lakde_screen_split_workspace(int64_t N, int64_t nb);
+int lakde_estep_main_screen_split(const lakde_estep_args *args, const void *frag, const void *xb,
+                                  const float *xnorm, void *split_ws, size_t split_bytes,
+                                  lakde_stream_t stream);
 
 /* The plain FP32 E-step on the matrix cores (17 <= D <= 63): every pair's
  * quadratic form y = Linv_n (x_m - x_n) on v_mfma_f32_32x32x2_f32 (FP32

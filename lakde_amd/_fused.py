@@ -200,13 +200,15 @@ def estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=
 
 
 def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
-          arena=None, b=None, compact_only=False, bf16=None, order=None, screen=None, m32=None):
+          arena=None, b=None, compact_only=False, bf16=None, order=None, screen=None, m32=None,
+          split=None):
     """One E-step launch sequence.  ``bf16=frag`` (the block's
     lakde_pack_frag_bf16 blocks) runs the main (quadratic form / LSE /
     ballot) kernel on the matrix cores in bf16x3; ``screen=(frag, points)``
     (the block's lakde_pack_screen blocks and the fit's ScreenPoints) runs
-    the screening E-step; ``m32=(frag, points)`` (lakde_pack_m32 blocks)
-    the plain FP32 E-step on the matrix cores."""
+    the screening E-step, in its split form with ``split`` (a uint8 tensor
+    of lakde_screen_split_workspace bytes); ``m32=(frag, points)``
+    (lakde_pack_m32 blocks) the plain FP32 E-step on the matrix cores."""
     args = estep_args(X, rec_blk, row0, mode, threshold, c, ws, c_old, old_colsum, arena, b,
                       order)
     lib = _lib.load()
@@ -217,6 +219,11 @@ def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
     if m32 is not None:
         frag, pts = m32
         check(lib.lakde_estep_main_m32(args, ptr(frag), ptr(pts.xnorm), st), "estep_main_m32")
+    elif screen is not None and split is not None:
+        frag, pts = screen
+        check(lib.lakde_estep_main_screen_split(args, ptr(frag), ptr(pts.xb), ptr(pts.xnorm),
+                                                ptr(split), split.numel(), st),
+              "estep_main_screen_split")
     elif screen is not None:
         frag, pts = screen
         lb = pts.lbound if mode == ESTEP_ACCUM else None
@@ -231,6 +238,11 @@ def estep(X, rec_blk, row0, mode, threshold, c, ws, c_old=None, old_colsum=None,
 
 def screen_frag_bytes(D):
     return int(_lib.load().lakde_screen_frag_bytes(D))
+
+
+def screen_split_workspace(N, nb, device):
+    n = int(_lib.load().lakde_screen_split_workspace(N, nb))
+    return torch.empty(max(n, 1), dtype=torch.uint8, device=device)
 
 
 class ScreenPoints:

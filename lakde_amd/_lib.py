@@ -57,6 +57,7 @@ class StepArgs(ctypes.Structure):
         ("fold_first", _int), ("bf16_frag", _vp), ("host_flags", _vp), ("ev_main", _vp * 2),
         ("ev_flags", _vp), ("flag_words", _vp), ("abort_flag", _vp),
         ("screen_xb", _vp), ("screen_xnorm", _vp), ("screen_origin", _vp), ("fp32_mfma", _int),
+        ("screen_split_ws", _vp), ("screen_split_bytes", _sz),
     ]
 
 
@@ -118,6 +119,9 @@ _SIGNATURES = {
     "lakde_pack_screen": (_int, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "lakde_screen_knn_bound": (_int, [_vp, _vp, _i64, _i64, _vp, _int, _i64, _i64, _vp, _vp]),
     "lakde_estep_main_screen": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp, _vp, _vp, _vp]),
+    "lakde_screen_split_workspace": (_sz, [_i64, _i64]),
+    "lakde_estep_main_screen_split": (_int, [ctypes.POINTER(EstepArgs), _vp, _vp, _vp, _vp, _sz,
+                                             _vp]),
     "lakde_pack_records": (_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp]),
     "lakde_pred_backward_workspace": (_sz, [_i64, _i64, _i64]),
     "lakde_pred_backward": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -474,6 +474,66 @@ int lakde_screen_knn_bound(const float *rec, const float *X, int64_t N, int64_t 
   return rc;
 }
 
+// The split UPDATE form's ranges: about 128 kernels each, at most 8.
+static void screen_split_geometry(int64_t nb, int *nsplit, int64_t *len) {
+  int64_t sp = nb >= 256 ? nb / 128 : 1;
+  if (sp > 8) sp = 8;
+  *len = ceil_div(nb, sp);
+  *nsplit = (int)ceil_div(nb, *len);
+}
+
+static size_t align256z(size_t x) { return (x + 255) / 256 * 256; }
+
+size_t lakde_screen_split_workspace(int64_t N, int64_t nb) {
+  if (N < 1 || nb < 1) return 0;
+  int S;
+  int64_t len;
+  screen_split_geometry(nb, &S, &len);
+  const size_t SN = (size_t)S * (size_t)N;
+  return 2 * align256z(SN * 4) + align256z(SN) + align256z(SN * kKeepQ * 2) +
+         align256z(SN * kKeepQ * 4) + align256z((size_t)N);
+}
+
+int lakde_estep_main_screen_split(const lakde_estep_args *args, const void *frag, const void *xb,
+                                  const float *xnorm, void *split_ws, size_t split_bytes,
+                                  lakde_stream_t stream) {
+  LAKDE_CHECK_ARG(args != nullptr && frag && xb && xnorm && split_ws, "null args");
+  LAKDE_CHECK_ARG(args->D >= 8 && args->D <= 63, "the screening form needs 8 <= D <= 63");
+  LAKDE_CHECK_ARG(args->nb <= 32767, "the screening E-step takes blocks of at most 32767 kernels");
+  LAKDE_CHECK_ARG(args->col_perm == nullptr, "the screening E-step takes columns in index order");
+  LAKDE_CHECK_ARG(split_bytes >= lakde_screen_split_workspace(args->N, args->nb),
+                  "split workspace too small");
+  EParams p;
+  int rc = estep_fill_params(*args, &p);
+  if (rc) return rc;
+  SParams sp{(const uint8_t *)frag, (const __bf16 *)xb, xnorm, p.skip_margin, nullptr};
+  if (args->mode == LAKDE_ESTEP_UPDATE) {
+    screen_split_geometry(args->nb, &sp.nsplit, &sp.split_len);
+    const size_t SN = (size_t)sp.nsplit * (size_t)args->N;
+    char *w = (char *)split_ws;
+    sp.part_m = (float *)w;
+    w += align256z(SN * 4);
+    sp.part_s = (float *)w;
+    w += align256z(SN * 4);
+    sp.kcount = (uint8_t *)w;
+    w += align256z(SN);
+    sp.kq_n = (int16_t *)w;
+    w += align256z(SN * kKeepQ * 2);
+    sp.kq_x = (float *)w;
+    w += align256z(SN * kKeepQ * 4);
+    sp.only = (uint8_t *)w;
+  }
+  hipStream_t s = as_stream(stream);
+  rc = LAKDE_EINVAL;
+  switch (args->D) {
+#define CASE(d) case d: rc = launch_estep_screen<d>(p, sp, args->mode, s); break;
+    LAKDE_FOR_EACH_D(CASE)
+#undef CASE
+  }
+  if (rc) set_error("lakde_estep_main_screen_split: launch failed");
+  return rc;
+}
+
 int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, const void *xb,
                             const float *xnorm, const float *lbound, lakde_stream_t stream) {
   LAKDE_CHECK_ARG(args != nullptr && frag && xb && xnorm, "null args");

@@ -101,9 +101,21 @@ __host__ __device__ constexpr int64_t sc_frag_bytes(int D) {
 struct SParams {
   const uint8_t *frag;   // nb fragment blocks
   const __bf16 *xb;      // N x Kp: bf16(x_m - o), zero padded
-  const float *xnorm;    // N: |x_m - o| rounded up
+  const float *xnorm;    // 2N: |x_m - o|, then the bf16 residual norms, rounded up
   float margin;          // screening margin (nats)
   const float *lbound;   // ACCUM (optional): a lower bound of each column's final normaliser
+  // split UPDATE form (optional, see launch_estep_screen): the block's
+  // kernels in nsplit ranges of split_len, one workgroup per (column group,
+  // range); per (range, column) the partial log-sum-exp, the count of keep
+  // candidates and the first kKeepQ of them
+  int64_t split_len = 0;
+  int nsplit = 0;
+  float *part_m = nullptr, *part_s = nullptr;
+  uint8_t *kcount = nullptr;
+  int16_t *kq_n = nullptr;
+  float *kq_x = nullptr;
+  // FIXED (optional): only the columns with only[m] != 0 are processed
+  uint8_t *only = nullptr;
 };
 
 // ------------------------------------------------------------- preparation
@@ -278,7 +290,7 @@ __device__ __forceinline__ float log_rho_lane(const float *rec, const float *__r
   return log_rho<D>(r, xv);
 }
 
-template <int D, int MODE>
+template <int D, int MODE, bool SPLIT = false>
 __global__ void __launch_bounds__(256, 3)
 estep_screen_kernel(EParams p, SParams sp) {
   using L = ScreenLds<D>;
@@ -304,9 +316,14 @@ estep_screen_kernel(EParams p, SParams sp) {
   const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int64_t base = (int64_t)blockIdx.x * 256 + w * kWave;  // the wave's 64 columns = one chunk
   const int64_t m = base + lane;
-  const bool valid = m < p.N;
+  // FIXED with a column filter: the other columns take no part
+  const bool valid = m < p.N && (MODE != LAKDE_ESTEP_FIXED || sp.only == nullptr || sp.only[m] != 0);
+  if (MODE == LAKDE_ESTEP_FIXED && sp.only != nullptr && !__syncthreads_or(valid)) return;
   const int64_t chunk = base / kWave;
   const int64_t nb = p.nb;
+  // this workgroup's kernels [kb_, ke_) (SPLIT: one range of the block)
+  const int64_t kb_ = SPLIT ? (int64_t)blockIdx.y * sp.split_len : 0;
+  const int64_t ke_ = SPLIT ? (kb_ + sp.split_len < nb ? kb_ + sp.split_len : nb) : nb;
   const int64_t diag_n = m - p.row0;
 
   // this lane's B fragments (both column tiles) and column norm
@@ -412,7 +429,7 @@ estep_screen_kernel(EParams p, SParams sp) {
   // entry in its wave's list | (entries << 16).
   auto pass = [&](bool keep_pass, int64_t resume, int64_t &next_resume) -> int {
     int cnt = 0;
-    next_resume = nb;
+    next_resume = ke_;
     auto consider = [&](int64_t n, float hi, float lo) {
       bool surv;
       if (keep_pass) {
@@ -422,7 +439,7 @@ estep_screen_kernel(EParams p, SParams sp) {
         // NaN bounds survive (and propagate through the exact values)
         surv = valid && n != diag_n && !(hi <= B - sp.margin);
       }
-      if (surv && n >= resume && next_resume == nb) {
+      if (surv && n >= resume && next_resume == ke_) {
         if (cnt < kScreenQ) {
           sq[cnt][tid] = (int16_t)n;
           ++cnt;
@@ -432,9 +449,9 @@ estep_screen_kernel(EParams p, SParams sp) {
       }
     };
     auto stage_kernels = [&](int64_t n0) -> int {
-      return n0 >= nb ? 0 : (nb - n0 < kScreenKps ? (int)(nb - n0) : kScreenKps);
+      return n0 >= ke_ ? 0 : (ke_ - n0 < kScreenKps ? (int)(ke_ - n0) : kScreenKps);
     };
-    screen_stage_copy<D>(sp.frag, ring0, stage_kernels(0), w, lane);
+    screen_stage_copy<D>(sp.frag + kb_ * FB, ring0, stage_kernels(kb_), w, lane);
     __syncthreads();
     // one stage: copy the next into the other array, screen this one's kernels
     auto stage = [&](int64_t n0, const uint8_t *fb, uint8_t *next) {
@@ -442,7 +459,7 @@ estep_screen_kernel(EParams p, SParams sp) {
       screen_stage_copy<D>(sp.frag + nx * FB, next, stage_kernels(nx), w, lane);
 #pragma unroll
       for (int kk = 0; kk < kScreenKps; ++kk) {
-        if (n0 + kk < nb) {
+        if (n0 + kk < ke_) {
           float hi, lo;
           bounds(fb + kk * L::kOne, n0 + kk, hi, lo);
           consider(n0 + kk, hi, lo);
@@ -450,9 +467,9 @@ estep_screen_kernel(EParams p, SParams sp) {
       }
       __syncthreads();
     };
-    for (int64_t n0 = 0; n0 < nb; n0 += 2 * kScreenKps) {
+    for (int64_t n0 = kb_; n0 < ke_; n0 += 2 * kScreenKps) {
       stage(n0, ring0, ring1);
-      if (n0 + kScreenKps < nb) stage(n0 + kScreenKps, ring1, ring0);
+      if (n0 + kScreenKps < ke_) stage(n0 + kScreenKps, ring1, ring0);
     }
     // exact phase (wave-local): compact the lanes' queues into one list,
     // one pair per lane, in kernel order per lane
@@ -498,7 +515,7 @@ estep_screen_kernel(EParams p, SParams sp) {
   bool keep_phase = MODE == LAKDE_ESTEP_FIXED;
   int64_t resume = 0;
   while (true) {
-    int64_t next = nb;
+    int64_t next = ke_;
     const int r = pass(keep_phase, resume, next);
     const int excl = r & 0xffff, cnt = r >> 16;
     const ScreenWaveList &W = wl[w];
@@ -525,11 +542,27 @@ estep_screen_kernel(EParams p, SParams sp) {
       // the largest exact value bounds the normaliser too (later passes)
       B = fmaxf(B, lse.m);
     }
-    if (__syncthreads_or(next < nb)) {
+    if (__syncthreads_or(next < ke_)) {
       resume = next;
       continue;
     }
     if (keep_phase || MODE == LAKDE_ESTEP_FIXED) return;
+    if constexpr (SPLIT) {
+      // this range's part: estep_screen_combine_kernel finishes the column
+      if (valid) {
+        const int64_t o = (int64_t)blockIdx.y * p.N + m;
+        sp.part_m[o] = lse.m;
+        sp.part_s[o] = lse.s;
+        sp.kcount[o] = (uint8_t)(kcnt < 255 ? kcnt : 255);
+        const int kk = kcnt < kKeepQ ? kcnt : kKeepQ;
+        for (int k = 0; k < kk; ++k) {
+          const int64_t q = ((int64_t)blockIdx.y * kKeepQ + k) * p.N + m;
+          sp.kq_n[q] = kq_n[k][tid];
+          sp.kq_x[q] = kq_x[k][tid];
+        }
+      }
+      return;
+    }
     // the log-sum-exp is complete: the normaliser update
     const float l = lse.value();
     if (valid) {
@@ -788,12 +821,74 @@ int launch_pred_screen(const PParams &p, const uint8_t *frag, const __bf16 *yb,
   }
 }
 
+// Completes a split UPDATE E-step per column: the ranges' partial
+// log-sum-exps merged into the new normaliser (the same update as the
+// unsplit kernel), the lazy rebalance's unsafe-step test, and the keep
+// tests of the ranges' candidates against the final normaliser.  A column
+// with more candidates in some range than that range kept is marked in
+// `only` and finished by a FIXED keep pass over all kernels instead.
+// (Templated on D only so that each translation unit has its own copy.)
+template <int D>
+__global__ void __launch_bounds__(256) estep_screen_combine_kernel(EParams p, SParams sp) {
+  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool valid = m < p.N;
+  float cm = 0.f;
+  bool over = false;
+  if (valid) {
+    const float co = p.c_old[m];
+    const float lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
+    Lse t;
+    t.init();
+    for (int sidx = 0; sidx < sp.nsplit; ++sidx)
+      t.merge(sp.part_m[sidx * p.N + m], sp.part_s[sidx * p.N + m]);
+    cm = logaddexp_f(lb, t.value());
+    p.c[m] = cm;
+    for (int sidx = 0; sidx < sp.nsplit; ++sidx) over = over || sp.kcount[sidx * p.N + m] > kKeepQ;
+    sp.only[m] = over ? 1 : 0;
+  }
+  if (p.lazy_unsafe != nullptr) lazy_unsafe_test(p, valid, m, valid ? p.c_old[m] : 0.f, cm);
+  if (!valid || over) return;
+  const int64_t chunk = m / kWave;
+  const int64_t cseg = chunk / p.seg_len;
+  for (int sidx = 0; sidx < sp.nsplit; ++sidx) {
+    const int kc = sp.kcount[sidx * p.N + m];
+    for (int k = 0; k < kc; ++k) {
+      const int64_t q = ((int64_t)sidx * kKeepQ + k) * p.N + m;
+      const int64_t n = sp.kq_n[q];
+      const float d = __fsub_rn(sp.kq_x[q], cm);
+      float r;
+      if (keep_resp(d, p.threshold, p.log_t_lo, &r)) {
+        atomicOr((unsigned long long *)&p.masks[chunk * p.nb + n], 1ull << lane);
+        atomicAdd(&p.seg_tot[cseg * p.nb + n], 1);
+        atomicAdd(&p.row_tot[n], 1);
+      }
+    }
+  }
+}
+
+// The split UPDATE form (sp.nsplit > 1; the block's kernels in nsplit
+// ranges, one workgroup per (column group, range)): the ranges' screens,
+// the per-column combine, and a FIXED keep pass over all kernels for the
+// columns whose keep candidates overflowed (its workgroups without such
+// columns return at once).  A launch of ceil(N / 256) workgroups of the
+// unsplit kernel is about five waves of workgroups at three per CU, and the
+// last, partial wave costs as much as a full one; with the kernels split in
+// ranges the launch is many more, shorter workgroups.
 template <int D>
 int launch_estep_screen(const EParams &p, const SParams &sp, int mode, hipStream_t s) {
   if constexpr (D < 8) {
     return LAKDE_EINVAL;
   } else {
     const dim3 grid((unsigned)ceil_div(p.N, 256));
+    if (mode == LAKDE_ESTEP_UPDATE && sp.nsplit > 1) {
+      hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_UPDATE, true>),
+                         dim3(grid.x, (unsigned)sp.nsplit), dim3(256), 0, s, p, sp);
+      hipLaunchKernelGGL((estep_screen_combine_kernel<D>), grid, dim3(256), 0, s, p, sp);
+      hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_FIXED>), grid, dim3(256), 0, s, p,
+                         sp);
+      return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
+    }
     if (mode == LAKDE_ESTEP_ACCUM)
       hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_ACCUM>), grid, dim3(256), 0, s, p, sp);
     else if (mode == LAKDE_ESTEP_FIXED)

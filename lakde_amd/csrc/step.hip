@@ -194,6 +194,9 @@ int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
   if (a->ev_main[0]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[0], s));
   if (a->bf16_frag != nullptr && a->fp32_mfma)
     rc = lakde_estep_main_m32(&em, a->bf16_frag, a->screen_xnorm, s);
+  else if (a->bf16_frag != nullptr && a->screen_xb != nullptr && a->screen_split_ws != nullptr)
+    rc = lakde_estep_main_screen_split(&em, a->bf16_frag, a->screen_xb, a->screen_xnorm,
+                                       a->screen_split_ws, a->screen_split_bytes, s);
   else if (a->bf16_frag != nullptr && a->screen_xb != nullptr)
     rc = lakde_estep_main_screen(&em, a->bf16_frag, a->screen_xb, a->screen_xnorm, nullptr, s);
   else if (a->bf16_frag != nullptr)

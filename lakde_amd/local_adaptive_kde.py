@@ -57,6 +57,10 @@ class LocallyAdaptiveKDE(AbstractKDE):
     # off only logarithmically, so about a tenth of the pairs need the
     # exact term)
     pred_screen = False
+    # precision form: the E-step's UPDATE launches in the split form (the
+    # block's kernels in ranges screened by separate workgroups, combined
+    # per column)
+    screen_split = True
     _screen = None
 
     def __init__(self, nu_0, a_0, b_0, block_size, verbose, logs, precision="fp32",
@@ -164,6 +168,11 @@ class LocallyAdaptiveKDE(AbstractKDE):
             self._fb = F.screen_frag_bytes(D)
             self._m32_fb = F.m32_frag_bytes(D) if D >= 17 else 0
             self._frag_blk = torch.empty(bs * max(self._fb, self._m32_fb), dtype=torch.uint8,
+                                         device=dev)
+            # the screening E-step's split form (UPDATE steps): the block's
+            # kernels in ranges screened by separate workgroups
+            self._split_bytes = int(_lib.load().lakde_screen_split_workspace(N, bs))
+            self._split_ws = torch.empty(max(self._split_bytes, 1), dtype=torch.uint8,
                                          device=dev)
 
     def _init_arena(self, X, graph, sparsity_threshold):
@@ -378,6 +387,9 @@ class LocallyAdaptiveKDE(AbstractKDE):
             a.fp32_mfma = 1 if m32 else 0
             a.screen_xnorm = sc.xnorm.data_ptr() if sc is not None else None
             a.screen_origin = sc.origin.data_ptr() if sc is not None else None
+            split = bf16 and self.screen_split
+            a.screen_split_ws = self._split_ws.data_ptr() if split else None
+            a.screen_split_bytes = self._split_bytes if split else 0
             ev = self._estep_timer.pair() if self._estep_timer is not None else None
             if ev is not None:
                 a.ev_main[0], a.ev_main[1] = ev[0].cuda_event, ev[1].cuda_event
@@ -414,13 +426,15 @@ class LocallyAdaptiveKDE(AbstractKDE):
                 old = self._step_cache["arr"][b]
                 keep = (old.estep.threshold, old.post.prior, old.bf16_frag, old.ev_main[0],
                         old.ev_main[1], old.fold_first)
-                keep_sc = (old.screen_xb, old.screen_xnorm, old.screen_origin, old.fp32_mfma)
+                keep_sc = (old.screen_xb, old.screen_xnorm, old.screen_origin, old.fp32_mfma,
+                           old.screen_split_ws, old.screen_split_bytes)
                 arena.grow(int(self._flags[b]))
                 self._step_cache = {}
                 for j in range(b, end):
                     a = self._step_args(X, j)
                     a.estep.threshold, a.post.prior, a.bf16_frag = keep[0], keep[1], keep[2]
-                    a.screen_xb, a.screen_xnorm, a.screen_origin, a.fp32_mfma = keep_sc
+                    (a.screen_xb, a.screen_xnorm, a.screen_origin, a.fp32_mfma,
+                     a.screen_split_ws, a.screen_split_bytes) = keep_sc
                     if j == b:
                         a.ev_main[0], a.ev_main[1], a.fold_first = keep[3], keep[4], keep[5]
                 a = self._step_cache["arr"][b]

@@ -306,3 +306,72 @@ def test_screen_predictive_matches_fp32(name, n, m, k):
     bar = 1e-4 * np.maximum(1.0, np.abs(ref))
     assert np.all(np.abs(outsc[:ref.size] - ref) <= bar)
     assert np.all(np.abs(out32[:ref.size] - ref) <= bar)
+
+
+def _csr_of(arena, nb):
+    cnt = int(arena.nnz_dev[0])
+    assert cnt <= arena.cap
+    return (arena.rowptr[0, :nb + 1].cpu().numpy(), arena.row[0, :cnt].cpu().numpy(),
+            arena.col[0, :cnt].cpu().numpy(), arena.val[0, :cnt].cpu().numpy())
+
+
+@pytest.mark.parametrize("name,n,k,t", [("mix128_d63", 3000, 96, 1 / 96),
+                                        ("mix64_d21", 3000, 32, 1 / 32),
+                                        ("normal8", 2500, 64, 0.0)])
+def test_screen_split_update_matches_unsplit(name, n, k, t):
+    """The split form of the screening UPDATE E-step (the block's kernels in
+    ranges of ~128 screened by separate workgroups, then a per-column
+    combine and a keep pass for overflowing columns) against the unsplit
+    kernel from the same fitted state: normalisers within float32
+    summation-order differences (1e-6 relative), the kept entries identical
+    up to values within 1e-5 of the threshold (the normalisers differ in
+    their last bits) and their values within the normalisers' difference
+    (r = exp(x - c)) plus 4 (|c| + |x - c|) 2^-23 relative.  normal8
+    with threshold 0 overflows the keep queues, so the keep pass runs."""
+    from lakde_amd import _fused as F
+    from lakde_amd._lib import ESTEP_UPDATE
+    X, Y = _screen_case(name, n, k)
+    graph = O.compute_knn_graph(X, k)
+    model, _ = _fit(X, Y, graph, "fp32", 512, t if t else 1 / k, 1)
+    model.materialize()
+    Xd = _t(X)
+    N, D = X.shape
+    pts = F.ScreenPoints(Xd)
+    split = F.screen_split_workspace(N, 512, Xd.device)
+    for b in (0, (N - 1) // 512):
+        s, e = b * 512, min(N, b * 512 + 512)
+        nb = e - s
+        rec = model.rec[s:e]
+        frag = torch.empty(nb * F.screen_frag_bytes(D), dtype=torch.uint8, device=DEV)
+        F.pack_screen(rec, D, pts.origin, frag)
+        ws = F.EstepWorkspace(N, nb, Xd.device)
+        ind, val = model.arena.block(b)
+        old = torch.zeros(N, device=DEV)
+        old.index_add_(0, ind[1].long(), val)
+        res = []
+        for sp in (None, split):
+            c = torch.empty(N, device=DEV)
+            arena = F.Arena(1, nb, 1 << 22, Xd.device)
+            F.estep(Xd, rec, s, ESTEP_UPDATE, t, c, ws, c_old=model.rnm_log_consts,
+                    old_colsum=old, arena=arena, b=0, screen=(frag, pts), split=sp)
+            res.append((c.cpu().numpy(), _csr_of(arena, nb)))
+        np.testing.assert_allclose(res[1][0], res[0][0], rtol=1e-6, atol=1e-5)
+        (pa, ra, ca, va), (pb, rb, cb, vb) = res[0][1], res[1][1]
+        cmax = float(np.abs(res[0][0]).max())
+        ea = dict(zip((ra.astype(np.int64) << 32 | ca).tolist(), va.tolist()))
+        eb = dict(zip((rb.astype(np.int64) << 32 | cb).tolist(), vb.tolist()))
+        flips = set(ea) ^ set(eb)
+        for key in flips:
+            v = ea.get(key, eb.get(key))
+            assert abs(v - t) <= 1e-5 * max(t, 1e-30), (name, b, key, v)
+        common = sorted(set(ea) & set(eb))
+        a = np.array([ea[q] for q in common])
+        bb = np.array([eb[q] for q in common])
+        # r = exp(x - c): the values move by the normalisers' own difference,
+        # and d = x - c is rounded to float32 (an ulp of |d| <= -log r_min)
+        dc = float(np.abs(res[1][0] - res[0][0]).max())
+        dmax = float(-np.log(max(a.min(), 1e-45)))
+        np.testing.assert_allclose(bb, a, rtol=1.5 * dc + 4 * (cmax + dmax) * 2.0 ** -23 + 1e-7,
+                                   atol=0)
+        print(f"[{name}] block {b}: nb={nb}, {len(ea)} kept entries, {len(flips)} at the "
+              f"threshold", flush=True)
