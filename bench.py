@@ -43,11 +43,12 @@ SECONDARY = "mix32_d6"
 # counters read for a plain copy of known size; valid for the kernel source
 # whose sha256 is recorded (the line reports null for any other source).
 MEASURED_TRAFFIC = {
-    "kernel": "estep_screen_kernel<63, 2>",
-    "source_sha256": "614dd0767a3e7df85ff520835ed030686a78d96b5d80550dbbe9de24ff38647c",
-    "fetch_bytes": 3918575000, "write_bytes": 4144000,
-    "profile": "profiles/estep_screen_mix128_d63.md: FETCH_SIZE 1,913,366.9 KiB x 2.0000 + "
-               "WRITE_SIZE 4,046.9 KiB x 1.0000 per dispatch (copy-probe scale factors)",
+    "kernel": "estep_screen_kernel<63, 2, true> + estep_screen_combine_kernel<63>",
+    "source_sha256": "728c4f31a425063f0cd586ae4899d5cfb09ba1feb4f1565fb36bf04a302591d2",
+    "fetch_bytes": 1566687000, "write_bytes": 441350000,
+    "profile": "profiles/estep_screen_mix128_d63.md: FETCH_SIZE (740,788.4 + 24,210.6) KiB x "
+               "1.99996 + WRITE_SIZE (425,982.0 + 5,023.6) KiB x 1.0000 per E-step (copy-probe "
+               "scale factors; the keep pass, which reads 1 MB of column marks, not counted)",
 }
 
 
