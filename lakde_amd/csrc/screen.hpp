@@ -4,8 +4,10 @@
 // matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulate) in the centre
 // form
 //     y~ = bf16(U_n) bf16(x_m - o) - c_n,   c_n = U_n (x_n - o) in FP32,
-// with U_n = Linv_n, o a fixed origin (the data mean), c_n loaded as the
-// accumulators' initial value and bf16(x_m - o) prepared once per fit.
+// with U_n = Linv_n, o a fixed origin (the data mean) and bf16(x_m - o)
+// prepared once per fit.  When D is not a multiple of 16, -c_n enters as
+// one more MFMA column, bf16(-c_n) against a constant coordinate 1 of the
+// points; otherwise it is the accumulators' initial value.
 // Write v = fl(x_m - o) = (x_m - o)(1 + eps) (|eps| <= 2^-24), bf16(v) =
 // v - r and bf16(U) = U - dU, with the residuals r and dU exact in FP32.  The
 // error of y~ is -U r - dU v + dU r + U (x_m - o) eps, plus the FP32
@@ -17,7 +19,9 @@
 // b_n = |x_n - o| (all rounded up), P_n = F_n (1 + u) (U r and dU r, with
 // dF_n <= u F_n), Q_n = dF_n (1 + 2^-24) + 67 * 2^-24 F_n (dU v, the
 // accumulation and eps) and C_n = kb F_n b_n (kb = 1.2e-5 >= 65 * 2^-24
-// (1 + u)^2, the rounding of c_n).  The actual residuals are about 0.4 u of
+// (1 + u)^2, the rounding of c_n), plus |c_n - bf16(c_n)|_2, computed
+// exactly when c_n is packed, in the form with the MFMA column.  The
+// actual residuals are about 0.4 u of
 // their operands' norms on average, against the worst case u of the bound
 // (2u + u^2) F_n a_m that the form without them would need.  Hence the exact
 // log rho = const_n - nu_n/2 Q lies between
@@ -68,10 +72,18 @@ constexpr int kKeepQ = 4;      // keep-candidate queue entries per lane
 
 __host__ __device__ constexpr int sc_kp(int D) { return (D + 15) / 16 * 16; }
 __host__ __device__ constexpr int sc_dp(int D) { return (D + 31) / 32 * 32; }
+// The operands hold the coordinates in reverse order (operand index
+// j' = D - 1 - j), so that the factor U (lower triangular) becomes upper
+// triangular: row i' has nonzeros in the columns j' >= i' only.  With a
+// padding column (D < Kp) the centre term rides in the MFMA itself: the
+// points carry the constant 1 in column D and the A tiles carry bf16(-c_n)
+// there, which lies in the last K-step that every row tile uses, so the
+// accumulators start at zero and no tile is added.
+__host__ __device__ constexpr bool sc_aug(int D) { return D < sc_kp(D); }
 // MFMA A tiles of U (32 rows x 16 columns) that are not all zero: K-step s
-// of row tile rt holds nonzeros iff 16 s <= 32 rt + 31 (U is lower
-// triangular); stored K-step-major in that order
-__host__ __device__ constexpr bool sc_tile_used(int rt, int s) { return s * 16 <= rt * 32 + 31; }
+// of row tile rt holds nonzeros iff 16 s + 15 >= 32 rt; stored K-step-major
+// in that order
+__host__ __device__ constexpr bool sc_tile_used(int rt, int s) { return s * 16 + 15 >= rt * 32; }
 __host__ __device__ constexpr int sc_ntiles(int D) {
   int t = 0;
   for (int s = 0; s < sc_kp(D) / 16; ++s)
@@ -89,12 +101,13 @@ __host__ __device__ constexpr int sc_tile_index(int D, int rt_, int s_) {
 }
 // fragment block per kernel, laid out as its LDS image: the used A tiles of
 // bf16(U) (1 KB each: lane l's 8 bf16 operand values at byte 16 l, row
-// 32 rt + l % 32, columns 16 s + 8 (l / 32) + [0, 8)), then -c_n [Dp] f32,
+// 32 rt + l % 32, columns 16 s + 8 (l / 32) + [0, 8), reversed coordinates),
 // then scal[8]: P_n, Q_n, C_n (the error bound), E-step const, E-step
-// exponent (-nu/2), three unused
+// exponent (-nu/2), three unused; then, without a padding column, -c_n [Dp]
+// f32 in the accumulators' row order
 constexpr int kScreenScal = 8;
 __host__ __device__ constexpr int64_t sc_frag_bytes(int D) {
-  return (int64_t)sc_ntiles(D) * 1024 + (int64_t)sc_dp(D) * 4 + 4 * kScreenScal;
+  return (int64_t)sc_ntiles(D) * 1024 + 4 * kScreenScal + (sc_aug(D) ? 0 : (int64_t)sc_dp(D) * 4);
 }
 
 // Parameters of the screening launches beyond EParams.
@@ -119,9 +132,10 @@ struct SParams {
 };
 
 // ------------------------------------------------------------- preparation
-// bf16(x_m - o) (zero padded to Kp) per point; xnorm[m] = |x_m - o| and
-// xnorm[N + m] = |fl(x_m - o) - bf16(x_m - o)|, the norm of the rounding
-// residual, both rounded up.
+// bf16(x_m - o) per point in reversed coordinates, the constant 1 in column
+// D when there is a padding column, zero padded to Kp; xnorm[m] = |x_m - o|
+// and xnorm[N + m] = |fl(x_m - o) - bf16(x_m - o)|, the norm of the
+// rounding residual, both rounded up.
 template <int D>
 __global__ void __launch_bounds__(256) screen_points_kernel(const float *__restrict__ X, int64_t N,
                                                            const float *__restrict__ origin,
@@ -133,9 +147,9 @@ __global__ void __launch_bounds__(256) screen_points_kernel(const float *__restr
   double s = 0.0, rs = 0.0;
 #pragma unroll
   for (int j = 0; j < Kp; ++j) {
-    float v = 0.f;
+    float v = j == D ? 1.f : 0.f;  // the constant coordinate is exact
     if (j < D) {
-      v = X[m * D + j] - origin[j];
+      v = X[m * D + (D - 1 - j)] - origin[D - 1 - j];
       s += (double)v * (double)v;
     }
     const __bf16 vb = (__bf16)v;
@@ -160,22 +174,11 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
   uint8_t *f = frag + n * sc_frag_bytes(D);
   constexpr int NT = sc_ntiles(D);
   __bf16 *uh = reinterpret_cast<__bf16 *>(f);
-  float *cneg = reinterpret_cast<float *>(f + NT * 1024);
-  float *scal = cneg + Dp;
-  for (int s = 0; s < Kp / 16; ++s)
-    for (int rt = 0; rt < Dp / 32; ++rt) {
-      if (!sc_tile_used(rt, s)) continue;
-      __bf16 *tile = uh + sc_tile_index(D, rt, s) * 512 + lane * 8;
-      const int i = rt * 32 + (lane & 31);
-      for (int e = 0; e < 8; ++e) {
-        const int j = s * 16 + 8 * (lane >> 5) + e;
-        const float v = (i < D && j <= i) ? r[i * (i + 1) / 2 + j] : 0.f;
-        tile[e] = (__bf16)v;
-      }
-    }
+  float *scal = reinterpret_cast<float *>(f + NT * 1024);
+  float *cneg = scal + kScreenScal;  // without a padding column only
   // c_n row by row (lane = row), |U|_F^2, |U - bf16(U)|_F^2 and |x_n - o|^2
   // in double
-  double f2 = 0.0, d2 = 0.0, b2 = 0.0;
+  double f2 = 0.0, d2 = 0.0, b2 = 0.0, e2 = 0.0;
   float c = 0.f;
   if (lane < D) {
     const float *row = r + lane * (lane + 1) / 2;
@@ -188,17 +191,39 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
     }
     const float dl = r[rec_x(D) + lane] - origin[lane];
     b2 = (double)dl * (double)dl;
+    // the rounding of -c_n to bf16 in the padding column
+    const double ec = (double)(-c) - (double)(float)(__bf16)(-c);
+    e2 = sc_aug(D) ? ec * ec : 0.0;
   }
-  if (lane < Dp) cneg[lane] = lane < D ? -c : 0.f;
+  // operand row i' is row D - 1 - i' of U
+  for (int s = 0; s < Kp / 16; ++s)
+    for (int rt = 0; rt < Dp / 32; ++rt) {
+      if (!sc_tile_used(rt, s)) continue;
+      __bf16 *tile = uh + sc_tile_index(D, rt, s) * 512 + lane * 8;
+      const int ip = rt * 32 + (lane & 31), i = D - 1 - ip;
+      const float ci = __shfl(c, ip < D ? i : 0, kWave);
+      for (int e = 0; e < 8; ++e) {
+        const int jp = s * 16 + 8 * (lane >> 5) + e, j = D - 1 - jp;
+        float v = 0.f;
+        if (ip < D && jp < D && jp >= ip) v = r[i * (i + 1) / 2 + j];
+        if (sc_aug(D) && ip < D && jp == D) v = -ci;
+        tile[e] = (__bf16)v;
+      }
+    }
+  if (!sc_aug(D) && lane < Dp) {
+    const float ci = __shfl(c, lane < D ? D - 1 - lane : 0, kWave);
+    cneg[lane] = lane < D ? -ci : 0.f;
+  }
   f2 = wave_sum(f2);
   d2 = wave_sum(d2);
   b2 = wave_sum(b2);
+  e2 = wave_sum(e2);
   if (lane == 0) {
     const double up = 1.0 + 1e-6;
     const double F = sqrt(f2) * up, dF = sqrt(d2) * up, b = sqrt(b2) * up;
     scal[0] = (float)(F * (1.0 + kScreenU) * up);
     scal[1] = (float)((dF * (1.0 + 0x1p-24) + kScreenAcc * F) * up);
-    scal[2] = (float)((double)kScreenKb * F * b * up);
+    scal[2] = (float)(((double)kScreenKb * F * b + sqrt(e2)) * up * up);
     scal[3] = r[rec_c(D) + 0];
     scal[4] = r[rec_c(D) + 1];
     scal[5] = 0.f;
@@ -268,6 +293,34 @@ __device__ __forceinline__ void screen_stage_copy(const uint8_t *__restrict__ sr
     const int off = j * 1024 + lane * 16;
     if (j * 1024 < bytes && off < bytes)
       __builtin_amdgcn_global_load_lds(src + off, (sc_lds_ptr)(dst + j * 1024), 16, 0, 0);
+  }
+}
+
+// initial accumulators of one kernel's two column tiles: zero when the
+// centre term rides in the padding column, else -c_n (cneg in the LDS
+// image, indexed by operand row)
+template <int D>
+__device__ __forceinline__ void sc_init_acc(sc_f32x16 (&acc)[sc_dp(D) / 32][2],
+                                            const float *cneg, int lane) {
+#pragma unroll
+  for (int rt = 0; rt < sc_dp(D) / 32; ++rt) {
+    sc_f32x16 ci;
+    if constexpr (sc_aug(D)) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) ci[r] = 0.f;
+    } else {
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const sc_f32x4 v =
+            *reinterpret_cast<const sc_f32x4 *>(cneg + rt * 32 + 8 * q4 + 4 * (lane >> 5));
+        ci[4 * q4 + 0] = v[0];
+        ci[4 * q4 + 1] = v[1];
+        ci[4 * q4 + 2] = v[2];
+        ci[4 * q4 + 3] = v[3];
+      }
+    }
+    acc[rt][0] = ci;
+    acc[rt][1] = ci;
   }
 }
 
@@ -361,29 +414,14 @@ estep_screen_kernel(EParams p, SParams sp) {
   // bounds [lo, hi] of the exact log rho of kernel n (LDS image fb) at this
   // lane's column; -inf for the excluded diagonal and invalid columns
   auto bounds = [&](const uint8_t *fb, int64_t n, float &hi, float &lo) {
-    const float *cneg = reinterpret_cast<const float *>(fb + sc_ntiles(D) * 1024);
-    const float *scal = cneg + sc_dp(D);
+    const float *scal = reinterpret_cast<const float *>(fb + sc_ntiles(D) * 1024);
     sc_f32x16 acc[RT][2];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-      sc_f32x16 ci;
-#pragma unroll
-      for (int q4 = 0; q4 < 4; ++q4) {
-        const sc_f32x4 v =
-            *reinterpret_cast<const sc_f32x4 *>(cneg + rt * 32 + 8 * q4 + 4 * (lane >> 5));
-        ci[4 * q4 + 0] = v[0];
-        ci[4 * q4 + 1] = v[1];
-        ci[4 * q4 + 2] = v[2];
-        ci[4 * q4 + 3] = v[3];
-      }
-      acc[rt][0] = ci;
-      acc[rt][1] = ci;
-    }
+    sc_init_acc<D>(acc, scal + kScreenScal, lane);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
-        if (!sc_tile_used(rt, s)) continue;  // U is lower triangular
+        if (!sc_tile_used(rt, s)) continue;  // upper triangular in operand order
         const sc_bf16x8 a = *reinterpret_cast<const sc_bf16x8 *>(
             fb + sc_tile_index(D, rt, s) * 1024 + lane * 16);
         acc[rt][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[0][s], acc[rt][0], 0, 0, 0);
@@ -673,26 +711,11 @@ pred_screen_kernel(PParams p, const uint8_t *__restrict__ frag, const __bf16 *__
   // bounds [lo, hi] of the exact term of kernel n (LDS image fb) at this
   // lane's test point
   auto bounds = [&](const uint8_t *fb, int64_t n, float &hi, float &lo) {
-    const float *cneg = reinterpret_cast<const float *>(fb + sc_ntiles(D) * 1024);
-    const float *scal = cneg + sc_dp(D);
+    const float *scal = reinterpret_cast<const float *>(fb + sc_ntiles(D) * 1024);
     const float *rc = p.rec + n * R + rec_c(D);
     const float cst = rc[2], h = rc[3];
     sc_f32x16 acc[RT][2];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-      sc_f32x16 ci;
-#pragma unroll
-      for (int q4 = 0; q4 < 4; ++q4) {
-        const sc_f32x4 v =
-            *reinterpret_cast<const sc_f32x4 *>(cneg + rt * 32 + 8 * q4 + 4 * (lane >> 5));
-        ci[4 * q4 + 0] = v[0];
-        ci[4 * q4 + 1] = v[1];
-        ci[4 * q4 + 2] = v[2];
-        ci[4 * q4 + 3] = v[3];
-      }
-      acc[rt][0] = ci;
-      acc[rt][1] = ci;
-    }
+    sc_init_acc<D>(acc, scal + kScreenScal, lane);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
