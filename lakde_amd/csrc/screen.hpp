@@ -102,9 +102,10 @@ __host__ __device__ constexpr int sc_tile_index(int D, int rt_, int s_) {
 // fragment block per kernel, laid out as its LDS image: the used A tiles of
 // bf16(U) (1 KB each: lane l's 8 bf16 operand values at byte 16 l, row
 // 32 rt + l % 32, columns 16 s + 8 (l / 32) + [0, 8), reversed coordinates),
-// then scal[8]: P_n, Q_n, C_n (the error bound), E-step const, E-step
-// exponent (-nu/2), three unused; then, without a padding column, -c_n [Dp]
-// f32 in the accumulators' row order
+// then scal[8]: P_n, Q_n, C_n (the error bound), the E-step constant and
+// exponent (-nu/2) widened for the upper bound, then for the lower bound,
+// one unused; then, without a padding column, -c_n [Dp] f32 indexed by
+// operand row
 constexpr int kScreenScal = 8;
 __host__ __device__ constexpr int64_t sc_frag_bytes(int D) {
   return (int64_t)sc_ntiles(D) * 1024 + 4 * kScreenScal + (sc_aug(D) ? 0 : (int64_t)sc_dp(D) * 4);
@@ -224,10 +225,15 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
     scal[0] = (float)(F * (1.0 + kScreenU) * up);
     scal[1] = (float)((dF * (1.0 + 0x1p-24) + kScreenAcc * F) * up);
     scal[2] = (float)(((double)kScreenKb * F * b + sqrt(e2)) * up * up);
-    scal[3] = r[rec_c(D) + 0];
-    scal[4] = r[rec_c(D) + 1];
-    scal[5] = 0.f;
-    scal[6] = 0.f;
+    // log rho = cst + expo Q; hi = cst_hi + expo_hi Q_lo and lo = cst_lo +
+    // expo_lo Q_hi in FP32, with the slack 0.01 + 1e-4 |cst| for the
+    // rounding of the products, of the exact term and of these constants
+    const double cst = r[rec_c(D) + 0], expo = r[rec_c(D) + 1];
+    const double slack = 0.01 + 1e-4 * fabs(cst);
+    scal[3] = (float)(cst + slack);
+    scal[4] = (float)(expo * (1.0 - 1e-5));
+    scal[5] = (float)(cst - slack);
+    scal[6] = (float)(expo * (1.0 + 1e-5));
     scal[7] = 0.f;
   }
 }
@@ -446,13 +452,12 @@ estep_screen_kernel(EParams p, SParams sp) {
     const float E = fmaf(scal[0], rm, fmaf(scal[1], am, scal[2]));
     // v_sqrt_f32 (within 1 ulp; the 4e-6 factors below cover it)
     const float sqt = __builtin_amdgcn_sqrtf(qt);
-    const float rl = fmaf(sqt, 1.f - 4e-6f, -E);
-    const float qlb = rl > 0.f ? rl * rl * (1.f - 1e-5f) : 0.f;
+    // the slack and the 1 -+ 1e-5 factors (the rounding of the squares and
+    // of the products) are in the kernel's constants
+    const float rl = fmaxf(fmaf(sqt, 1.f - 4e-6f, -E), 0.f);
     const float rh = fmaf(sqt, 1.f + 4e-6f, E);
-    const float cst = scal[3], expo = scal[4];
-    const float slack = fmaf(1e-4f, fabsf(cst), 0.01f);
-    hi = fmaf(expo, qlb, cst) + slack;
-    lo = fmaf(expo, rh * rh * (1.f + 1e-5f), cst) - slack;
+    hi = fmaf(scal[4], rl * rl, scal[3]);
+    lo = fmaf(scal[6], rh * rh, scal[5]);
     if (!valid || n == diag_n) {
       hi = neg_inf();
       lo = neg_inf();
