@@ -531,10 +531,11 @@ int lakde_estep_main_screen(const lakde_estep_args *args, const void *frag, cons
                             const float *xnorm, const float *lbound, lakde_stream_t stream);
 /* lakde_estep_main_screen in the split form (the same outputs up to float32
  * summation order): in UPDATE mode the block's kernels are divided into
- * ranges of about 128, screened by separate workgroups, and each column's
- * ranges are combined by a second kernel (normaliser update, unsafe-step
- * test, keep tests; a third kernel does the keep pass for the columns whose
- * keep candidates overflowed).  Other modes: as lakde_estep_main_screen
+ * ranges of about 128, screened by separate workgroups that queue their
+ * surviving pairs, and each column's ranges are combined by a second kernel
+ * (the exact FP32 values of the queued pairs of 64 columns in one list,
+ * normaliser update, unsafe-step test, keep tests; a third kernel does the
+ * keep pass for the columns whose keep candidates overflowed).  Other modes: as lakde_estep_main_screen
  * without a bound.  split_ws: lakde_screen_split_workspace(N, nb) bytes,
  * no contents needed on entry. */
 size_t lakde_screen_split_workspace(int64_t N, int64_t nb);

@@ -491,7 +491,9 @@ size_t lakde_screen_split_workspace(int64_t N, int64_t nb) {
   screen_split_geometry(nb, &S, &len);
   const size_t SN = (size_t)S * (size_t)N;
   return 2 * align256z(SN * 4) + align256z(SN) + align256z(SN * kKeepQ * 2) +
-         align256z(SN * kKeepQ * 4) + align256z((size_t)N);
+         align256z(SN * kKeepQ * 4) + align256z((size_t)N) + align256z(SN) +
+         align256z(SN * kScreenQ * 4) + align256z((size_t)nb * 4) +
+         2 * align256z((size_t)nb * kScreenListCap * 4);
 }
 
 int lakde_estep_main_screen_split(const lakde_estep_args *args, const void *frag, const void *xb,
@@ -522,6 +524,17 @@ int lakde_estep_main_screen_split(const lakde_estep_args *args, const void *frag
     sp.kq_x = (float *)w;
     w += align256z(SN * kKeepQ * 4);
     sp.only = (uint8_t *)w;
+    w += align256z((size_t)args->N);
+    sp.sv_cnt = (uint8_t *)w;
+    w += align256z(SN);
+    sp.sv_e = (int32_t *)w;
+    w += align256z(SN * kScreenQ * 4);
+    sp.kl_cnt = (int32_t *)w;
+    w += align256z((size_t)args->nb * 4);
+    sp.kl_m = (int32_t *)w;
+    w += align256z((size_t)args->nb * kScreenListCap * 4);
+    sp.kl_x = (float *)w;
+    sp.kl_cap = kScreenListCap;
   }
   hipStream_t s = as_stream(stream);
   rc = LAKDE_EINVAL;

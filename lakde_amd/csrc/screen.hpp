@@ -130,6 +130,19 @@ struct SParams {
   float *kq_x = nullptr;
   // FIXED (optional): only the columns with only[m] != 0 are processed
   uint8_t *only = nullptr;
+  // split UPDATE form, survivors left to the later kernels: per (range,
+  // column) sv_cnt[r N + m] of them (0 where the range finished them
+  // itself), entry k at sv_e[(r kScreenQ + k) N + m] = (slot + 1) << 15 |
+  // kernel, slot = the pair's place in the kernel's list (-1: the list was
+  // full); per kernel n of the block kl_cnt[n] pairs claimed, the columns of
+  // the first kl_cap at kl_m[n kl_cap + slot] and their exact values at
+  // kl_x[n kl_cap + slot]
+  uint8_t *sv_cnt = nullptr;
+  int32_t *sv_e = nullptr;
+  int32_t *kl_cnt = nullptr;
+  int32_t *kl_m = nullptr;
+  float *kl_x = nullptr;
+  int kl_cap = 0;
 };
 
 // ------------------------------------------------------------- preparation
@@ -514,6 +527,38 @@ estep_screen_kernel(EParams p, SParams sp) {
       stage(n0, ring0, ring1);
       if (n0 + kScreenKps < ke_) stage(n0 + kScreenKps, ring1, ring0);
     }
+    if constexpr (SPLIT) {
+      // the usual case: no queue of the workgroup filled up, so the range's
+      // survivors are all queued; each is appended to its kernel's list,
+      // evaluated there by estep_screen_exact_kernel with the kernel's
+      // record read once per wave, and combined per column later
+      if (!keep_pass && resume == 0 && !__syncthreads_or(next_resume < ke_)) {
+        if (valid) {
+          const int64_t o = (int64_t)blockIdx.y * p.N + m;
+          sp.part_m[o] = neg_inf();
+          sp.part_s[o] = 0.f;
+          sp.kcount[o] = 0;
+          sp.sv_cnt[o] = (uint8_t)cnt;
+          // four list slots claimed at a time (their atomics in flight together)
+          for (int k0 = 0; k0 < cnt; k0 += 4) {
+            int32_t slot[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (k0 + i < cnt) slot[i] = atomicAdd(&sp.kl_cnt[sq[k0 + i][tid]], 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (k0 + i < cnt) {
+                const int32_t n = sq[k0 + i][tid];
+                const bool in = slot[i] < sp.kl_cap;
+                if (in) sp.kl_m[(int64_t)n * sp.kl_cap + slot[i]] = (int32_t)m;
+                sp.sv_e[((int64_t)blockIdx.y * kScreenQ + k0 + i) * p.N + m] =
+                    ((in ? slot[i] + 1 : 0) << 15) | n;
+              }
+          }
+        }
+        return -1;
+      }
+    }
     // exact phase (wave-local): compact the lanes' queues into one list,
     // one pair per lane, in kernel order per lane
     int incl = cnt;
@@ -560,6 +605,7 @@ estep_screen_kernel(EParams p, SParams sp) {
   while (true) {
     int64_t next = ke_;
     const int r = pass(keep_phase, resume, next);
+    if (SPLIT && r < 0) return;  // deferred to the combine kernel
     const int excl = r & 0xffff, cnt = r >> 16;
     const ScreenWaveList &W = wl[w];
     if (keep_phase) {
@@ -597,6 +643,7 @@ estep_screen_kernel(EParams p, SParams sp) {
         sp.part_m[o] = lse.m;
         sp.part_s[o] = lse.s;
         sp.kcount[o] = (uint8_t)(kcnt < 255 ? kcnt : 255);
+        sp.sv_cnt[o] = 0;
         const int kk = kcnt < kKeepQ ? kcnt : kKeepQ;
         for (int k = 0; k < kk; ++k) {
           const int64_t q = ((int64_t)blockIdx.y * kKeepQ + k) * p.N + m;
@@ -849,48 +896,175 @@ int launch_pred_screen(const PParams &p, const uint8_t *frag, const __bf16 *yb,
   }
 }
 
-// Completes a split UPDATE E-step per column: the ranges' partial
-// log-sum-exps merged into the new normaliser (the same update as the
-// unsplit kernel), the lazy rebalance's unsafe-step test, and the keep
-// tests of the ranges' candidates against the final normaliser.  A column
-// with more candidates in some range than that range kept is marked in
-// `only` and finished by a FIXED keep pass over all kernels instead.
-// (Templated on D only so that each translation unit has its own copy.)
+// The exact values of the pairs in the kernels' lists: work items (kernel
+// n, 256 list entries), chunk-major, strided over a fixed grid of
+// workgroups; a workgroup copies the kernel's record into LDS once (all of
+// it in flight together) and its lanes, the entries' columns, read the
+// factor as broadcast LDS loads (log_rho<D> as everywhere: the same bits).
+constexpr int kExactGrid = 1024;
+
 template <int D>
-__global__ void __launch_bounds__(256) estep_screen_combine_kernel(EParams p, SParams sp) {
-  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int lane = threadIdx.x & (kWave - 1);
+__global__ void __launch_bounds__(256) estep_screen_exact_kernel(const float *__restrict__ recs,
+                                                                 const float *__restrict__ X,
+                                                                 int64_t nb, SParams sp) {
+  constexpr int64_t R = rec_floats(D);  // a multiple of 4
+  __shared__ sc_f32x4 srec[R / 4];
+  const int64_t items = nb * (sp.kl_cap / 256);
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int64_t n = it % nb;
+    const int j0 = (int)(it / nb) * 256;
+    const int c = sp.kl_cnt[n];
+    const int cnt = c < sp.kl_cap ? c : sp.kl_cap;
+    if (j0 >= cnt) continue;  // workgroup-uniform
+    const sc_f32x4 *g = reinterpret_cast<const sc_f32x4 *>(recs + n * R);
+    __syncthreads();  // the previous item's reads of srec are done
+    for (int i = threadIdx.x; i < (int)(R / 4); i += 256) srec[i] = g[i];
+    const int j = j0 + threadIdx.x;
+    float xv[D];
+    int64_t m = 0;
+    if (j < cnt) {
+      m = sp.kl_m[n * sp.kl_cap + j];
+#pragma unroll
+      for (int t = 0; t < D; ++t) xv[t] = X[m * D + t];
+    }
+    __syncthreads();
+    if (j < cnt) sp.kl_x[n * sp.kl_cap + j] = log_rho<D>(reinterpret_cast<const float *>(srec), xv);
+  }
+}
+
+template <int D>  // (D: a copy per translation unit)
+__global__ void __launch_bounds__(256) screen_zero_i32_kernel(int32_t *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Completes a split UPDATE E-step for 64 columns (one wave): each column's
+// survivors from all ranges in range, then queue order (kernel order) --
+// their exact values from the kernels' lists, or, for pairs whose kernel
+// list was full, evaluated here (all such pairs of the 64 columns in one
+// list, one pair per lane) -- and the ranges' own partial log-sum-exps
+// merged into the new normaliser (the same update as the unsplit kernel),
+// the lazy rebalance's unsafe-step test, and the keep tests of the exact
+// values against the final normaliser.  A column with more keep
+// candidates in a finished range than that range kept, or a wave with more
+// pairs to evaluate than its list holds, is marked in `only` and finished
+// by a FIXED keep pass over all kernels instead.
+constexpr int kCombineList = 1024;  // pairs per wave list (4 KB of LDS)
+constexpr int kScreenListCap = 16384;  // pairs per kernel list (a multiple of 256)
+
+template <int D>
+__global__ void __launch_bounds__(64) estep_screen_combine_kernel(EParams p, SParams sp) {
+  constexpr int64_t R = rec_floats(D);
+  __shared__ int32_t ent[kCombineList];
+  const int lane = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kWave;
+  const int64_t m = base + lane;
   const bool valid = m < p.N;
-  float cm = 0.f;
+  float co = 0.f, lb = 0.f;
+  Lse t;
+  t.init();
   bool over = false;
+  int nf = 0;  // this column's pairs to evaluate here
   if (valid) {
-    const float co = p.c_old[m];
-    const float lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
-    Lse t;
-    t.init();
-    for (int sidx = 0; sidx < sp.nsplit; ++sidx)
-      t.merge(sp.part_m[sidx * p.N + m], sp.part_s[sidx * p.N + m]);
+    co = p.c_old[m];
+    lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
+    for (int r = 0; r < sp.nsplit; ++r) {
+      t.merge(sp.part_m[r * p.N + m], sp.part_s[r * p.N + m]);
+      over = over || sp.kcount[r * p.N + m] > kKeepQ;
+      const int c = sp.sv_cnt[r * p.N + m];
+      for (int k = 0; k < c; ++k) nf += (sp.sv_e[((int64_t)r * kScreenQ + k) * p.N + m] >> 15) == 0;
+    }
+  }
+  int incl = nf;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int u = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += u;
+  }
+  const int excl = incl - nf;
+  const int total = __shfl(incl, kWave - 1, kWave);
+  // value of entry e: from its kernel's list, or the next of this column's
+  // evaluated pairs (list position f)
+  auto value = [&](int32_t e, int &f, int w0) -> float {
+    const int32_t slot = (e >> 15) - 1;
+    if (slot >= 0) return sp.kl_x[(int64_t)(e & 0x7fff) * sp.kl_cap + slot];
+    return __int_as_float(ent[(f++) - w0]);
+  };
+  // the column's entries in order, up to the first evaluated pair at list
+  // position >= lim
+  int rc = 0, kc = 0, fc = excl;
+  auto walk = [&](int lim, int w0) {
+    while (rc < sp.nsplit) {
+      if (kc >= sp.sv_cnt[rc * p.N + m]) {
+        ++rc;
+        kc = 0;
+        continue;
+      }
+      const int32_t e = sp.sv_e[((int64_t)rc * kScreenQ + kc) * p.N + m];
+      if ((e >> 15) == 0 && fc >= lim) return;
+      t.add1(value(e, fc, w0));
+      ++kc;
+    }
+  };
+  // the pairs to evaluate, in windows of kCombineList (one window but in
+  // dense regions)
+  for (int w0 = 0; w0 < total; w0 += kCombineList) {
+    if (valid && nf > 0 && excl < w0 + kCombineList && excl + nf > w0) {
+      int o = excl;
+      for (int r = 0; r < sp.nsplit; ++r) {
+        const int c = sp.sv_cnt[r * p.N + m];
+        for (int k = 0; k < c; ++k) {
+          const int32_t e = sp.sv_e[((int64_t)r * kScreenQ + k) * p.N + m];
+          if ((e >> 15) != 0) continue;
+          if (o >= w0 && o < w0 + kCombineList) ent[o - w0] = ((e & 0x7fff) << 6) | lane;
+          ++o;
+        }
+      }
+    }
+    __syncthreads();
+    const int wn = total - w0 < kCombineList ? total - w0 : kCombineList;
+    for (int j = lane; j < wn; j += kWave) {
+      const int32_t e = ent[j];
+      ent[j] = __float_as_int(log_rho_lane<D>(p.rec + (int64_t)(e >> 6) * R, p.X + (base + (e & 63)) * D));
+    }
+    __syncthreads();
+    if (valid) walk(w0 + kCombineList, w0);
+    __syncthreads();
+  }
+  if (valid) walk(0x7fffffff, 0);
+  // the evaluated values stay for the keep tests only when one window held
+  // them all
+  over = over || total > kCombineList;
+  float cm = 0.f;
+  if (valid) {
     cm = logaddexp_f(lb, t.value());
     p.c[m] = cm;
-    for (int sidx = 0; sidx < sp.nsplit; ++sidx) over = over || sp.kcount[sidx * p.N + m] > kKeepQ;
     sp.only[m] = over ? 1 : 0;
   }
-  if (p.lazy_unsafe != nullptr) lazy_unsafe_test(p, valid, m, valid ? p.c_old[m] : 0.f, cm);
+  if (p.lazy_unsafe != nullptr) lazy_unsafe_test(p, valid, m, co, cm);
   if (!valid || over) return;
   const int64_t chunk = m / kWave;
   const int64_t cseg = chunk / p.seg_len;
-  for (int sidx = 0; sidx < sp.nsplit; ++sidx) {
-    const int kc = sp.kcount[sidx * p.N + m];
-    for (int k = 0; k < kc; ++k) {
-      const int64_t q = ((int64_t)sidx * kKeepQ + k) * p.N + m;
-      const int64_t n = sp.kq_n[q];
-      const float d = __fsub_rn(sp.kq_x[q], cm);
-      float r;
-      if (keep_resp(d, p.threshold, p.log_t_lo, &r)) {
-        atomicOr((unsigned long long *)&p.masks[chunk * p.nb + n], 1ull << lane);
-        atomicAdd(&p.seg_tot[cseg * p.nb + n], 1);
-        atomicAdd(&p.row_tot[n], 1);
-      }
+  auto keep = [&](int64_t n, float x) {
+    const float d = __fsub_rn(x, cm);
+    float rr;
+    if (keep_resp(d, p.threshold, p.log_t_lo, &rr)) {
+      atomicOr((unsigned long long *)&p.masks[chunk * p.nb + n], 1ull << lane);
+      atomicAdd(&p.seg_tot[cseg * p.nb + n], 1);
+      atomicAdd(&p.row_tot[n], 1);
+    }
+  };
+  int f = excl;
+  for (int r = 0; r < sp.nsplit; ++r) {
+    const int c = sp.sv_cnt[r * p.N + m];
+    for (int k = 0; k < c; ++k) {
+      const int32_t e = sp.sv_e[((int64_t)r * kScreenQ + k) * p.N + m];
+      keep(e & 0x7fff, value(e, f, 0));
+    }
+    const int kq = sp.kcount[r * p.N + m];
+    for (int k = 0; k < kq; ++k) {
+      const int64_t q = ((int64_t)r * kKeepQ + k) * p.N + m;
+      keep(sp.kq_n[q], sp.kq_x[q]);
     }
   }
 }
@@ -910,9 +1084,14 @@ int launch_estep_screen(const EParams &p, const SParams &sp, int mode, hipStream
   } else {
     const dim3 grid((unsigned)ceil_div(p.N, 256));
     if (mode == LAKDE_ESTEP_UPDATE && sp.nsplit > 1) {
+      hipLaunchKernelGGL((screen_zero_i32_kernel<D>), dim3((unsigned)ceil_div(p.nb, 256)), dim3(256), 0,
+                         s, sp.kl_cnt, p.nb);
       hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_UPDATE, true>),
                          dim3(grid.x, (unsigned)sp.nsplit), dim3(256), 0, s, p, sp);
-      hipLaunchKernelGGL((estep_screen_combine_kernel<D>), grid, dim3(256), 0, s, p, sp);
+      hipLaunchKernelGGL((estep_screen_exact_kernel<D>), dim3(kExactGrid), dim3(256), 0, s, p.rec,
+                         p.X, p.nb, sp);
+      hipLaunchKernelGGL((estep_screen_combine_kernel<D>), dim3((unsigned)ceil_div(p.N, kWave)),
+                         dim3(kWave), 0, s, p, sp);
       hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_FIXED>), grid, dim3(256), 0, s, p,
                          sp);
       return hipGetLastError() == hipSuccess ? LAKDE_OK : LAKDE_EHIP;
