@@ -477,7 +477,7 @@ int lakde_screen_knn_bound(const float *rec, const float *X, int64_t N, int64_t 
 // The split UPDATE form's ranges: about 128 kernels each, at most 8.
 static void screen_split_geometry(int64_t nb, int *nsplit, int64_t *len) {
   int64_t sp = nb >= 256 ? nb / 128 : 1;
-  if (sp > 8) sp = 8;
+  if (sp > kMaxSplit) sp = kMaxSplit;
   *len = ceil_div(nb, sp);
   *nsplit = (int)ceil_div(nb, *len);
 }
@@ -493,7 +493,8 @@ size_t lakde_screen_split_workspace(int64_t N, int64_t nb) {
   return 2 * align256z(SN * 4) + align256z(SN) + align256z(SN * kKeepQ * 2) +
          align256z(SN * kKeepQ * 4) + align256z((size_t)N) + align256z(SN) +
          align256z(SN * kScreenQ * 4) + align256z((size_t)nb * 4) +
-         2 * align256z((size_t)nb * kScreenListCap * 4);
+         2 * align256z((size_t)nb * kScreenListCap * 4) +
+         align256z((size_t)nb * (kScreenListCap / 256) * 4) + 256;
 }
 
 int lakde_estep_main_screen_split(const lakde_estep_args *args, const void *frag, const void *xb,
@@ -534,6 +535,10 @@ int lakde_estep_main_screen_split(const lakde_estep_args *args, const void *frag
     sp.kl_m = (int32_t *)w;
     w += align256z((size_t)args->nb * kScreenListCap * 4);
     sp.kl_x = (float *)w;
+    w += align256z((size_t)args->nb * kScreenListCap * 4);
+    sp.items = (int32_t *)w;
+    w += align256z((size_t)args->nb * (kScreenListCap / 256) * 4);
+    sp.items_total = (int32_t *)w;
     sp.kl_cap = kScreenListCap;
   }
   hipStream_t s = as_stream(stream);

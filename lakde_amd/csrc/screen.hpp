@@ -132,7 +132,7 @@ struct SParams {
   uint8_t *only = nullptr;
   // split UPDATE form, survivors left to the later kernels: per (range,
   // column) sv_cnt[r N + m] of them (0 where the range finished them
-  // itself), entry k at sv_e[(r kScreenQ + k) N + m] = (slot + 1) << 15 |
+  // itself), entry k at sv_e[(m nsplit + r) kScreenQ + k] = (slot + 1) << 15 |
   // kernel, slot = the pair's place in the kernel's list (-1: the list was
   // full); per kernel n of the block kl_cnt[n] pairs claimed, the columns of
   // the first kl_cap at kl_m[n kl_cap + slot] and their exact values at
@@ -143,6 +143,10 @@ struct SParams {
   int32_t *kl_m = nullptr;
   float *kl_x = nullptr;
   int kl_cap = 0;
+  // the nonempty (kernel << 8 | 256-entry chunk) work items of the lists,
+  // items_total[0] of them
+  int32_t *items = nullptr;
+  int32_t *items_total = nullptr;
 };
 
 // ------------------------------------------------------------- preparation
@@ -551,7 +555,7 @@ estep_screen_kernel(EParams p, SParams sp) {
                 const int32_t n = sq[k0 + i][tid];
                 const bool in = slot[i] < sp.kl_cap;
                 if (in) sp.kl_m[(int64_t)n * sp.kl_cap + slot[i]] = (int32_t)m;
-                sp.sv_e[((int64_t)blockIdx.y * kScreenQ + k0 + i) * p.N + m] =
+                sp.sv_e[(m * sp.nsplit + blockIdx.y) * kScreenQ + k0 + i] =
                     ((in ? slot[i] + 1 : 0) << 15) | n;
               }
           }
@@ -896,26 +900,57 @@ int launch_pred_screen(const PParams &p, const uint8_t *frag, const __bf16 *yb,
   }
 }
 
-// The exact values of the pairs in the kernels' lists: work items (kernel
-// n, 256 list entries), chunk-major, strided over a fixed grid of
-// workgroups; a workgroup copies the kernel's record into LDS once (all of
-// it in flight together) and its lanes, the entries' columns, read the
-// factor as broadcast LDS loads (log_rho<D> as everywhere: the same bits).
+// The work items of the kernels' lists, dense: one workgroup scans the
+// per-kernel counts (each thread a run of kernels, then a scan of the runs)
+// and writes (kernel << 8 | chunk) for every 256 entries of every list.
+template <int D>  // (D: a copy per translation unit)
+__global__ void __launch_bounds__(1024) screen_items_kernel(int64_t nb, SParams sp) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (nb + 1023) / 1024;
+  const int64_t n0 = t * per, n1 = n0 + per < nb ? n0 + per : nb;
+  auto chunks = [&](int64_t n) {
+    const int c = sp.kl_cnt[n];
+    return ((c < sp.kl_cap ? c : sp.kl_cap) + 255) / 256;
+  };
+  int own = 0;
+  for (int64_t n = n0; n < n1; ++n) own += chunks(n);
+  part[t] = own;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int o = part[t] - own;
+  for (int64_t n = n0; n < n1; ++n) {
+    const int c = chunks(n);
+    for (int ch = 0; ch < c; ++ch) sp.items[o++] = (int32_t)(n << 8) | ch;
+  }
+  if (t == 1023) sp.items_total[0] = part[1023];
+}
+
+// The exact values of the pairs in the kernels' lists: the work items over
+// a fixed grid of workgroups; a workgroup copies the item's kernel record
+// into LDS once (all of it in flight together) and its lanes, the entries'
+// columns, read the factor as broadcast LDS loads (log_rho<D> as
+// everywhere: the same bits).
 constexpr int kExactGrid = 1024;
 
 template <int D>
 __global__ void __launch_bounds__(256) estep_screen_exact_kernel(const float *__restrict__ recs,
                                                                  const float *__restrict__ X,
-                                                                 int64_t nb, SParams sp) {
+                                                                 SParams sp) {
   constexpr int64_t R = rec_floats(D);  // a multiple of 4
   __shared__ sc_f32x4 srec[R / 4];
-  const int64_t items = nb * (sp.kl_cap / 256);
-  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
-    const int64_t n = it % nb;
-    const int j0 = (int)(it / nb) * 256;
+  const int total = sp.items_total[0];
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    const int32_t e = sp.items[it];
+    const int64_t n = e >> 8;
+    const int j0 = (e & 255) * 256;
     const int c = sp.kl_cnt[n];
     const int cnt = c < sp.kl_cap ? c : sp.kl_cap;
-    if (j0 >= cnt) continue;  // workgroup-uniform
     const sc_f32x4 *g = reinterpret_cast<const sc_f32x4 *>(recs + n * R);
     __syncthreads();  // the previous item's reads of srec are done
     for (int i = threadIdx.x; i < (int)(R / 4); i += 256) srec[i] = g[i];
@@ -925,7 +960,7 @@ __global__ void __launch_bounds__(256) estep_screen_exact_kernel(const float *__
     if (j < cnt) {
       m = sp.kl_m[n * sp.kl_cap + j];
 #pragma unroll
-      for (int t = 0; t < D; ++t) xv[t] = X[m * D + t];
+      for (int q = 0; q < D; ++q) xv[q] = X[m * D + q];
     }
     __syncthreads();
     if (j < cnt) sp.kl_x[n * sp.kl_cap + j] = log_rho<D>(reinterpret_cast<const float *>(srec), xv);
@@ -950,7 +985,9 @@ __global__ void __launch_bounds__(256) screen_zero_i32_kernel(int32_t *p, int64_
 // pairs to evaluate than its list holds, is marked in `only` and finished
 // by a FIXED keep pass over all kernels instead.
 constexpr int kCombineList = 1024;  // pairs per wave list (4 KB of LDS)
-constexpr int kScreenListCap = 16384;  // pairs per kernel list (a multiple of 256)
+constexpr int kScreenListCap = 16384;  // pairs per kernel list (a multiple of 256, at most 65536)
+
+constexpr int kMaxSplit = 8;  // ranges of a split E-step at most
 
 template <int D>
 __global__ void __launch_bounds__(64) estep_screen_combine_kernel(EParams p, SParams sp) {
@@ -960,21 +997,94 @@ __global__ void __launch_bounds__(64) estep_screen_combine_kernel(EParams p, SPa
   const int64_t base = (int64_t)blockIdx.x * kWave;
   const int64_t m = base + lane;
   const bool valid = m < p.N;
-  float co = 0.f, lb = 0.f;
+  const int S = sp.nsplit;
+  auto svi = [&](int r, int k) -> int64_t { return (m * S + r) * kScreenQ + k; };
+  // the column's counts, partial sums and first entries of every range,
+  // loaded together
+  int cr[kMaxSplit], kq[kMaxSplit];
+  float pm[kMaxSplit], ps[kMaxSplit];
+#pragma unroll
+  for (int r = 0; r < kMaxSplit; ++r) {
+    const bool in = valid && r < S;
+    cr[r] = in ? sp.sv_cnt[r * p.N + m] : 0;
+    kq[r] = in ? sp.kcount[r * p.N + m] : 0;
+    pm[r] = in ? sp.part_m[r * p.N + m] : neg_inf();
+    ps[r] = in ? sp.part_s[r * p.N + m] : 0.f;
+  }
+  const float co = valid ? p.c_old[m] : 0.f;
+  const float lb = valid ? logsubexp_f(co, logf(p.old_colsum[m]) + co) : 0.f;
+  constexpr int KR = 2;  // entries per range held in registers
+  int32_t e4[kMaxSplit][KR];
+#pragma unroll
+  for (int r = 0; r < kMaxSplit; ++r)
+#pragma unroll
+    for (int k = 0; k < KR; ++k) e4[r][k] = k < cr[r] ? sp.sv_e[svi(r, k)] : 0;
   Lse t;
   t.init();
   bool over = false;
-  int nf = 0;  // this column's pairs to evaluate here
-  if (valid) {
-    co = p.c_old[m];
-    lb = logsubexp_f(co, logf(p.old_colsum[m]) + co);
-    for (int r = 0; r < sp.nsplit; ++r) {
-      t.merge(sp.part_m[r * p.N + m], sp.part_s[r * p.N + m]);
-      over = over || sp.kcount[r * p.N + m] > kKeepQ;
-      const int c = sp.sv_cnt[r * p.N + m];
-      for (int k = 0; k < c; ++k) nf += (sp.sv_e[((int64_t)r * kScreenQ + k) * p.N + m] >> 15) == 0;
-    }
+  int nf = 0;  // this column's pairs to evaluate here (their kernel list was full)
+#pragma unroll
+  for (int r = 0; r < kMaxSplit; ++r) {
+    t.merge(pm[r], ps[r]);
+    over = over || kq[r] > kKeepQ;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) nf += (k < cr[r] && (e4[r][k] >> 15) == 0) ? 1 : 0;
+    for (int k = KR; k < cr[r]; ++k) nf += (sp.sv_e[svi(r, k)] >> 15) == 0;
   }
+  auto listed = [&](int32_t e) -> float {
+    return sp.kl_x[(int64_t)(e & 0x7fff) * sp.kl_cap + ((e >> 15) - 1)];
+  };
+  const int64_t chunk = m / kWave;
+  const int64_t cseg = chunk / p.seg_len;
+  auto keep = [&](int64_t n, float x, float cm) {
+    const float d = __fsub_rn(x, cm);
+    float rr;
+    if (keep_resp(d, p.threshold, p.log_t_lo, &rr)) {
+      atomicOr((unsigned long long *)&p.masks[chunk * p.nb + n], 1ull << lane);
+      atomicAdd(&p.seg_tot[cseg * p.nb + n], 1);
+      atomicAdd(&p.row_tot[n], 1);
+    }
+  };
+  auto keep_queued = [&](int r, float cm) {
+    for (int k = 0; k < kq[r]; ++k) {
+      const int64_t q = ((int64_t)r * kKeepQ + k) * p.N + m;
+      keep(sp.kq_n[q], sp.kq_x[q], cm);
+    }
+  };
+
+  if (__ballot(nf > 0) == 0) {
+    // every value is in the kernels' lists: in range, then queue order
+#pragma unroll
+    for (int r = 0; r < kMaxSplit; ++r) {
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        if (k < cr[r]) t.add1(listed(e4[r][k]));
+      for (int k = KR; k < cr[r]; ++k) t.add1(listed(sp.sv_e[svi(r, k)]));
+    }
+    float cm = 0.f;
+    if (valid) {
+      cm = logaddexp_f(lb, t.value());
+      p.c[m] = cm;
+      sp.only[m] = over ? 1 : 0;
+    }
+    if (p.lazy_unsafe != nullptr) lazy_unsafe_test(p, valid, m, co, cm);
+    if (!valid || over) return;
+#pragma unroll
+    for (int r = 0; r < kMaxSplit; ++r) {
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        if (k < cr[r]) keep(e4[r][k] & 0x7fff, listed(e4[r][k]), cm);
+      for (int k = KR; k < cr[r]; ++k) {
+        const int32_t e = sp.sv_e[svi(r, k)];
+        keep(e & 0x7fff, listed(e), cm);
+      }
+      if (r < S) keep_queued(r, cm);
+    }
+    return;
+  }
+
+  // some pairs of the wave are evaluated here, one per lane, in windows of
+  // kCombineList (one window but in dense regions)
   int incl = nf;
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
@@ -986,35 +1096,32 @@ __global__ void __launch_bounds__(64) estep_screen_combine_kernel(EParams p, SPa
   // value of entry e: from its kernel's list, or the next of this column's
   // evaluated pairs (list position f)
   auto value = [&](int32_t e, int &f, int w0) -> float {
-    const int32_t slot = (e >> 15) - 1;
-    if (slot >= 0) return sp.kl_x[(int64_t)(e & 0x7fff) * sp.kl_cap + slot];
+    if ((e >> 15) != 0) return listed(e);
     return __int_as_float(ent[(f++) - w0]);
   };
   // the column's entries in order, up to the first evaluated pair at list
   // position >= lim
   int rc = 0, kc = 0, fc = excl;
   auto walk = [&](int lim, int w0) {
-    while (rc < sp.nsplit) {
+    while (rc < S) {
       if (kc >= sp.sv_cnt[rc * p.N + m]) {
         ++rc;
         kc = 0;
         continue;
       }
-      const int32_t e = sp.sv_e[((int64_t)rc * kScreenQ + kc) * p.N + m];
+      const int32_t e = sp.sv_e[svi(rc, kc)];
       if ((e >> 15) == 0 && fc >= lim) return;
       t.add1(value(e, fc, w0));
       ++kc;
     }
   };
-  // the pairs to evaluate, in windows of kCombineList (one window but in
-  // dense regions)
   for (int w0 = 0; w0 < total; w0 += kCombineList) {
     if (valid && nf > 0 && excl < w0 + kCombineList && excl + nf > w0) {
       int o = excl;
-      for (int r = 0; r < sp.nsplit; ++r) {
+      for (int r = 0; r < S; ++r) {
         const int c = sp.sv_cnt[r * p.N + m];
         for (int k = 0; k < c; ++k) {
-          const int32_t e = sp.sv_e[((int64_t)r * kScreenQ + k) * p.N + m];
+          const int32_t e = sp.sv_e[svi(r, k)];
           if ((e >> 15) != 0) continue;
           if (o >= w0 && o < w0 + kCombineList) ent[o - w0] = ((e & 0x7fff) << 6) | lane;
           ++o;
@@ -1043,29 +1150,14 @@ __global__ void __launch_bounds__(64) estep_screen_combine_kernel(EParams p, SPa
   }
   if (p.lazy_unsafe != nullptr) lazy_unsafe_test(p, valid, m, co, cm);
   if (!valid || over) return;
-  const int64_t chunk = m / kWave;
-  const int64_t cseg = chunk / p.seg_len;
-  auto keep = [&](int64_t n, float x) {
-    const float d = __fsub_rn(x, cm);
-    float rr;
-    if (keep_resp(d, p.threshold, p.log_t_lo, &rr)) {
-      atomicOr((unsigned long long *)&p.masks[chunk * p.nb + n], 1ull << lane);
-      atomicAdd(&p.seg_tot[cseg * p.nb + n], 1);
-      atomicAdd(&p.row_tot[n], 1);
-    }
-  };
   int f = excl;
-  for (int r = 0; r < sp.nsplit; ++r) {
+  for (int r = 0; r < S; ++r) {
     const int c = sp.sv_cnt[r * p.N + m];
     for (int k = 0; k < c; ++k) {
-      const int32_t e = sp.sv_e[((int64_t)r * kScreenQ + k) * p.N + m];
-      keep(e & 0x7fff, value(e, f, 0));
+      const int32_t e = sp.sv_e[svi(r, k)];
+      keep(e & 0x7fff, value(e, f, 0), cm);
     }
-    const int kq = sp.kcount[r * p.N + m];
-    for (int k = 0; k < kq; ++k) {
-      const int64_t q = ((int64_t)r * kKeepQ + k) * p.N + m;
-      keep(sp.kq_n[q], sp.kq_x[q]);
-    }
+    keep_queued(r, cm);
   }
 }
 
@@ -1088,8 +1180,9 @@ int launch_estep_screen(const EParams &p, const SParams &sp, int mode, hipStream
                          s, sp.kl_cnt, p.nb);
       hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_UPDATE, true>),
                          dim3(grid.x, (unsigned)sp.nsplit), dim3(256), 0, s, p, sp);
+      hipLaunchKernelGGL((screen_items_kernel<D>), dim3(1), dim3(1024), 0, s, p.nb, sp);
       hipLaunchKernelGGL((estep_screen_exact_kernel<D>), dim3(kExactGrid), dim3(256), 0, s, p.rec,
-                         p.X, p.nb, sp);
+                         p.X, sp);
       hipLaunchKernelGGL((estep_screen_combine_kernel<D>), dim3((unsigned)ceil_div(p.N, kWave)),
                          dim3(kWave), 0, s, p, sp);
       hipLaunchKernelGGL((estep_screen_kernel<D, LAKDE_ESTEP_FIXED>), grid, dim3(256), 0, s, p,
