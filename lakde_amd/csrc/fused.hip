@@ -185,6 +185,52 @@ __global__ void __launch_bounds__(256) rebal_rescale_kernel(int64_t cap,
     vb[e] = vb[e] * coeff[cb[e]];
 }
 
+// A lazy step the stored form cannot represent, done exactly in two passes
+// over the arena.  First: the other blocks' stored values made true (u G,
+// relative to the old normalisers) and rescaled to the new ones, their
+// sums in A; the step's own block, written as true values by the
+// compaction, summed in B.  Second (after rebal_norm_kernel): every value
+// times 1/s, and U = the new stored (= true, G = 1) column sums.  The same
+// roundings as a fold, the exact rebalance and a second fold in sequence.
+__global__ void __launch_bounds__(256) rebal_fold_scale_sum_kernel(
+    int64_t cap, int64_t exclude, const int32_t *__restrict__ nnz, const int32_t *__restrict__ cols,
+    float *__restrict__ vals, const float *__restrict__ G, const float *__restrict__ coeff,
+    Fx *__restrict__ A, Fx *__restrict__ B) {
+  const int64_t b = blockIdx.y;
+  const int32_t n = nnz[b];
+  const int32_t *cb = cols + b * cap;
+  float *vb = vals + b * cap;
+  const bool scale = b != exclude;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int32_t col = cb[e];
+    if (scale) {
+      const float v = vb[e] * G[col];
+      vb[e] = v * coeff[col];
+      fx_atomic_add(&A[col], v);
+    } else {
+      fx_atomic_add(&B[col], vb[e]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) rebal_rescale_sum_kernel(int64_t cap,
+                                                               const int32_t *__restrict__ nnz,
+                                                               const int32_t *__restrict__ cols,
+                                                               float *__restrict__ vals,
+                                                               const float *__restrict__ coeff,
+                                                               Fx *__restrict__ U) {
+  const int64_t b = blockIdx.y;
+  const int32_t n = nnz[b];
+  const int32_t *cb = cols + b * cap;
+  float *vb = vals + b * cap;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int32_t col = cb[e];
+    const float v = vb[e] * coeff[col];
+    vb[e] = v;
+    fx_atomic_add(&U[col], v);
+  }
+}
+
 // ------------------------------------------------------- lazy rebalance
 // Stored value u_e, true value u_e * G[col_e]; U[m] = sum of stored values
 // of column m, an order-independent fixed-point sum: U - U_old is exact, so
@@ -532,6 +578,38 @@ int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *fl
                      : nullptr;
   hipLaunchKernelGGL(estep_offsets_kernel, dim3((unsigned)ceil_div(p.nb, kWave) + 1u), dim3(1024),
                      0, s, p);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+int rebalance_lazy_unsafe(int64_t N, int64_t num_blocks, int64_t cap, int64_t block,
+                          const int32_t *nnz, const int32_t *cols, float *vals,
+                          const float *c_prev, float *c, float *G, Fx *U, void *ws,
+                          hipStream_t s) {
+  LAKDE_CHECK_ARG(N >= 1 && num_blocks >= 1 && num_blocks <= 65535 && cap >= 1 &&
+                      cap < (1LL << 31) && block >= 0 && block < num_blocks,
+                  "bad size");
+  LAKDE_CHECK_ARG(nnz && cols && vals && c_prev && c && G && U && ws, "null pointer");
+  LAKDE_CHECK_ARG(((uintptr_t)ws & 15) == 0, "ws must be 16-byte aligned");
+  float *coeff = (float *)ws;
+  Fx *A = (Fx *)((char *)ws + ((size_t)N * 4 + 15) / 16 * 16), *B = A + N;
+  const unsigned gN = (unsigned)ceil_div(N, 256);
+  int64_t gx = ceil_div(cap, 256 * 4);
+  if (gx > 64) gx = 64;
+  const dim3 ge((unsigned)gx, (unsigned)num_blocks);
+  hipLaunchKernelGGL(rebal_coeff_kernel, dim3(gN), dim3(256), 0, s, N, c_prev, c, coeff);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(rebal_fold_scale_sum_kernel, ge, dim3(256), 0, s, cap, block, nnz, cols, vals,
+                     G, coeff, A, B);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(rebal_norm_kernel, dim3(gN), dim3(256), 0, s, N, c, A, B, coeff);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(zero_fx_kernel, dim3(gN), dim3(256), 0, s, U, N);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(rebal_rescale_sum_kernel, ge, dim3(256), 0, s, cap, nnz, cols, vals, coeff,
+                     U);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(fold_reset_kernel, dim3(gN), dim3(256), 0, s, N, G);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }

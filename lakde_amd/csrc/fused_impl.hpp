@@ -714,6 +714,10 @@ int estep_count(const lakde_estep_args &a, hipStream_t s,
 int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
                            const float *c_prev, float *c, float *G, Fx *U, Fx *U_old, Fx *ws,
                            LazyGuard guard, hipStream_t s);
+int rebalance_lazy_unsafe(int64_t N, int64_t num_blocks, int64_t cap, int64_t block,
+                          const int32_t *nnz, const int32_t *cols, float *vals,
+                          const float *c_prev, float *c, float *G, Fx *U, void *ws,
+                          hipStream_t s);
 void pred_geometry(int64_t N, int64_t M, int64_t *per_split, int64_t *nsplit);
 int estep_fill_params(const lakde_estep_args &a, EParams *p);
 int pred_fill_params(const float *rec, int64_t N, int64_t M, float *out, void *workspace,

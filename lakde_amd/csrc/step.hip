@@ -101,16 +101,12 @@ int step_rebalance(const lakde_step_args *a, hipStream_t s) {
     return rebalance_lazy_guarded(N, nullptr, nullptr, nullptr, e.c_old, e.c, a->G, a->U,
                                   a->U_old, a->lazy_ws, LazyGuard{nullptr, 0, nullptr}, s);
   }
-  // the stored form cannot represent this step: fold, rebalance the true
-  // values exactly, recompute the totals, clear the per-step sums
-  rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all, a->G,
-                            a->U, s);
-  if (rc) return rc;
-  rc = lakde_rebalance(N, a->num_blocks, a->cap, a->block, a->nnz_all, a->cols_all, a->vals_all,
-                       e.c_old, e.c, a->rebal_ws, s);
-  if (rc) return rc;
-  rc = lakde_rebalance_fold(N, a->num_blocks, a->cap, a->nnz_all, a->cols_all, a->vals_all, a->G,
-                            a->U, s);
+  // the stored form cannot represent this step: rebalance the true values
+  // exactly (the other blocks' stored values folded on the way; the step's
+  // block holds true values already), recompute the totals, clear the
+  // per-step sums
+  rc = rebalance_lazy_unsafe(N, a->num_blocks, a->cap, a->block, a->nnz_all, a->cols_all,
+                             a->vals_all, e.c_old, e.c, a->G, a->U, a->rebal_ws, s);
   if (rc) return rc;
   LAKDE_HIP(hipMemsetAsync(a->U_old, 0, (size_t)N * sizeof(lakde_fx128), s));
   return LAKDE_OK;

@@ -361,22 +361,36 @@ def test_state_dict_round_trip(golden):
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5)
 
 
-def test_lazy_rebalance_matches_exact_mid_sweep():
+@pytest.mark.parametrize("name,n,k", [("mix32_d6", 4000, 32), ("mix64_d21", 6000, 64)])
+def test_lazy_rebalance_matches_exact_mid_sweep(name, n, k):
     """The lazy rebalance (per-column factors, folded once per sweep) against
     the exact whole-arena rebalance on the same inputs, read mid-sweep (the
     responsibilities handed out fold the pending factors) and after the fit:
     values within 1e-4 relative on the common entries, log normalisers within
-    1e-5 relative, index sets equal up to a few threshold-crossing entries."""
+    1e-5 relative, index sets equal up to a few threshold-crossing entries.
+    mix64_d21 moves normalisers by more than the stored form's window
+    (2^40) within single steps, so most of its lazy steps fall back to the
+    exact rebalance, some of them after a normal lazy step of the same sweep
+    (per-column factors G != 1): the fallback must not apply G to the step's
+    own block, whose values the compaction wrote as true values.  There the
+    two modes' float32 roundings move a few entries across the threshold
+    (and their kernels' posteriors with them), so up to 2% of the
+    normalisers may differ, each by less than 0.05, and up to 5% of the
+    values after three sweeps (with G applied to the step's block: 42% of
+    the normalisers, by up to 0.64)."""
     from lakde_amd import LocalKNearestKDE
     from lakde_amd import synthetic
-    X, Y = synthetic.make("mix32_d6", 4000, 300)
-    graph = O.compute_knn_graph(X, 32)
-    t = 1 / 32
+    X, Y = synthetic.make(name, n, 300)
+    graph = O.compute_knn_graph(X, k)
+    t = 1 / k
     Xd = _t(X)
     snaps = {}
+    flags = []
 
     def grab(tag):
         def cb(X_, model, step):
+            if tag == "lazy":
+                flags.append(int(model._flags[model.arena.num_blocks + 1]) != 0)
             if step == 5:
                 snaps[tag] = (_pairs(model), model.rnm_log_consts.cpu().numpy())
         return cb
@@ -386,18 +400,25 @@ def test_lazy_rebalance_matches_exact_mid_sweep():
         m = LocalKNearestKDE(k_or_knn_graph=_t(graph), block_size=512, rebalance=reb)
         m.fit(Xd, iterations=3, sparsity_threshold=t, callbacks=grab(reb))
         ms[reb] = m
+    nbk = (n - 1) // 512 + 1
+    after_safe = sum(1 for i in range(1, len(flags)) if flags[i] and not flags[i - 1] and i % nbk)
+    print(f"[{name}] lazy steps {len(flags)}, unsafe {sum(flags)}, unsafe after a lazy step of "
+          f"the same sweep {after_safe}")
+    if name == "mix64_d21":
+        assert after_safe > 0
+    frac, vfrac = (0.995, 0.995) if name == "mix32_d6" else (0.98, 0.95)
     for stage in ("mid", "end"):
         if stage == "mid":
             (pl, cl), (pe, ce) = snaps["lazy"], snaps["exact"]
         else:
             pl, cl = _pairs(ms["lazy"]), ms["lazy"].rnm_log_consts.cpu().numpy()
             pe, ce = _pairs(ms["exact"]), ms["exact"].rnm_log_consts.cpu().numpy()
-        _close_but_flips(cl, ce)
+        _close_but_flips(cl, ce, frac=frac)
         N = X.shape[0]
         kl = pl[0] * N + pl[1]
         ke = pe[0] * N + pe[1]
         common, il, ie = np.intersect1d(kl, ke, return_indices=True)
-        _close_but_flips(pl[2][il], pe[2][ie], rtol=1e-4, atol=1e-9, flip_tol=0.05)
+        _close_but_flips(pl[2][il], pe[2][ie], rtol=1e-4, atol=1e-9, frac=vfrac, flip_tol=0.05)
         # a float32 rounding difference can move an entry across the
         # threshold at its E-step; such entries are rare
         ndiff = kl.size + ke.size - 2 * common.size
@@ -406,7 +427,13 @@ def test_lazy_rebalance_matches_exact_mid_sweep():
               f"only exact {ke.size - common.size}")
     a = ms["lazy"].log_pred_density(Xd, _t(Y)).cpu().numpy()
     b = ms["exact"].log_pred_density(Xd, _t(Y)).cpu().numpy()
-    _close_but_flips(a, b, flip_tol=1e-3)
+    if name == "mix32_d6":
+        _close_but_flips(a, b, flip_tol=1e-3)
+    else:
+        # held-out density within 1e-3 nats per point, 1e-4 on average
+        d = np.abs(a.astype(np.float64) - b)
+        print(f"[{name}] held-out |lazy - exact|: max {d.max():.2e}, mean {d.mean():.2e}")
+        assert d.max() < 1e-3 and d.mean() < 1e-4, (d.max(), d.mean())
 
 
 def test_estep_skip_matches_full_evaluation():
