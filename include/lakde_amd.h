@@ -261,6 +261,12 @@ typedef struct {
    * NULL: columns in index order, no skipping. */
   const int32_t *col_perm;
   const float *chunk_box;
+  /* optional (with lazy_unsafe, UPDATE): N floats, zero on entry.  A column
+   * whose lazy_G[m] clamp(exp(c_old - c)) falls below 2^-40 but stays above
+   * 2^-100, with clamp(exp(c_old - c)) >= 2^-100, gets that factor here and
+   * *lazy_unsafe |= 2 (re-basing the column, see lakde_step_args) instead of
+   * *lazy_unsafe |= 1.  NULL: every column outside the window sets 1. */
+  float *lazy_factor;
 } lakde_estep_args;
 
 size_t lakde_estep_workspace(int64_t N, int64_t nb);
@@ -345,7 +351,12 @@ int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32
  *   bf16_frag != NULL: lakde_pack_frag_bf16 of estep.rec and
  *     lakde_estep_main_bf16, else lakde_estep_main (ev_main[0], ev_main[1]:
  *     optional hipEvent_t recorded around it); lakde_estep_finish;
- *   the row counts of the E-step; lazy: lakde_rebalance_lazy_check;
+ *   the row counts of the E-step; lazy: lakde_rebalance_lazy_check, with
+ *     the re-basing test (estep.lazy_factor in rebal_ws): a step whose only
+ *     out-of-window columns are re-basable (flag bit 1, bit 0 clear) has
+ *     those columns' stored values in the other blocks multiplied by their
+ *     factor f = G coeff, their totals recomputed and G divided by f on the
+ *     device before the compaction, and continues as a lazy step;
  *   host_flags[0 .. num_blocks+1] <- nnz_all (per-block counts, status,
  *     unsafe flag) [record ev_flags]; the compaction and the guarded lazy
  *     rebalance; wait for ev_flags (or the whole stream without it), so the
@@ -354,9 +365,10 @@ int lakde_rebalance_fold(int64_t N, int64_t num_blocks, int64_t cap, const int32
  *     (the caller grows the arena, updates the pointers and calls
  *     lakde_partial_step_resume, which compacts again and rebalances);
  *   otherwise the rebalance: lazy (lakde_rebalance_lazy, enqueued before the
- *   copy with a device-side guard that skips it for an unsafe or
- *   overflowing step; for an unsafe step fold + lakde_rebalance + fold
- *   after the synchronisation) or lakde_rebalance(exclude block).
+ *   copy with a device-side guard that skips it for an unsafe (flag bit 0)
+ *   or overflowing step; for an unsafe step the exact rebalance of the true
+ *   values, the other blocks folded on the way, after the synchronisation)
+ *   or lakde_rebalance(exclude block).
  *
  * estep must be an ESTEP_UPDATE step whose count, status and entry arrays are
  * the block's slots of the arena: estep.nnz_dev == nnz_all + block,
@@ -377,8 +389,9 @@ typedef struct {
   float *G;                /* N, lazy rebalance factors, or NULL (exact) */
   lakde_fx128 *U, *U_old;  /* N each (lazy) */
   lakde_fx128 *lazy_ws;    /* 3 N (lazy) */
-  void *rebal_ws;          /* lakde_rebalance_workspace(N) bytes: exact rebalance, the
-                              exact path's old column sums and the lazy fallback */
+  void *rebal_ws;          /* lakde_rebalance_workspace(N) bytes, zero-filled once: exact
+                              rebalance, the exact path's old column sums, the lazy
+                              fallback and the lazy re-basing state */
   int fold_first;          /* lazy: fold G into the values before the step */
   void *bf16_frag;         /* nb * lakde_bf16_frag_bytes(D) bytes, or NULL */
   int32_t *host_flags;     /* pinned host, num_blocks + 2 */

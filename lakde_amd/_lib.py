@@ -43,6 +43,7 @@ class EstepArgs(ctypes.Structure):
         ("rowptr", _vp), ("out_row", _vp), ("out_col", _vp), ("out_val", _vp),
         ("cap", _i64), ("nnz_dev", _vp), ("workspace", _vp), ("workspace_bytes", _sz),
         ("lazy_G", _vp), ("lazy_unsafe", _vp), ("col_perm", _vp), ("chunk_box", _vp),
+        ("lazy_factor", _vp),
     ]
 
 

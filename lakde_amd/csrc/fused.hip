@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(1024) estep_offsets_kernel(EParams p) {
     p.rowptr[p.nb] = sh[1023];
     *p.nnz_dev = sh[1023];
     if (p.abort_flag != nullptr)
-      *p.abort_flag = (sh[1023] > p.cap || *p.status_flag != 0 || *p.unsafe_flag != 0) ? 1 : 0;
+      *p.abort_flag = (sh[1023] > p.cap || *p.status_flag != 0 || (*p.unsafe_flag & 1) != 0) ? 1 : 0;
     if (p.flag_words != nullptr) {
       const unsigned long long tag = (unsigned long long)p.flag_seq << 32;
       const unsigned long long w[3] = {tag | (uint32_t)sh[1023], tag | (uint32_t)*p.status_flag,
@@ -231,6 +231,55 @@ __global__ void __launch_bounds__(256) rebal_rescale_sum_kernel(int64_t cap,
   }
 }
 
+// Re-basing the columns whose normaliser rose past the stored form's window
+// (lazy_unsafe_test: bit 1 of the flag, factor f = G coeff in the factor
+// array): the other blocks' stored values of such a column become its true
+// values relative to the step's normaliser (u f), their sum the column's
+// stored total, and G = G / f, so that G coeff = 1 for the step.  The
+// step's own block is skipped (its old entries are being replaced; the
+// compaction writes the new ones with the new G).  Nothing happens for a
+// step that takes the exact rebalance (bit 0), whose factors are cleared.
+__global__ void __launch_bounds__(256) rebase_scan_kernel(int64_t cap, int64_t exclude,
+                                                         const int32_t *__restrict__ nnz,
+                                                         const int32_t *__restrict__ cols,
+                                                         float *__restrict__ vals,
+                                                         const float *__restrict__ factor,
+                                                         Fx *__restrict__ U2,
+                                                         const int32_t *__restrict__ unsafe) {
+  if (*unsafe != 2) return;  // re-basing only
+  const int64_t b = blockIdx.y;
+  if (b == exclude) return;
+  const int32_t n = nnz[b];
+  const int32_t *cb = cols + b * cap;
+  float *vb = vals + b * cap;
+  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int32_t col = cb[e];
+    const float f = factor[col];
+    if (f != 0.f) {
+      const float v = vb[e] * f;
+      vb[e] = v;
+      fx_atomic_add(&U2[col], v);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) rebase_finish_kernel(int64_t N, float *__restrict__ factor,
+                                                           Fx *__restrict__ U2, float *__restrict__ G,
+                                                           Fx *__restrict__ U, Fx *__restrict__ U_old,
+                                                           const int32_t *__restrict__ unsafe) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  const float f = factor[m];
+  if (f == 0.f) return;
+  if (*unsafe == 2) {
+    U[m] = U2[m];
+    U_old[m] = fx_zero();
+    G[m] = G[m] / f;
+  }
+  U2[m] = fx_zero();
+  factor[m] = 0.f;
+}
+
 // ------------------------------------------------------- lazy rebalance
 // Stored value u_e, true value u_e * G[col_e]; U[m] = sum of stored values
 // of column m, an order-independent fixed-point sum: U - U_old is exact, so
@@ -286,7 +335,7 @@ __global__ void __launch_bounds__(256) lazy_check_kernel(int64_t N, const float 
 // the step's outcome: the kernels do nothing when the unsafe flag is set or
 // the block's entries did not fit the arena (the host then finishes the step).
 __device__ __forceinline__ bool lazy_skip(const LazyGuard &g) {
-  return g.unsafe != nullptr && (*g.unsafe != 0 || (int64_t)*g.nnz > g.cap);
+  return g.unsafe != nullptr && ((*g.unsafe & 1) != 0 || (int64_t)*g.nnz > g.cap);
 }
 
 // after the E-step: the block's true values r -> stored u = r / (G coeff),
@@ -465,6 +514,7 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
   p->nnz_dev = a.nnz_dev;
   p->lazy_G = nullptr;
   p->lazy_unsafe = nullptr;
+  p->lazy_factor = nullptr;
   p->flag_words = nullptr;
   p->status_flag = p->unsafe_flag = nullptr;
   p->flag_seq = 0;
@@ -489,6 +539,7 @@ int estep_fill_params(const lakde_estep_args &a, EParams *p) {
     LAKDE_CHECK_ARG(a.lazy_G != nullptr, "lazy_unsafe needs lazy_G");
     p->lazy_G = a.lazy_G;
     p->lazy_unsafe = a.lazy_unsafe;
+    p->lazy_factor = a.lazy_factor;
   }
   return LAKDE_OK;
 }
@@ -578,6 +629,31 @@ int estep_count(const lakde_estep_args &a, hipStream_t s, unsigned long long *fl
                      : nullptr;
   hipLaunchKernelGGL(estep_offsets_kernel, dim3((unsigned)ceil_div(p.nb, kWave) + 1u), dim3(1024),
                      0, s, p);
+  LAKDE_LAUNCHED();
+  return LAKDE_OK;
+}
+
+// the exact rebalance's workspace (coefficients, then two zeroed fixed-point
+// accumulators A and B) between exact steps: the factor array of the
+// re-based columns in B's first N floats, their new totals in A; both are
+// zero again after every step
+float *rebase_factors(int64_t N, void *ws) {
+  Fx *A = (Fx *)((char *)ws + ((size_t)N * 4 + 15) / 16 * 16);
+  return reinterpret_cast<float *>(A + N);
+}
+
+int rebalance_lazy_rebase(int64_t N, int64_t num_blocks, int64_t cap, int64_t block,
+                          const int32_t *nnz, const int32_t *cols, float *vals, float *G, Fx *U,
+                          Fx *U_old, void *ws, const int32_t *unsafe, hipStream_t s) {
+  Fx *A = (Fx *)((char *)ws + ((size_t)N * 4 + 15) / 16 * 16);
+  float *factor = rebase_factors(N, ws);
+  int64_t gx = ceil_div(cap, 256 * 4);
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(rebase_scan_kernel, dim3((unsigned)gx, (unsigned)num_blocks), dim3(256), 0, s,
+                     cap, block, nnz, cols, vals, factor, A, unsafe);
+  LAKDE_LAUNCHED();
+  hipLaunchKernelGGL(rebase_finish_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, N,
+                     factor, A, G, U, U_old, unsafe);
   LAKDE_LAUNCHED();
   return LAKDE_OK;
 }

@@ -32,6 +32,7 @@ struct EParams {
   int32_t *nnz_dev;
   const float *lazy_G;   // optional unsafe-step test of the lazy rebalance
   int32_t *lazy_unsafe;
+  float *lazy_factor;    // optional: the factors of the columns to re-base
   // optional (compaction of a lazy-rebalance step): unless the step is
   // unsafe, store u = r / (G coeff) instead of r and accumulate the column
   // sums of r and u (the work of lazy_block_kernel)
@@ -71,14 +72,31 @@ constexpr float kLazyGcLo = 0x1p-40f, kLazyGcHi = 0x1p40f;
 // the lazy rebalance's unsafe-step test for column m of an UPDATE step
 // (lakde_rebalance_lazy_check): G[m] clamp(exp(c_old - c)) outside the
 // window or NaN; one atomic per wave with an unsafe column
+// A column whose G coeff falls below the window (its normaliser rose by more
+// than 40 ln 2 nats since its factor was last 1) is re-based when the
+// factor and coefficient stay well inside float range: bit 1 of the flag,
+// its factor in lazy_factor[m].  Anything else outside the window (NaN,
+// above it, further below) takes the exact rebalance: bit 0.
+constexpr float kLazyRebaseMin = 0x1p-100f;
+
 __device__ __forceinline__ void lazy_unsafe_test(const EParams &p, bool valid, int64_t m,
                                                  float co, float cm) {
-  bool bad = false;
+  bool bad = false, rebase = false;
   if (valid) {
-    const float gc = p.lazy_G[m] * clamp_nan(expf(co - cm), FLT_MIN, FLT_MAX);
-    bad = !(gc >= kLazyGcLo && gc <= kLazyGcHi);
+    const float coeff = clamp_nan(expf(co - cm), FLT_MIN, FLT_MAX);
+    const float gc = p.lazy_G[m] * coeff;
+    if (!(gc >= kLazyGcLo && gc <= kLazyGcHi)) {
+      rebase = p.lazy_factor != nullptr && gc < kLazyGcLo && gc >= kLazyRebaseMin &&
+               coeff >= kLazyRebaseMin;
+      bad = !rebase;
+      if (rebase) p.lazy_factor[m] = gc;
+    }
   }
-  if (__ballot(bad) != 0 && (threadIdx.x & (kWave - 1)) == 0) atomicOr(p.lazy_unsafe, 1);
+  const bool any_bad = __ballot(bad) != 0, any_rebase = __ballot(rebase) != 0;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    if (any_bad) atomicOr(p.lazy_unsafe, 1);
+    if (any_rebase) atomicOr(p.lazy_unsafe, 2);
+  }
 }
 
 struct PParams {
@@ -398,7 +416,7 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
   constexpr int64_t R = rec_floats(D);
   const float *rec = p.rec + n * R;
   int64_t pos = (int64_t)p.rowptr[n] + s_lo;
-  const bool to_stored = p.lazy_Rnew != nullptr && *p.lazy_unsafe == 0;
+  const bool to_stored = p.lazy_Rnew != nullptr && (*p.lazy_unsafe & 1) == 0;
   // kept columns are gathered KB at a time so their point loads are in
   // flight together (the chain per lane is otherwise one load per entry)
   constexpr int KB = D <= 8 ? 4 : (D <= 24 ? 2 : 1);
@@ -488,7 +506,7 @@ __global__ void __launch_bounds__(256) estep_values_kernel(const float *__restri
   if (n >= p.nb) return;
   constexpr int64_t R = rec_floats(D);
   const float *rec = recs + n * R;
-  const bool to_stored = p.lazy_Rnew != nullptr && *p.lazy_unsafe == 0;
+  const bool to_stored = p.lazy_Rnew != nullptr && (*p.lazy_unsafe & 1) == 0;
   const int64_t beg = p.rowptr[n], end = p.rowptr[n + 1];
   for (int64_t e0 = beg; e0 < end; e0 += kWave) {
     const int64_t e = e0 + lane;
@@ -714,6 +732,10 @@ int estep_count(const lakde_estep_args &a, hipStream_t s,
 int rebalance_lazy_guarded(int64_t N, const int32_t *col, float *val, const int32_t *nnz_dev,
                            const float *c_prev, float *c, float *G, Fx *U, Fx *U_old, Fx *ws,
                            LazyGuard guard, hipStream_t s);
+int rebalance_lazy_rebase(int64_t N, int64_t num_blocks, int64_t cap, int64_t block,
+                          const int32_t *nnz, const int32_t *cols, float *vals, float *G, Fx *U,
+                          Fx *U_old, void *ws, const int32_t *unsafe, hipStream_t s);
+float *rebase_factors(int64_t N, void *ws);
 int rebalance_lazy_unsafe(int64_t N, int64_t num_blocks, int64_t cap, int64_t block,
                           const int32_t *nnz, const int32_t *cols, float *vals,
                           const float *c_prev, float *c, float *G, Fx *U, void *ws,

@@ -95,7 +95,7 @@ int step_rebalance(const lakde_step_args *a, hipStream_t s) {
     return lakde_rebalance(N, a->num_blocks, a->cap, a->block, a->nnz_all, a->cols_all,
                            a->vals_all, e.c_old, e.c, a->rebal_ws, s);
   }
-  const int32_t unsafe = a->host_flags[a->num_blocks + 1];
+  const int32_t unsafe = a->host_flags[a->num_blocks + 1] & 1;
   if (!unsafe) {
     // the compaction stored u and accumulated the block's sums: normalise
     return rebalance_lazy_guarded(N, nullptr, nullptr, nullptr, e.c_old, e.c, a->G, a->U,
@@ -186,6 +186,7 @@ int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
   if (a->G != nullptr) {
     em.lazy_G = a->G;
     em.lazy_unsafe = unsafe;
+    if (a->rebal_ws != nullptr) em.lazy_factor = rebase_factors(N, a->rebal_ws);
   }
   if (a->ev_main[0]) LAKDE_HIP(hipEventRecord((hipEvent_t)a->ev_main[0], s));
   if (a->bf16_frag != nullptr && a->fp32_mfma)
@@ -227,6 +228,12 @@ int step_rest(const lakde_step_args *a, hipStream_t s, uint32_t *seq_out) {
   // rebalance outcome run while the host reads the flags and prepares the
   // next step; the lazy kernels skip themselves for an unsafe or
   // overflowing step
+  if (a->G != nullptr && a->rebal_ws != nullptr) {
+    // columns to re-base (nothing unless only bit 1 of the flag is set)
+    rc = rebalance_lazy_rebase(N, a->num_blocks, a->cap, a->block, a->nnz_all, a->cols_all,
+                               a->vals_all, a->G, a->U, a->U_old, a->rebal_ws, unsafe, s);
+    if (rc) return rc;
+  }
   if (a->G != nullptr) {
     // the compaction also stores u = r / (G coeff) and the block's column
     // sums unless the step is unsafe; the normalisation skips itself for an
@@ -261,7 +268,7 @@ int step_wait(const lakde_step_args *a, hipStream_t s, uint32_t seq) {
     set_error("lakde_partial_step: the block's entries exceed the arena capacity");
     return LAKDE_ECAPACITY;
   }
-  if (a->G != nullptr && a->host_flags[a->num_blocks + 1] == 0) return LAKDE_OK;  // done
+  if (a->G != nullptr && (a->host_flags[a->num_blocks + 1] & 1) == 0) return LAKDE_OK;  // done
   return step_rebalance(a, s);
 }
 
@@ -333,9 +340,9 @@ int lakde_partial_sweep(lakde_step_args *steps, int64_t first, int64_t count, fl
     if (rc) return rc;
     // an unsafe step was finished by the host after the guarded posterior,
     // which did nothing: enqueue it again
-    if (a->G != nullptr && a->host_flags[a->num_blocks + 1] != 0) begun = false;
+    if (a->G != nullptr && (a->host_flags[a->num_blocks + 1] & 1) != 0) begun = false;
     *cur = 1 - *cur;
-    if (a->G != nullptr) *lazy_dirty = a->host_flags[a->num_blocks + 1] == 0 ? 1 : 0;
+    if (a->G != nullptr) *lazy_dirty = (a->host_flags[a->num_blocks + 1] & 1) == 0 ? 1 : 0;
     *done = i + 1;
   }
   return LAKDE_OK;

@@ -441,7 +441,7 @@ class LocallyAdaptiveKDE(AbstractKDE):
                 a.estep.c_old = self.rnm_log_consts.data_ptr()
                 a.estep.c = self._c_new.data_ptr()
                 check(lib.lakde_partial_step_resume(ctypes.byref(a), st), "partial_step_resume")
-                unsafe = int(self._flags[arena.num_blocks + 1]) != 0
+                unsafe = (int(self._flags[arena.num_blocks + 1]) & 1) != 0
                 self._account(b, b + 1, 1, int(lazy is not None and not unsafe))
                 b += 1
 

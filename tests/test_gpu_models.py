@@ -368,16 +368,19 @@ def test_lazy_rebalance_matches_exact_mid_sweep(name, n, k):
     responsibilities handed out fold the pending factors) and after the fit:
     values within 1e-4 relative on the common entries, log normalisers within
     1e-5 relative, index sets equal up to a few threshold-crossing entries.
-    mix64_d21 moves normalisers by more than the stored form's window
-    (2^40) within single steps, so most of its lazy steps fall back to the
-    exact rebalance, some of them after a normal lazy step of the same sweep
-    (per-column factors G != 1): the fallback must not apply G to the step's
-    own block, whose values the compaction wrote as true values.  There the
-    two modes' float32 roundings move a few entries across the threshold
-    (and their kernels' posteriors with them), so up to 2% of the
-    normalisers may differ, each by less than 0.05, and up to 5% of the
-    values after three sweeps (with G applied to the step's block: 42% of
-    the normalisers, by up to 0.64)."""
+    mix64_d21 raises normalisers by more than the stored form's window
+    (2^40) within single steps, so most of its lazy steps re-base columns
+    (their other blocks' values made true relative to the step's normaliser,
+    G = 1 / coeff), some of them after a lazy step of the same sweep
+    (G != 1); the exact fallback, for columns that cannot be re-based, must
+    not apply G to the step's own block, whose values the compaction wrote
+    as true values.  There the two modes' float32 roundings are amplified
+    (an entry moved across the threshold moves its kernel's posterior), so
+    mix64_d21 is held to bounds on the differences instead: normalisers
+    within 0.05 (1e-4 on average), values within 5e-3 (1e-4 on average), the
+    held-out density within 1e-3 nats (1e-4 on average); with G applied to
+    the step's block the normalisers differ by up to 0.64, 1.6e-2 on
+    average."""
     from lakde_amd import LocalKNearestKDE
     from lakde_amd import synthetic
     X, Y = synthetic.make(name, n, 300)
@@ -390,7 +393,7 @@ def test_lazy_rebalance_matches_exact_mid_sweep(name, n, k):
     def grab(tag):
         def cb(X_, model, step):
             if tag == "lazy":
-                flags.append(int(model._flags[model.arena.num_blocks + 1]) != 0)
+                flags.append(int(model._flags[model.arena.num_blocks + 1]))
             if step == 5:
                 snaps[tag] = (_pairs(model), model.rnm_log_consts.cpu().numpy())
         return cb
@@ -401,24 +404,41 @@ def test_lazy_rebalance_matches_exact_mid_sweep(name, n, k):
         m.fit(Xd, iterations=3, sparsity_threshold=t, callbacks=grab(reb))
         ms[reb] = m
     nbk = (n - 1) // 512 + 1
-    after_safe = sum(1 for i in range(1, len(flags)) if flags[i] and not flags[i - 1] and i % nbk)
-    print(f"[{name}] lazy steps {len(flags)}, unsafe {sum(flags)}, unsafe after a lazy step of "
-          f"the same sweep {after_safe}")
+    exact = [(f & 1) != 0 for f in flags]
+    rebased = [f == 2 for f in flags]
+    # steps whose G can differ from 1: not the first of a sweep, and after
+    # a lazy (not exact) step
+    g_set = [i % nbk != 0 and not exact[i - 1] for i in range(len(flags))]
+    print(f"[{name}] steps {len(flags)}: exact {sum(exact)} ({sum(e and g for e, g in zip(exact, g_set))} "
+          f"with G != 1), re-based {sum(rebased)} ({sum(r and g for r, g in zip(rebased, g_set))} "
+          f"with G != 1)")
     if name == "mix64_d21":
-        assert after_safe > 0
-    frac, vfrac = (0.995, 0.995) if name == "mix32_d6" else (0.98, 0.95)
+        assert sum(r and g for r, g in zip(rebased, g_set)) > 0
+    strict = name == "mix32_d6"
+
+    def bounded(a, b, mx, mean, what):
+        d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+        print(f"[{name}] {what}: max {d.max():.2e}, mean {d.mean():.2e}")
+        assert d.max() < mx and d.mean() < mean, (what, d.max(), d.mean())
+
     for stage in ("mid", "end"):
         if stage == "mid":
             (pl, cl), (pe, ce) = snaps["lazy"], snaps["exact"]
         else:
             pl, cl = _pairs(ms["lazy"]), ms["lazy"].rnm_log_consts.cpu().numpy()
             pe, ce = _pairs(ms["exact"]), ms["exact"].rnm_log_consts.cpu().numpy()
-        _close_but_flips(cl, ce, frac=frac)
+        if strict:
+            _close_but_flips(cl, ce)
+        else:
+            bounded(cl, ce, 0.05, 1e-4, f"{stage} normalisers")
         N = X.shape[0]
         kl = pl[0] * N + pl[1]
         ke = pe[0] * N + pe[1]
         common, il, ie = np.intersect1d(kl, ke, return_indices=True)
-        _close_but_flips(pl[2][il], pe[2][ie], rtol=1e-4, atol=1e-9, frac=vfrac, flip_tol=0.05)
+        if strict:
+            _close_but_flips(pl[2][il], pe[2][ie], rtol=1e-4, atol=1e-9, flip_tol=0.05)
+        else:
+            bounded(pl[2][il], pe[2][ie], 5e-3, 1e-4, f"{stage} values")
         # a float32 rounding difference can move an entry across the
         # threshold at its E-step; such entries are rare
         ndiff = kl.size + ke.size - 2 * common.size
@@ -427,13 +447,10 @@ def test_lazy_rebalance_matches_exact_mid_sweep(name, n, k):
               f"only exact {ke.size - common.size}")
     a = ms["lazy"].log_pred_density(Xd, _t(Y)).cpu().numpy()
     b = ms["exact"].log_pred_density(Xd, _t(Y)).cpu().numpy()
-    if name == "mix32_d6":
+    if strict:
         _close_but_flips(a, b, flip_tol=1e-3)
     else:
-        # held-out density within 1e-3 nats per point, 1e-4 on average
-        d = np.abs(a.astype(np.float64) - b)
-        print(f"[{name}] held-out |lazy - exact|: max {d.max():.2e}, mean {d.mean():.2e}")
-        assert d.max() < 1e-3 and d.mean() < 1e-4, (d.max(), d.mean())
+        bounded(a, b, 1e-3, 1e-4, "held-out density")
 
 
 def test_estep_skip_matches_full_evaluation():
