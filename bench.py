@@ -43,12 +43,15 @@ SECONDARY = "mix32_d6"
 # counters read for a plain copy of known size; valid for the kernel source
 # whose sha256 is recorded (the line reports null for any other source).
 MEASURED_TRAFFIC = {
-    "kernel": "estep_screen_kernel<63, 2, true> + estep_screen_combine_kernel<63>",
-    "source_sha256": "728c4f31a425063f0cd586ae4899d5cfb09ba1feb4f1565fb36bf04a302591d2",
-    "fetch_bytes": 1566687000, "write_bytes": 441350000,
-    "profile": "profiles/estep_screen_mix128_d63.md: FETCH_SIZE (740,788.4 + 24,210.6) KiB x "
-               "1.99996 + WRITE_SIZE (425,982.0 + 5,023.6) KiB x 1.0000 per E-step (copy-probe "
-               "scale factors; the keep pass, which reads 1 MB of column marks, not counted)",
+    "kernel": "estep_screen_kernel<63, 2, true> + estep_screen_exact_kernel<63> + "
+              "estep_screen_combine_kernel<63> + screen_items_kernel<63> + "
+              "screen_zero_i32_kernel<63>",
+    "source_sha256": "4bc30c2f561df9b2e8183fd88407e9a2942c9a9787dc3e418d5f269748b2632d",
+    "fetch_bytes": 1373932000, "write_bytes": 530200000,
+    "profile": "profiles/estep_screen_mix128_d63.md: FETCH_SIZE (443,778.4 + 149,992.9 + "
+               "77,101.9 + 5.5) KiB x 1.99996 + WRITE_SIZE (509,706.7 + 3,028.4 + 5,023.6 + "
+               "14.9) KiB x 1.0000 per E-step (copy-probe scale factors; the keep pass, which "
+               "reads 1 MB of column marks, not counted)",
 }
 
 
@@ -681,8 +684,9 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
                 "traffic": traffic, "traffic_note": tnote,
                 "kernel": f"estep_screen_kernel<{D},UPDATE> in its split form (the kernel ranges' "
-                          f"screens, the per-column combine and the keep pass, one event pair "
-                          f"around the three launches)", "kernel_ms": round(k_ms, 4),
+                          f"screens, the exact values of the kernels' survivor lists, the "
+                          f"per-column combine and the keep pass, one event pair around the six "
+                          f"launches)", "kernel_ms": round(k_ms, 4),
                 "launches_timed": k_n, "pairs_per_launch": pairs,
                 "achieved_executed_mfma": round(executed, 2),
                 "frac_executed_mfma": round(executed / PEAK_BF16_TFLOPS, 4),

@@ -239,26 +239,28 @@ __global__ void __launch_bounds__(256) rebal_rescale_sum_kernel(int64_t cap,
 // step's own block is skipped (its old entries are being replaced; the
 // compaction writes the new ones with the new G).  Nothing happens for a
 // step that takes the exact rebalance (bit 0), whose factors are cleared.
-__global__ void __launch_bounds__(256) rebase_scan_kernel(int64_t cap, int64_t exclude,
+__global__ void __launch_bounds__(256) rebase_scan_kernel(int64_t num_blocks, int64_t cap,
+                                                         int64_t exclude,
                                                          const int32_t *__restrict__ nnz,
                                                          const int32_t *__restrict__ cols,
                                                          float *__restrict__ vals,
                                                          const float *__restrict__ factor,
                                                          Fx *__restrict__ U2,
                                                          const int32_t *__restrict__ unsafe) {
-  if (*unsafe != 2) return;  // re-basing only
-  const int64_t b = blockIdx.y;
-  if (b == exclude) return;
-  const int32_t n = nnz[b];
-  const int32_t *cb = cols + b * cap;
-  float *vb = vals + b * cap;
-  for (int32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    const int32_t col = cb[e];
-    const float f = factor[col];
-    if (f != 0.f) {
-      const float v = vb[e] * f;
-      vb[e] = v;
-      fx_atomic_add(&U2[col], v);
+  if (*unsafe != 2) return;  // re-basing only (a fixed grid: an idle step costs one read)
+  for (int64_t b = blockIdx.x; b < num_blocks; b += gridDim.x) {
+    if (b == exclude) continue;
+    const int32_t n = nnz[b];
+    const int32_t *cb = cols + b * cap;
+    float *vb = vals + b * cap;
+    for (int32_t e = threadIdx.x; e < n; e += blockDim.x) {
+      const int32_t col = cb[e];
+      const float f = factor[col];
+      if (f != 0.f) {
+        const float v = vb[e] * f;
+        vb[e] = v;
+        fx_atomic_add(&U2[col], v);
+      }
     }
   }
 }
@@ -647,10 +649,9 @@ int rebalance_lazy_rebase(int64_t N, int64_t num_blocks, int64_t cap, int64_t bl
                           Fx *U_old, void *ws, const int32_t *unsafe, hipStream_t s) {
   Fx *A = (Fx *)((char *)ws + ((size_t)N * 4 + 15) / 16 * 16);
   float *factor = rebase_factors(N, ws);
-  int64_t gx = ceil_div(cap, 256 * 4);
-  if (gx > 64) gx = 64;
-  hipLaunchKernelGGL(rebase_scan_kernel, dim3((unsigned)gx, (unsigned)num_blocks), dim3(256), 0, s,
-                     cap, block, nnz, cols, vals, factor, A, unsafe);
+  const unsigned grid = (unsigned)(num_blocks < 1024 ? num_blocks : 1024);
+  hipLaunchKernelGGL(rebase_scan_kernel, dim3(grid), dim3(256), 0, s, num_blocks, cap, block, nnz,
+                     cols, vals, factor, A, unsafe);
   LAKDE_LAUNCHED();
   hipLaunchKernelGGL(rebase_finish_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, N,
                      factor, A, G, U, U_old, unsafe);
