@@ -6,8 +6,9 @@
 //     y~ = bf16(U_n) bf16(x_m - o) - c_n,   c_n = U_n (x_n - o) in FP32,
 // with U_n = Linv_n, o a fixed origin (the data mean) and bf16(x_m - o)
 // prepared once per fit.  When D is not a multiple of 16, -c_n enters as
-// one more MFMA column, bf16(-c_n) against a constant coordinate 1 of the
-// points; otherwise it is the accumulators' initial value.
+// up to three more MFMA columns, bf16 parts of -c_n (head and rounding
+// residues) against constant coordinates 1 of the points; otherwise it is
+// the accumulators' initial value.
 // Write v = fl(x_m - o) = (x_m - o)(1 + eps) (|eps| <= 2^-24), bf16(v) =
 // v - r and bf16(U) = U - dU, with the residuals r and dU exact in FP32.  The
 // error of y~ is -U r - dU v + dU r + U (x_m - o) eps, plus the FP32
@@ -19,8 +20,9 @@
 // b_n = |x_n - o| (all rounded up), P_n = F_n (1 + u) (U r and dU r, with
 // dF_n <= u F_n), Q_n = dF_n (1 + 2^-24) + 67 * 2^-24 F_n (dU v, the
 // accumulation and eps) and C_n = kb F_n b_n (kb = 1.2e-5 >= 65 * 2^-24
-// (1 + u)^2, the rounding of c_n), plus |c_n - bf16(c_n)|_2, computed
-// exactly when c_n is packed, in the form with the MFMA column.  The
+// (1 + u)^2, the rounding of c_n), plus the norm of what the bf16 parts
+// leave of c_n, computed exactly when c_n is packed, in the form with the
+// MFMA columns.  The
 // actual residuals are about 0.4 u of
 // their operands' norms on average, against the worst case u of the bound
 // (2u + u^2) F_n a_m that the form without them would need.  Hence the exact
@@ -69,17 +71,22 @@ constexpr double kScreenAcc = 67.0 * 0x1p-24;
 constexpr float kScreenKb = 1.2e-5f;
 constexpr int kScreenQ = 16;   // survivor queue entries per lane
 constexpr int kKeepQ = 4;      // keep-candidate queue entries per lane
+constexpr int kScreenMaxRange = 512;  // kernels per range with workgroup-aggregated list slots
 
 __host__ __device__ constexpr int sc_kp(int D) { return (D + 15) / 16 * 16; }
 __host__ __device__ constexpr int sc_dp(int D) { return (D + 31) / 32 * 32; }
 // The operands hold the coordinates in reverse order (operand index
 // j' = D - 1 - j), so that the factor U (lower triangular) becomes upper
-// triangular: row i' has nonzeros in the columns j' >= i' only.  With a
-// padding column (D < Kp) the centre term rides in the MFMA itself: the
-// points carry the constant 1 in column D and the A tiles carry bf16(-c_n)
-// there, which lies in the last K-step that every row tile uses, so the
-// accumulators start at zero and no tile is added.
-__host__ __device__ constexpr bool sc_aug(int D) { return D < sc_kp(D); }
+// triangular: row i' has nonzeros in the columns j' >= i' only.  With
+// padding columns (D < Kp) the centre term rides in the MFMA itself: the
+// points carry the constant 1 in columns D .. D + sc_naug - 1 and the A
+// tiles carry -c_n there, split into up to three bf16 parts (head, then the
+// rounding residues), all in the last K-step that every row tile uses, so
+// the accumulators start at zero and no tile is added.
+__host__ __device__ constexpr int sc_naug(int D) {
+  return sc_kp(D) - D < 3 ? sc_kp(D) - D : 3;
+}
+__host__ __device__ constexpr bool sc_aug(int D) { return sc_naug(D) > 0; }
 // MFMA A tiles of U (32 rows x 16 columns) that are not all zero: K-step s
 // of row tile rt holds nonzeros iff 16 s + 15 >= 32 rt; stored K-step-major
 // in that order
@@ -165,7 +172,7 @@ __global__ void __launch_bounds__(256) screen_points_kernel(const float *__restr
   double s = 0.0, rs = 0.0;
 #pragma unroll
   for (int j = 0; j < Kp; ++j) {
-    float v = j == D ? 1.f : 0.f;  // the constant coordinate is exact
+    float v = (j >= D && j < D + sc_naug(D)) ? 1.f : 0.f;  // the constant coordinates are exact
     if (j < D) {
       v = X[m * D + (D - 1 - j)] - origin[D - 1 - j];
       s += (double)v * (double)v;
@@ -209,9 +216,10 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
     }
     const float dl = r[rec_x(D) + lane] - origin[lane];
     b2 = (double)dl * (double)dl;
-    // the rounding of -c_n to bf16 in the padding column
-    const double ec = (double)(-c) - (double)(float)(__bf16)(-c);
-    e2 = sc_aug(D) ? ec * ec : 0.0;
+    // what the bf16 parts of -c_n in the padding columns leave out
+    float rest = -c;
+    for (int k = 0; k < sc_naug(D); ++k) rest -= (float)(__bf16)rest;  // exact subtractions
+    e2 = sc_aug(D) ? (double)rest * (double)rest : 0.0;
   }
   // operand row i' is row D - 1 - i' of U
   for (int s = 0; s < Kp / 16; ++s)
@@ -224,7 +232,12 @@ __global__ void __launch_bounds__(64) pack_screen_kernel(const float *__restrict
         const int jp = s * 16 + 8 * (lane >> 5) + e, j = D - 1 - jp;
         float v = 0.f;
         if (ip < D && jp < D && jp >= ip) v = r[i * (i + 1) / 2 + j];
-        if (sc_aug(D) && ip < D && jp == D) v = -ci;
+        if (ip < D && jp >= D && jp < D + sc_naug(D)) {
+          // part jp - D of -c_i: the bf16 head of what the earlier parts leave
+          float rest = -ci;
+          for (int k = 0; k < jp - D; ++k) rest -= (float)(__bf16)rest;
+          v = rest;
+        }
         tile[e] = (__bf16)v;
       }
     }
@@ -384,6 +397,11 @@ estep_screen_kernel(EParams p, SParams sp) {
   __shared__ __attribute__((aligned(1024))) uint8_t ring1[L::kBytes];
   __shared__ int16_t sq[kScreenQ][256];
   ScreenWaveList *wl = reinterpret_cast<ScreenWaveList *>(ring0);
+  // the split form's list-slot aggregation (per-kernel counters and bases,
+  // the lanes' local slots): in the stage arrays when they are large enough
+  constexpr int kAggCnt = 2 * kScreenMaxRange * 4, kAggSlots = kScreenQ * 256 * 2;
+  constexpr bool kAggRing = kRing - L::kBytes >= kAggCnt && L::kBytes >= kAggSlots;
+  __shared__ __attribute__((aligned(16))) uint8_t agg_own[(SPLIT && !kAggRing) ? kAggCnt + kAggSlots : 16];
   __shared__ int16_t kq_n[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
   __shared__ float kq_x[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
 
@@ -543,21 +561,37 @@ estep_screen_kernel(EParams p, SParams sp) {
           sp.part_s[o] = 0.f;
           sp.kcount[o] = 0;
           sp.sv_cnt[o] = (uint8_t)cnt;
-          // four list slots claimed at a time (their atomics in flight together)
-          for (int k0 = 0; k0 < cnt; k0 += 4) {
-            int32_t slot[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-              if (k0 + i < cnt) slot[i] = atomicAdd(&sp.kl_cnt[sq[k0 + i][tid]], 1);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-              if (k0 + i < cnt) {
-                const int32_t n = sq[k0 + i][tid];
-                const bool in = slot[i] < sp.kl_cap;
-                if (in) sp.kl_m[(int64_t)n * sp.kl_cap + slot[i]] = (int32_t)m;
-                sp.sv_e[(m * sp.nsplit + blockIdx.y) * kScreenQ + k0 + i] =
-                    ((in ? slot[i] + 1 : 0) << 15) | n;
-              }
+        }
+        // list slots: within the workgroup from per-kernel LDS counters,
+        // then one global atomic per kernel of the range reserves them (the
+        // stage arrays, free now, hold the counters, bases and local slots
+        // when they are large enough)
+        const int kb = (int)kb_, len = (int)(ke_ - kb_);
+        int32_t *lcnt = reinterpret_cast<int32_t *>(kAggRing ? ring0 : agg_own);
+        int32_t *gbase = lcnt + kScreenMaxRange;
+        int16_t(*lslot)[256] =
+            reinterpret_cast<int16_t(*)[256]>(kAggRing ? ring1 : agg_own + kAggCnt);
+        auto put = [&](int k, int32_t n, int32_t slot) {
+          const bool in = slot < sp.kl_cap;
+          if (in) sp.kl_m[(int64_t)n * sp.kl_cap + slot] = (int32_t)m;
+          sp.sv_e[(m * sp.nsplit + blockIdx.y) * kScreenQ + k] = ((in ? slot + 1 : 0) << 15) | n;
+        };
+        if (len <= kScreenMaxRange) {
+          for (int t = tid; t < len; t += 256) lcnt[t] = 0;
+          __syncthreads();
+          for (int k = 0; k < cnt; ++k) lslot[k][tid] = (int16_t)atomicAdd(&lcnt[sq[k][tid] - kb], 1);
+          __syncthreads();
+          for (int t = tid; t < len; t += 256)
+            if (lcnt[t] > 0) gbase[t] = atomicAdd(&sp.kl_cnt[kb + t], lcnt[t]);
+          __syncthreads();
+          for (int k = 0; k < cnt; ++k) {
+            const int32_t n = sq[k][tid];
+            put(k, n, gbase[n - kb] + lslot[k][tid]);
+          }
+        } else {
+          for (int k = 0; k < cnt; ++k) {
+            const int32_t n = sq[k][tid];
+            put(k, n, atomicAdd(&sp.kl_cnt[n], 1));
           }
         }
         return -1;
