@@ -46,10 +46,10 @@ MEASURED_TRAFFIC = {
     "kernel": "estep_screen_kernel<63, 2, true> + estep_screen_exact_kernel<63> + "
               "estep_screen_combine_kernel<63> + screen_items_kernel<63> + "
               "screen_zero_i32_kernel<63>",
-    "source_sha256": "4bc30c2f561df9b2e8183fd88407e9a2942c9a9787dc3e418d5f269748b2632d",
-    "fetch_bytes": 1373932000, "write_bytes": 530200000,
-    "profile": "profiles/estep_screen_mix128_d63.md: FETCH_SIZE (443,778.4 + 149,992.9 + "
-               "77,101.9 + 5.5) KiB x 1.99996 + WRITE_SIZE (509,706.7 + 3,028.4 + 5,023.6 + "
+    "source_sha256": "2f705943411bd73a329a2e8627012680597771080260868973decbee9af5439d",
+    "fetch_bytes": 1404500000, "write_bytes": 530395000,
+    "profile": "profiles/estep_screen_mix128_d63.md: FETCH_SIZE (458,912.4 + 149,992.5 + "
+               "76,894.9 + 5.0) KiB x 1.99996 + WRITE_SIZE (509,897.3 + 3,028.4 + 5,023.6 + "
                "14.9) KiB x 1.0000 per E-step (copy-probe scale factors; the keep pass, which "
                "reads 1 MB of column marks, not counted)",
 }

@@ -72,6 +72,12 @@ constexpr float kScreenKb = 1.2e-5f;
 constexpr int kScreenQ = 16;   // survivor queue entries per lane
 constexpr int kKeepQ = 4;      // keep-candidate queue entries per lane
 constexpr int kScreenMaxRange = 512;  // kernels per range with workgroup-aggregated list slots
+// List slots per workgroup (LDS counters, one global atomic per kernel of the
+// range) below D = 48: many survivors per kernel (mix64_d21: one contended
+// global atomic per pair took 1.54 s per VI iteration against 0.44); from 48
+// on the pairs survive sparsely and the aggregation's registers cost more
+// than its atomics save (mix128_d63: 3.07 against 2.96 ms per E-step).
+__host__ __device__ constexpr bool sc_agg_slots(int D) { return D < 48; }
 
 __host__ __device__ constexpr int sc_kp(int D) { return (D + 15) / 16 * 16; }
 __host__ __device__ constexpr int sc_dp(int D) { return (D + 31) / 32 * 32; }
@@ -401,7 +407,8 @@ estep_screen_kernel(EParams p, SParams sp) {
   // the lanes' local slots): in the stage arrays when they are large enough
   constexpr int kAggCnt = 2 * kScreenMaxRange * 4, kAggSlots = kScreenQ * 256 * 2;
   constexpr bool kAggRing = kRing - L::kBytes >= kAggCnt && L::kBytes >= kAggSlots;
-  __shared__ __attribute__((aligned(16))) uint8_t agg_own[(SPLIT && !kAggRing) ? kAggCnt + kAggSlots : 16];
+  __shared__ __attribute__((aligned(16)))
+  uint8_t agg_own[(SPLIT && sc_agg_slots(D) && !kAggRing) ? kAggCnt + kAggSlots : 16];
   __shared__ int16_t kq_n[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
   __shared__ float kq_x[MODE == LAKDE_ESTEP_UPDATE ? kKeepQ : 1][256];
 
@@ -576,7 +583,7 @@ estep_screen_kernel(EParams p, SParams sp) {
           if (in) sp.kl_m[(int64_t)n * sp.kl_cap + slot] = (int32_t)m;
           sp.sv_e[(m * sp.nsplit + blockIdx.y) * kScreenQ + k] = ((in ? slot + 1 : 0) << 15) | n;
         };
-        if (len <= kScreenMaxRange) {
+        if (sc_agg_slots(D) && len <= kScreenMaxRange) {
           for (int t = tid; t < len; t += 256) lcnt[t] = 0;
           __syncthreads();
           for (int k = 0; k < cnt; ++k) lslot[k][tid] = (int16_t)atomicAdd(&lcnt[sq[k][tid] - kb], 1);
@@ -589,9 +596,15 @@ estep_screen_kernel(EParams p, SParams sp) {
             put(k, n, gbase[n - kb] + lslot[k][tid]);
           }
         } else {
-          for (int k = 0; k < cnt; ++k) {
-            const int32_t n = sq[k][tid];
-            put(k, n, atomicAdd(&sp.kl_cnt[n], 1));
+          // four list slots claimed at a time (their atomics in flight together)
+          for (int k0 = 0; k0 < cnt; k0 += 4) {
+            int32_t slot[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (k0 + i < cnt) slot[i] = atomicAdd(&sp.kl_cnt[sq[k0 + i][tid]], 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (k0 + i < cnt) put(k0 + i, sq[k0 + i][tid], slot[i]);
           }
         }
         return -1;
