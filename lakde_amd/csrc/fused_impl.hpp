@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(256) estep_compact_kernel(EParams p) {
 
 // Values of the compacted entries for large D: a wave per row n (the block's
 // kernel), lanes = the row's entries, so the kernel's record is read once
-// per wave with scalar loads instead of once per lane (the compaction's
+// per wave (staged in LDS) instead of once per lane (the compaction's
 // lanes are 64 different kernels).  Same device functions as the fused
 // compaction: identical values, conversion and column sums.
 template <int D>
@@ -502,12 +502,30 @@ __global__ void __launch_bounds__(256) estep_values_kernel(const float *__restri
   const int64_t nnz = *p.nnz_dev;
   if (nnz > p.cap) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t n = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t n = (int64_t)blockIdx.x * 4 + w;
   if (n >= p.nb) return;
-  constexpr int64_t R = rec_floats(D);
-  const float *rec = recs + n * R;
+  constexpr int64_t R = rec_floats(D);  // a multiple of 4
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  // the wave's record staged in LDS with all its loads in flight, read back
+  // as broadcast loads (a chain of dependent scalar loads otherwise)
+  __shared__ f32x4 srec[4][R / 4];
   const bool to_stored = p.lazy_Rnew != nullptr && (*p.lazy_unsafe & 1) == 0;
   const int64_t beg = p.rowptr[n], end = p.rowptr[n + 1];
+  if (beg >= end) return;
+  const f32x4 *g = reinterpret_cast<const f32x4 *>(recs + n * R);
+  constexpr int kPer = (int)((R / 4 + kWave - 1) / kWave);
+  f32x4 st[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (lane + k * kWave < (int)(R / 4)) st[k] = g[lane + k * kWave];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (lane + k * kWave < (int)(R / 4)) srec[w][lane + k * kWave] = st[k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const float *rec = reinterpret_cast<const float *>(srec[w]);
   for (int64_t e0 = beg; e0 < end; e0 += kWave) {
     const int64_t e = e0 + lane;
     if (e < end) {
